@@ -1,0 +1,273 @@
+/*
+ * xmcml.h -- C-ABI of the MI355X photon random-walk engine.
+ *
+ * The structures below are plain-old-data mirrors of xmcml's InputInfo /
+ * OutputInfo family (base/xmcml/xmcml/mcml_kernel_types.h, mcml_types.h) so
+ * that a caller which fills InputInfo today can fill xmc_input field for field.
+ * Every pointer is owned by the caller; the library copies what it needs to the
+ * GPU.  No C++ or HIP types appear in any signature.
+ *
+ * The entry points replace the photon random-walk seam of xmcml:
+ *   GenerateBVHTree()  (base/xmcml/xmcml/mcml_intersection.h:78,
+ *                       called from base/xmcml/xmcmlLauncher/launcher_mpi.cpp:396)
+ *       -> xmc_scene_create()   builds the same SAH BVH on the host, flattens it
+ *                               and uploads geometry, layers and detectors.
+ *   LaunchOMP()        (base/xmcml/xmcmlLauncher/launcher_omp.h:10,
+ *                       called from launcher_mpi.cpp:369,374)
+ *       -> xmc_launch()         runs numberOfPhotons photons on the GPU and adds
+ *                               the tallies into the caller's xmc_output.
+ *   InitOutput()/FreeOutput() (launcher_omp.h:8-9)
+ *       -> xmc_init_output()/xmc_free_output().
+ *   ComputeSpecularReflectance() (base/xmcml/xmcml/mcml_kernel.h:10)
+ *       -> xmc_specular_reflectance().
+ *
+ * Differences from the reference seam, all deliberate:
+ *   - Functions return an int status (XMC_OK or a negative code) instead of
+ *     void; xmc_last_error() gives the message.  LaunchOMP reports nothing.
+ *   - Random streams are per photon, not per thread: photon p of a run with
+ *     seed s draws the MCG59 sequence that starts at (2s+1) * a^(p * 2^24)
+ *     mod 2^59 (a = 13^13).  The result therefore does not depend on how
+ *     photons are scheduled.  xmc_rng carries (seed, next photon) where
+ *     LaunchOMP carries MCG59[numThreads]; both are advanced in place.
+ *   - Termination can follow xmcml (die when weight < minWeight,
+ *     mcml_kernel.cpp:121-124) or MCML Russian roulette
+ *     (stable/plane_parallel/1.0/McmlGpu/kernel/kernel.cu:214-222).
+ *   - The scattering grid and the per-detector voxel trajectory maps
+ *     (mcml_kernel.cpp:750-810) are not produced by the GPU path; their output
+ *     arrays are left untouched.
+ */
+#ifndef XMCML_AMD_XMCML_H
+#define XMCML_AMD_XMCML_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XMC_MAX_LAYERS 8 /* MAX_LAYERS, mcml_kernel_types.h:7 */
+
+/* Status codes. */
+#define XMC_OK 0
+#define XMC_ERR_ARG (-1)     /* invalid argument / input */
+#define XMC_ERR_HIP (-2)     /* HIP runtime failure */
+#define XMC_ERR_LIMIT (-3)   /* input exceeds a documented limit */
+#define XMC_ERR_NOGPU (-4)   /* no usable GPU */
+
+/* double3 / int3, mcml_types.h:8-24 */
+typedef struct xmc_double3 { double x, y, z; } xmc_double3;
+typedef struct xmc_int3 { int x, y, z; } xmc_int3;
+
+/* Area, mcml_types.h:26-31: voxel grid of the absorption tally.  A photon
+ * whose position leaves the Area dies (mcml_kernel.cpp:63-71,90-117). */
+typedef struct xmc_area {
+    xmc_double3 corner;
+    xmc_double3 length;
+    xmc_int3 partitionNumber;
+} xmc_area;
+
+/* Surface, mcml_types.h:33-39: a triangle mesh. */
+typedef struct xmc_surface {
+    const xmc_double3* vertices;
+    int numberOfVertices;
+    const xmc_int3* triangles;
+    int numberOfTriangles;
+} xmc_surface;
+
+/* LayerInfo, mcml_kernel_types.h:54-62.  Layer 0 is the ambient medium; a
+ * layer with absorption == scattering == 0 is "glass". */
+typedef struct xmc_layer {
+    double refractiveIndex;
+    double absorptionCoefficient;
+    double scatteringCoefficient;
+    double anisotropy;
+    const int* surfaceId;
+    int numberOfSurfaces;
+} xmc_layer;
+
+/* CubeDetector / RingDetector, mcml_kernel_types.h:19-34 (bool -> uint8_t). */
+typedef struct xmc_cube_detector {
+    xmc_double3 center;
+    xmc_double3 length;
+    uint8_t filterLayers[XMC_MAX_LAYERS];
+    double targetAngle; /* degrees */
+} xmc_cube_detector;
+
+typedef struct xmc_ring_detector {
+    xmc_double3 center;
+    double smallRadius;
+    double bigRadius;
+    uint8_t filterLayers[XMC_MAX_LAYERS];
+    double targetAngle; /* degrees */
+} xmc_ring_detector;
+
+#define XMC_TERMINATE_KILL 0     /* die when weight < minWeight (xmcml) */
+#define XMC_TERMINATE_ROULETTE 1 /* MCML roulette: below minWeight survive with
+                                    probability rouletteChance, weight /= chance */
+
+/* InputInfo, mcml_kernel_types.h:64-91, minus the fields of the biasing
+ * extension (useBiasing..targetPoint), which the GPU path does not implement. */
+typedef struct xmc_input {
+    uint64_t numberOfPhotons; /* photons for this call (launch) */
+    const xmc_area* area;
+    const xmc_layer* layerInfo;
+    int numberOfLayers;
+    const xmc_surface* surface;
+    int numberOfSurfaces;
+    const xmc_cube_detector* cubeDetector;
+    int numberOfCubeDetectors;
+    const xmc_ring_detector* ringDetector;
+    int numberOfRingDetectors;
+    double minWeight;
+    double timeStart;  /* time unit: optical path n*s in mm (mcml_kernel.cpp:161) */
+    double timeFinish;
+    int timeScaleSize;
+    xmc_double3 startPosition;
+    xmc_double3 startDirection; /* used as given, not normalised (mcml_kernel.cpp:142-144) */
+    uint8_t targetRangeLayers[XMC_MAX_LAYERS];
+    int terminationMode; /* XMC_TERMINATE_* */
+    double rouletteChance;
+} xmc_input;
+
+/* TimeInfo, mcml_kernel_types.h:10-17. */
+typedef struct xmc_time_info {
+    double timeStart;
+    double timeFinish;
+    uint64_t numberOfPhotons;
+    double weight;
+    double targetWeight;
+} xmc_time_info;
+
+/* DetectorInfo, mcml_kernel_types.h:36-45. */
+typedef struct xmc_detector_info {
+    uint64_t numberOfPhotons;
+    uint64_t* trajectory; /* may be NULL; not written by the GPU path */
+    int trajectorySize;
+    xmc_time_info* timeScale;
+    int timeScaleSize;
+    double targetWeight;
+    double weight;
+} xmc_detector_info;
+
+/* OutputInfo, mcml_kernel_types.h:93-102.  Detector index order: cube
+ * detectors first, then ring detectors (mcml_kernel.cpp:663-742). */
+typedef struct xmc_output {
+    uint64_t numberOfPhotons;
+    double specularReflectance;
+    double* absorption; /* gridSize doubles; may be NULL (tally skipped) */
+    double* scattering; /* gridSize doubles; not written by the GPU path */
+    int gridSize;
+    xmc_detector_info* detectorInfo;
+    int numberOfDetectors;
+} xmc_output;
+
+/* Random stream state of a run (replaces MCG59[numThreads] of LaunchOMP). */
+typedef struct xmc_rng {
+    uint64_t seed;
+    uint64_t nextPhoton;
+} xmc_rng;
+
+/* Event counters of the last launch on a scene. */
+typedef struct xmc_stats {
+    uint64_t photons;
+    uint64_t scatterEvents;  /* hop + drop + spin */
+    uint64_t boundaryEvents; /* Fresnel reflect / refract / escape */
+    uint64_t rouletteEvents;
+    uint64_t stackRestarts;  /* BVH short-stack overflows (traversal restarted) */
+    double kernelMs;         /* device time of the transport kernel (HIP events) */
+} xmc_stats;
+
+/* One recorded step of a traced photon (debug / parity).  Positions and
+ * weights are stored as double even when the transport runs in float. */
+#define XMC_EV_LAUNCH 0
+#define XMC_EV_SCATTER 1
+#define XMC_EV_REFLECT 2
+#define XMC_EV_TRANSMIT 3
+#define XMC_EV_ESCAPE 4       /* transmitted into layer 0; aux = detector id or -1 */
+#define XMC_EV_DIE_AREA 5     /* left the Area */
+#define XMC_EV_DIE_WEIGHT 6   /* weight < minWeight, terminationMode KILL */
+#define XMC_EV_ROULETTE_LIVE 7
+#define XMC_EV_ROULETTE_DIE 8
+#define XMC_EV_GLASS_MISS 9   /* no surface ahead in a glass layer */
+typedef struct xmc_trace_event {
+    int32_t type;
+    int32_t layer;
+    int32_t aux;
+    uint32_t draws; /* uniforms drawn by this photon so far */
+    double pos[3];
+    double dir[3];
+    double weight;
+    double time;
+} xmc_trace_event;
+
+/* Launch options (all zero = defaults). */
+typedef struct xmc_options {
+    int blockSize;        /* threads per workgroup; 0 = autotuned/default */
+    int blocksPerCU;      /* persistent workgroups per CU; 0 = default */
+    int stackDepth;       /* LDS short-stack entries per lane; 0 = default */
+    int absorptionTile;   /* 1 = privatise the hottest absorption voxels in LDS */
+    int device;           /* HIP device ordinal */
+} xmc_options;
+
+typedef struct xmc_scene xmc_scene;
+
+/* --- Scene: geometry + optics uploaded once (GenerateBVHTree + copy). --- */
+int xmc_scene_create(const xmc_input* input, const xmc_options* options, xmc_scene** scene);
+int xmc_scene_destroy(xmc_scene* scene);
+
+/* Runs photons [firstPhoton, firstPhoton + numberOfPhotons) of stream `seed`
+ * asynchronously on `hipStream` (NULL = the scene's stream), accumulating
+ * into the scene's device tallies.  Inputs stay resident in HBM. */
+int xmc_scene_run(xmc_scene* scene, uint64_t seed, uint64_t firstPhoton,
+                  uint64_t numberOfPhotons, void* hipStream);
+
+/* Waits for the scene's work, adds the device tallies into `output`
+ * (+=, like LaunchOMP), sets output->specularReflectance and zeroes the
+ * device tallies.  output->numberOfPhotons is not touched. */
+int xmc_scene_collect(xmc_scene* scene, xmc_output* output);
+
+/* Counters of the work since the last collect/reset. */
+int xmc_scene_stats(xmc_scene* scene, xmc_stats* stats);
+
+/* Traced run (debug / parity): photons [firstPhoton, firstPhoton+n) run with
+ * tracing; photon firstPhoton+i writes at most maxEvents events at
+ * events[i*maxEvents ...] and its event count in counts[i].  Host buffers. */
+int xmc_scene_trace(xmc_scene* scene, uint64_t seed, uint64_t firstPhoton, uint64_t numberOfPhotons,
+                    xmc_trace_event* events, int maxEvents, int* counts);
+
+/* Nearest-hit ray queries against the scene's flattened BVH, on the GPU.
+ * rays: origin[3], direction[3] as double (converted to float like the
+ * transport does).  step[i] <= 0 means unlimited.  Outputs: found (0/1),
+ * distance, surface id, triangle id (in the surface), unit normal[3].
+ * bruteForce = 1 tests every triangle instead of walking the BVH. */
+int xmc_scene_query(xmc_scene* scene, const double* origins, const double* directions,
+                    const double* steps, int numberOfRays, int bruteForce,
+                    int* found, double* distance, int* surfaceId, int* triangleId,
+                    double* normal);
+
+/* BVH built by the scene, in the reference's layout (for parity tests):
+ * node arrays in the order GenerateBVHTree emits them (post-order, root
+ * last).  Pass NULL arrays to query the sizes. */
+int xmc_scene_bvh(xmc_scene* scene, int* numberOfNodes, int* root, int* numberOfTriangles,
+                  double* boxes /* 6 per node */, int* links /* left,right,offset,count */,
+                  int* triangleSurface, int* triangleIndex);
+
+/* --- The LaunchOMP replacement. --- */
+/* Builds a scene, runs input->numberOfPhotons photons starting at
+ * rng->nextPhoton, adds tallies into output, advances rng->nextPhoton. */
+int xmc_launch(const xmc_input* input, xmc_output* output, xmc_rng* rng,
+               const xmc_options* options);
+
+/* InitOutput / FreeOutput equivalents (malloc-based). */
+int xmc_init_output(const xmc_input* input, xmc_output* output);
+void xmc_free_output(xmc_output* output);
+
+double xmc_specular_reflectance(const xmc_layer* layers);
+const char* xmc_last_error(void);
+const char* xmc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XMCML_AMD_XMCML_H */

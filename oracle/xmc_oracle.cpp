@@ -1,0 +1,1426 @@
+/*
+ * xmc_oracle.cpp -- CPU oracle for the photon random-walk path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates the reference's CPU algorithm
+ * so that tests can check the GPU engine against it and bench.py can time it
+ * as the CPU baseline.  The product library never links it.
+ *
+ * What each part follows (paths relative to the reference repository):
+ *   MCG59 generator ............ base/xmcml/xmcml/mcml_mcg59.cpp:3-41
+ *   vector math ................ base/xmcml/xmcml/mcml_math.cpp:6-94
+ *   slab box test .............. base/xmcml/xmcml/mcml_intersection.cpp:11-42
+ *   Moller-Trumbore ............ mcml_intersection.cpp:44-86
+ *   SAH BVH build .............. mcml_intersection.cpp:88-344
+ *   BVH walk (step-limited) .... mcml_intersection.cpp:354-466
+ *   BVH walk (unlimited) ....... mcml_intersection.cpp:468-584
+ *   photon loop ................ base/xmcml/xmcml/mcml_kernel.cpp:39-126
+ *   launch, move, area index ... mcml_kernel.cpp:128-227
+ *   boundary / Fresnel ......... mcml_kernel.cpp:229-284, 497-616, 644-661
+ *   spin, step, drop ........... mcml_kernel.cpp:442-495, 618-642
+ *   detectors, TOF, ranges ..... mcml_kernel.cpp:663-858
+ *   scattering grid/trajectory . mcml_kernel.cpp:750-810
+ *   output init, thread grids .. base/xmcml/xmcmlLauncher/launcher_omp.cpp:10-163
+ *   roulette ................... stable/plane_parallel/1.0/McmlGpu/kernel/kernel.cu:214-222,
+ *                                called as in kernel.cu:449-452
+ *
+ * Parity pins: the 64-bit mode is compared bit for bit with the reference
+ * sources compiled under oracle/_ref (same streams, same libm), and with the
+ * golden values of stable/1.2/xmcml/xmcml_tests (tests/golden).  The 32-bit
+ * mode is the same template instantiated in float; its float-only rules are
+ * listed at struct Prec<float> and its special functions are checked against
+ * libm over every input the engine can produce.
+ */
+#include "xmc_oracle.h"
+
+#include <omp.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace {
+
+// ----------------------------------------------------------------------------
+// MCG59 (mcml_mcg59.cpp:3-41)
+// ----------------------------------------------------------------------------
+constexpr uint64_t kMcgC = 302875106592253ULL;      // 13^13
+constexpr uint64_t kMcgM = 576460752303423488ULL;   // 2^59
+constexpr uint64_t kMcgMask = 576460752303423487ULL;
+constexpr int kStreamLog2 = 24;                     // draws reserved per photon
+
+uint64_t mcg_pow(uint64_t a, uint64_t e) {
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r *= a;
+        a *= a;
+        e >>= 1;
+    }
+    return r;
+}
+
+struct Mcg59 {
+    uint64_t value;
+    uint64_t offset;
+    uint32_t draws;
+    uint64_t next() {
+        value = (value * offset) & kMcgMask;
+        ++draws;
+        return value;
+    }
+};
+
+uint64_t photon_stream(uint64_t seed, uint64_t photon) {
+    const uint64_t jump = mcg_pow(kMcgC, 1ULL << kStreamLog2);
+    return ((2 * seed + 1) * mcg_pow(jump, photon)) & kMcgMask;
+}
+
+// ----------------------------------------------------------------------------
+// float special functions of the 32-bit mode.
+// log: x = 2^k (1+f), 1+f in [sqrt(2)/2, sqrt(2)); log(1+f) = 2 atanh(s),
+// s = f/(2+f), written as f - (f^2/2 - s (f^2/2 + R)), R = series of
+// 2 atanh(s)/s - 2 truncated after s^8 (Abramowitz & Stegun 4.1.27).
+// sincos(2 pi u): q = rint(4u), r = 4u - q (exact), theta = r pi/2 in
+// [-pi/4, pi/4], Taylor series to theta^9 / theta^10, quadrant rotation by q.
+// ----------------------------------------------------------------------------
+inline float f_logf(float x) {
+    uint32_t ix;
+    std::memcpy(&ix, &x, 4);
+    int32_t t = (int32_t)(ix - 0x3f3504f3u);
+    int32_t k = t >> 23;
+    uint32_t im = ix - ((uint32_t)k << 23);
+    float m;
+    std::memcpy(&m, &im, 4);
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float z = s * s;
+    float R = std::fmaf(z, 2.0f / 9.0f, 2.0f / 7.0f);
+    R = std::fmaf(z, R, 2.0f / 5.0f);
+    R = std::fmaf(z, R, 2.0f / 3.0f);
+    R = z * R;
+    float hfsq = 0.5f * (f * f);
+    float l1p = f - (hfsq - s * (hfsq + R));
+    float kf = (float)k;
+    return std::fmaf(kf, 0.693145751953125f, std::fmaf(kf, 1.42860677e-06f, l1p));
+}
+
+inline void f_sincos2pi(float u, float* sp, float* cp) {
+    float u4 = 4.0f * u;
+    float q = std::rint(u4);
+    float r = u4 - q;
+    int iq = ((int)q) & 3;
+    float th = r * 1.57079637f;
+    float t2 = th * th;
+    float ps = std::fmaf(t2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+    ps = std::fmaf(t2, ps, 1.0f / 120.0f);
+    ps = std::fmaf(t2, ps, -1.0f / 6.0f);
+    float s = std::fmaf(th * t2, ps, th);
+    float pc = std::fmaf(t2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
+    pc = std::fmaf(t2, pc, -1.0f / 720.0f);
+    pc = std::fmaf(t2, pc, 1.0f / 24.0f);
+    pc = std::fmaf(t2, pc, -0.5f);
+    float c = std::fmaf(t2, pc, 1.0f);
+    switch (iq) {
+        case 0: *sp = s; *cp = c; break;
+        case 1: *sp = c; *cp = -s; break;
+        case 2: *sp = -s; *cp = -c; break;
+        default: *sp = -c; *cp = s; break;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// precision policies
+// ----------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846;
+
+template <class R> struct Prec;
+
+template <> struct Prec<double> {
+    static constexpr double kCosZero = 1.0 - 1.0e-12;  // mcml_kernel.cpp:11
+    static constexpr double kCos90 = 1.0e-12;           // :12
+    static constexpr double kMinDistance = 1.0e-8;      // :13
+    static constexpr double kMaxDistance = 1.0e256;     // mcml_intersection.h:7
+    static constexpr double kEps = 1.0e-6;              // mcml_intersection.h:6
+    static double uniform(uint64_t x) { return (double)x / (double)kMcgM; }  // mcml_mcg59.cpp:40
+    static double log_(double u) { return log(u); }
+    static void sincos_2pi(double u, double* s, double* c) {
+        double psi = 2.0 * kPi * u;  // mcml_kernel.cpp:453-455
+        *c = cos(psi);
+        *s = sin(psi);
+    }
+    static double one_minus_sq(double x) { return 1.0 - x * x; }
+};
+
+// Float-only rules (mirrored by the GPU kernel):
+//  - uniforms carry the top 24 bits of the MCG59 state: (x >> 35) * 2^-24;
+//  - log and sincos(2 pi u) are f_logf / f_sincos2pi above;
+//  - 1 - x^2 is evaluated as (1 - x)(1 + x), and the "cosine of zero"
+//    threshold is 1 - 1e-6 (1 - 1e-12 rounds to 1.0f, which would send
+//    normal-incidence directions through a 0/0 in the spin);
+//  - the 1e-8 mm nudges of the reference (launch, the 1e-10 mm stop short of
+//    a surface, and the step after every reflection or refraction) are below
+//    the float spacing of coordinates, so they are replaced by an offset
+//    scaled to the coordinates, delta = max(|x|,|y|,|z|) * 2^-20 + 2^-22 mm,
+//    applied along the unit surface normal: the photon moves to the hit
+//    point h, is placed at h - delta*n on the side it came from for the Area
+//    test and the detectors, and after a reflection or refraction at
+//    h + delta*n on the side its new direction points to (at launch: +delta
+//    along z).  It therefore starts every ray strictly on the correct side of
+//    the surface it stands on (and of coplanar neighbours sharing an edge).
+template <> struct Prec<float> {
+    static constexpr float kCosZero = 1.0f - 1.0e-6f;
+    static constexpr float kCos90 = 1.0e-12f;
+    static constexpr float kMinDistance = 1.0e-8f;
+    static constexpr float kMaxDistance = std::numeric_limits<float>::infinity();
+    static constexpr float kEps = 1.0e-6f;
+    static float uniform(uint64_t x) { return (float)(x >> 35) * 0x1p-24f; }
+    static float log_(float u) { return f_logf(u); }
+    static void sincos_2pi(float u, float* s, float* c) { f_sincos2pi(u, s, c); }
+    static float one_minus_sq(float x) { return (1.0f - x) * (1.0f + x); }
+};
+
+// ----------------------------------------------------------------------------
+// vectors (mcml_math.cpp)
+// ----------------------------------------------------------------------------
+template <class R> struct V3 {
+    R x, y, z;
+    R& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    const R& operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+template <class R> inline V3<R> sub(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> divv(V3<R> a, R b) { return {a.x / b, a.y / b, a.z / b}; }
+template <class R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class R> inline R length(V3<R> a) { return std::sqrt(dot(a, a)); }
+template <class R> inline V3<R> normalize(V3<R> a) { return divv(a, length(a)); }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> inline V3<R> plane_normal(V3<R> a, V3<R> b, V3<R> c) {
+    return cross(sub(a, b), sub(a, c));
+}
+// GetPlaneSegmentIntersectionPoint, mcml_math.cpp:59-94
+inline V3<double> plane_segment_point(V3<double> a, V3<double> b, double z) {
+    const double kMax = 1.0e256;
+    V3<double> n = {0.0, 0.0, 1.0};
+    V3<double> v = {-a.x, -a.y, z - a.z};
+    double d = dot(n, v);
+    V3<double> w = sub(b, a);
+    double e = dot(n, w);
+    V3<double> r;
+    if (e != 0.0) {
+        r.x = a.x + w.x * (d / e);
+        r.y = a.y + w.y * (d / e);
+        r.z = a.z + w.z * (d / e);
+        if (dot(sub(a, r), sub(b, r)) > 0.0) r = {kMax, kMax, kMax};
+    } else {
+        r = {kMax, kMax, kMax};
+    }
+    return r;
+}
+
+template <class R> V3<R> cvt(const xmc_double3& v) { return {(R)v.x, (R)v.y, (R)v.z}; }
+
+// ----------------------------------------------------------------------------
+// geometry: triangle test, box test, BVH
+// ----------------------------------------------------------------------------
+template <class R>
+R triangle_distance(V3<R> o, V3<R> d, V3<R> a, V3<R> b, V3<R> c) {  // :44-86
+    const R eps = Prec<R>::kEps;
+    V3<R> e1 = sub(b, a);
+    V3<R> e2 = sub(c, a);
+    V3<R> p = cross(d, e2);
+    R det = dot(e1, p);
+    if (det < eps && det > -eps) return Prec<R>::kMaxDistance;
+    R inv = (R)1.0 / det;
+    V3<R> t = sub(o, a);
+    R u = dot(t, p) * inv;
+    if (u < (R)0.0 || u > (R)1.0) return Prec<R>::kMaxDistance;
+    V3<R> q = cross(t, e1);
+    R v = dot(d, q) * inv;
+    if (v < (R)0.0 || u + v > (R)1.0) return Prec<R>::kMaxDistance;
+    return dot(e2, q) * inv;
+}
+
+template <class R> struct Box { V3<R> lo, hi; };
+
+template <class R>
+bool box_hit(const Box<R>& b, const V3<R>& o, const V3<R>& d, R& tnear, R& tfar) {  // :11-42
+    const R kMax = Prec<R>::kMaxDistance;
+    R tmin = -kMax, tmax = kMax;
+    for (int i = 0; i < 3; i++) {
+        if (std::fabs(d[i]) > Prec<R>::kEps) {
+            R t = (R)1.0 / d[i];
+            R t1 = (b.lo[i] - o[i]) * t;
+            R t2 = (b.hi[i] - o[i]) * t;
+            if (t2 < t1) { R tt = t1; t1 = t2; t2 = tt; }
+            if (t1 > tmin) tmin = t1;
+            if (t2 < tmax) tmax = t2;
+            if (tmin > tmax || tmax < 0) { tnear = kMax; tfar = kMax; return false; }
+        } else {
+            if ((b.lo[i] - o[i] > 0) || (b.hi[i] - o[i] < 0)) { tnear = kMax; tfar = kMax; return false; }
+        }
+    }
+    tnear = tmin;
+    tfar = tmax;
+    return true;
+}
+
+struct Node {
+    Box<double> box;
+    int right, left, offset, count;
+};
+struct Tri { int surf, index; };
+
+struct Bvh {
+    std::vector<Node> nodes;
+    std::vector<Tri> tris;
+    int root = 0;
+};
+
+struct Mesh {  // double geometry as given by the caller
+    const xmc_surface* s;
+    int n;
+    const xmc_double3& vert(const Tri& t, int k) const {
+        const xmc_int3& tr = s[t.surf].triangles[t.index];
+        int vi = k == 0 ? tr.x : (k == 1 ? tr.y : tr.z);
+        return s[t.surf].vertices[vi];
+    }
+};
+
+inline double comp(const xmc_double3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+Box<double> create_box(const Mesh& m, const Tri* t, int n) {  // :88-123
+    Box<double> b;
+    const double kMax = 1.0e256;
+    b.lo = {kMax, kMax, kMax};
+    b.hi = {-kMax, -kMax, -kMax};
+    for (int i = 0; i < n; i++) {
+        for (int a = 0; a < 3; a++)
+            for (int k = 0; k < 3; k++)
+                if (comp(m.vert(t[i], k), a) < b.lo[a]) b.lo[a] = comp(m.vert(t[i], k), a);
+        for (int a = 0; a < 3; a++)
+            for (int k = 0; k < 3; k++)
+                if (comp(m.vert(t[i], k), a) > b.hi[a]) b.hi[a] = comp(m.vert(t[i], k), a);
+    }
+    return b;
+}
+
+constexpr int kBins = 16;   // BIN_NUMBER
+constexpr int kLeafMax = 5; // MAX_TRIANGLES_IN_LEAVES
+
+inline double centroid(const Mesh& m, const Tri& t, int a) {
+    return (comp(m.vert(t, 0), a) + comp(m.vert(t, 1), a) + comp(m.vert(t, 2), a)) / 3.0;
+}
+
+void find_split(const Box<double>& box, const Mesh& m, const Tri* t, int n, double& min_sah,
+                int& split, int& axis_out) {  // :125-200
+    int bins[3][kBins] = {};
+    double size[3], d[3];
+    for (int a = 0; a < 3; a++) {
+        size[a] = box.hi[a] - box.lo[a];
+        d[a] = size[a] / kBins;
+    }
+    for (int i = 0; i < n; i++) {
+        for (int a = 0; a < 3; a++) {
+            int bi;
+            if (d[a] > 1e-6) {
+                double c = centroid(m, t[i], a);
+                bi = (int)((c - box.lo[a]) / d[a]);
+                if (bi == kBins) bi--;
+            } else {
+                bi = 0;
+            }
+            bins[a][bi]++;
+        }
+    }
+    double inv_area = 0.5 / (size[0] * size[1] + size[1] * size[2] + size[0] * size[2]);
+    min_sah = 1.0e256;
+    for (int a = 0; a < 3; a++) {
+        if (d[a] > 1e-6) {
+            int lc = 0, rc = n;
+            double tmp = size[a];
+            for (int i = 1; i < kBins; i++) {
+                lc += bins[a][i - 1];
+                rc -= bins[a][i - 1];
+                if (lc > 0 && rc > 0) {
+                    size[a] = i * d[a];
+                    double la = 2.0 * (size[0] * size[1] + size[1] * size[2] + size[2] * size[0]);
+                    size[a] = tmp - size[a];
+                    double ra = 2.0 * (size[0] * size[1] + size[1] * size[2] + size[2] * size[0]);
+                    double sah = inv_area * (lc * la + ra * rc);
+                    if (sah < min_sah) {
+                        min_sah = sah;
+                        axis_out = a;
+                        split = i;
+                    }
+                }
+            }
+            size[a] = tmp;
+        }
+    }
+}
+
+int build_node(Bvh& bvh, const Mesh& m, int offset, int n) {  // :202-314
+    Tri* t = bvh.tris.data() + offset;
+    Box<double> box = create_box(m, t, n);
+    auto leaf = [&]() {
+        Node nd;
+        nd.box = box;
+        nd.right = -1;
+        nd.left = -1;
+        nd.count = n;
+        nd.offset = offset;
+        bvh.nodes.push_back(nd);
+        return (int)bvh.nodes.size() - 1;
+    };
+    if (n <= kLeafMax) return leaf();
+    int split = 0, axis = 0;
+    double min_sah;
+    find_split(box, m, t, n, min_sah, split, axis);
+    double sah = n;
+    if (sah <= min_sah) return leaf();
+    double d = (box.hi[axis] - box.lo[axis]) / kBins;
+    auto bin_of = [&](int i) {
+        int b = (int)((centroid(m, t[i], axis) - box.lo[axis]) / d);
+        if (b == kBins) b--;
+        return b;
+    };
+    int lc = 0, k = 0, j = n - 1;
+    int i1 = bin_of(k), i2 = bin_of(j);
+    while (k < j) {
+        if (i1 < split) {
+            k++;
+            lc++;
+            if (k < n) i1 = bin_of(k);
+        }
+        if (i2 >= split) {
+            j--;
+            if (j >= 0) i2 = bin_of(j);
+        }
+        if (i1 >= split && i2 < split) {
+            Tri tmp = t[k];
+            t[k] = t[j];
+            t[j] = tmp;
+            k++;
+            j--;
+            lc++;
+            if (k < n) i1 = bin_of(k);
+            if (j >= 0) i2 = bin_of(j);
+        }
+    }
+    Node nd;
+    nd.box = box;
+    nd.count = 0;
+    nd.offset = -1;
+    nd.left = build_node(bvh, m, offset, lc);
+    nd.right = build_node(bvh, m, offset + lc, n - lc);
+    bvh.nodes.push_back(nd);
+    return (int)bvh.nodes.size() - 1;
+}
+
+void build_bvh(Bvh& bvh, const xmc_surface* s, int ns) {  // :316-344
+    bvh.nodes.clear();
+    bvh.tris.clear();
+    for (int i = 0; i < ns; i++)
+        for (int j = 0; j < s[i].numberOfTriangles; j++) bvh.tris.push_back({i, j});
+    Mesh m{s, ns};
+    bvh.root = build_node(bvh, m, 0, (int)bvh.tris.size());
+}
+
+// ----------------------------------------------------------------------------
+// the scene in precision R
+// ----------------------------------------------------------------------------
+template <class R> struct Hit {
+    V3<R> normal;
+    R distance;
+    int surface;
+    int found;
+    int tri;  // position in bvh.tris
+};
+
+template <class R> struct Scene {
+    const xmc_input* in = nullptr;
+    Bvh bvh;
+    std::vector<Box<R>> boxes;
+    std::vector<V3<R>> tv;  // 3 vertices per bvh triangle, in bvh.tris order
+    struct Layer { R n, mua, mus, g; bool glass; };
+    std::vector<Layer> layers;
+    V3<R> area_corner, area_length;
+    R area_part[3];
+    int part[3];
+    struct Cube { V3<R> center, length; uint8_t filter[XMC_MAX_LAYERS]; R cos_target; };
+    struct Ring { V3<R> center; R rs, rb; uint8_t filter[XMC_MAX_LAYERS]; R cos_target; };
+    std::vector<Cube> cubes;
+    std::vector<Ring> rings;
+    double rsp = 0.0;
+    R w0;
+    R min_weight, chance;
+    V3<R> start_pos, start_dir;
+    // time bins as InitOutput builds them (launcher_omp.cpp:38-55)
+    std::vector<double> t_start, t_finish;
+
+    void init(const xmc_input* input) {
+        in = input;
+        build_bvh(bvh, input->surface, input->numberOfSurfaces);
+        boxes.resize(bvh.nodes.size());
+        for (size_t i = 0; i < bvh.nodes.size(); i++) {
+            const Node& n = bvh.nodes[i];
+            boxes[i].lo = {(R)n.box.lo.x, (R)n.box.lo.y, (R)n.box.lo.z};
+            boxes[i].hi = {(R)n.box.hi.x, (R)n.box.hi.y, (R)n.box.hi.z};
+        }
+        Mesh m{input->surface, input->numberOfSurfaces};
+        tv.resize(bvh.tris.size() * 3);
+        for (size_t i = 0; i < bvh.tris.size(); i++)
+            for (int k = 0; k < 3; k++) tv[3 * i + k] = cvt<R>(m.vert(bvh.tris[i], k));
+        layers.resize(input->numberOfLayers);
+        for (int i = 0; i < input->numberOfLayers; i++) {
+            const xmc_layer& l = input->layerInfo[i];
+            layers[i] = {(R)l.refractiveIndex, (R)l.absorptionCoefficient, (R)l.scatteringCoefficient,
+                         (R)l.anisotropy,
+                         l.absorptionCoefficient == 0.0 && l.scatteringCoefficient == 0.0};
+        }
+        area_corner = cvt<R>(input->area->corner);
+        area_length = cvt<R>(input->area->length);
+        part[0] = input->area->partitionNumber.x;
+        part[1] = input->area->partitionNumber.y;
+        part[2] = input->area->partitionNumber.z;
+        for (int a = 0; a < 3; a++) area_part[a] = (R)part[a];
+        for (int i = 0; i < input->numberOfCubeDetectors; i++) {
+            const xmc_cube_detector& c = input->cubeDetector[i];
+            Cube q;
+            q.center = cvt<R>(c.center);
+            q.length = cvt<R>(c.length);
+            std::memcpy(q.filter, c.filterLayers, XMC_MAX_LAYERS);
+            q.cos_target = (R)cos(c.targetAngle * kPi / 180.0);
+            cubes.push_back(q);
+        }
+        for (int i = 0; i < input->numberOfRingDetectors; i++) {
+            const xmc_ring_detector& c = input->ringDetector[i];
+            Ring q;
+            q.center = cvt<R>(c.center);
+            q.rs = (R)c.smallRadius;
+            q.rb = (R)c.bigRadius;
+            std::memcpy(q.filter, c.filterLayers, XMC_MAX_LAYERS);
+            q.cos_target = (R)cos(c.targetAngle * kPi / 180.0);
+            rings.push_back(q);
+        }
+        rsp = xmco_specular(input->layerInfo);
+        w0 = (R)(1.0 - rsp);
+        min_weight = (R)input->minWeight;
+        chance = (R)input->rouletteChance;
+        start_pos = cvt<R>(input->startPosition);
+        start_dir = cvt<R>(input->startDirection);
+        int T = input->timeScaleSize;
+        t_start.assign(T > 0 ? T : 0, 0.0);
+        t_finish.assign(T > 0 ? T : 0, 0.0);
+        if (T > 0) {
+            double dt = (input->timeFinish - input->timeStart) / T;
+            if (dt < 0.0) dt = 0.0;
+            for (int j = 0; j < T - 1; j++) {
+                t_start[j] = input->timeStart + j * dt;
+                t_finish[j] = t_start[j] + dt;
+            }
+            t_start[T - 1] = input->timeStart + (T - 1) * dt;
+            t_finish[T - 1] = input->timeFinish;
+        }
+    }
+
+    V3<R> tri_normal(int ti) const {
+        return normalize(plane_normal(tv[3 * ti], tv[3 * ti + 1], tv[3 * ti + 2]));
+    }
+
+    R tri_t(int ti, const V3<R>& o, const V3<R>& d) const {
+        return triangle_distance(o, d, tv[3 * ti], tv[3 * ti + 1], tv[3 * ti + 2]);
+    }
+
+    // ComputeBVHIntersection (:354-466) when limited, else
+    // ComputeBVHIntersectionWithoutStep (:468-584).
+    Hit<R> intersect(const V3<R>& o, const V3<R>& d, R step, bool limited) const {
+        Hit<R> res;
+        res.found = 0;
+        res.distance = Prec<R>::kMaxDistance;
+        res.surface = -1;
+        res.tri = -1;
+        res.normal = {0, 0, 0};
+        R tn1, tn2, tf;
+        int cur = bvh.root;
+        if (!box_hit(boxes[cur], o, d, tn1, tf)) return res;
+        if (limited && step < tn1) return res;
+        struct Entry { int node; R tnear; };
+        std::vector<Entry> stack;
+        stack.reserve(64);
+        auto pop = [&](bool& done) {
+            if (stack.empty()) { done = true; return; }
+            Entry info = stack.back();
+            cur = info.node;
+            stack.pop_back();
+            if (res.distance < info.tnear) {
+                while (!stack.empty() && res.distance < info.tnear) {
+                    info = stack.back();
+                    stack.pop_back();
+                }
+                if (res.distance < info.tnear || (limited && step < info.tnear)) {
+                    done = true;
+                    return;
+                }
+                cur = info.node;
+            }
+        };
+        while (true) {
+            const Node& nd = bvh.nodes[cur];
+            bool use_stack;
+            if (nd.offset >= 0) {
+                int best = -1;
+                for (int i = 0; i < nd.count; i++) {
+                    int ti = nd.offset + i;
+                    R t = tri_t(ti, o, d);
+                    if (t >= (R)0.0 && t < res.distance) {
+                        res.distance = t;
+                        best = ti;
+                    }
+                }
+                if (best >= 0 && (!limited || res.distance <= step)) {
+                    res.surface = bvh.tris[best].surf;
+                    res.tri = best;
+                    res.found = 1;
+                    res.normal = tri_normal(best);
+                }
+                use_stack = true;
+            } else {
+                int left = nd.left, right = nd.right;
+                bool f1 = box_hit(boxes[left], o, d, tn1, tf);
+                bool f2 = box_hit(boxes[right], o, d, tn2, tf);
+                if (limited) {
+                    use_stack = false;
+                    if (f1 && f2) {
+                        if (tn1 <= tn2) {
+                            if (tn1 <= step) {
+                                cur = left;
+                                if (tn2 <= step) stack.push_back({right, tn2});
+                            } else use_stack = true;
+                        } else {
+                            if (tn2 <= step) {
+                                cur = right;
+                                if (tn1 <= step) stack.push_back({left, tn1});
+                            } else use_stack = true;
+                        }
+                    } else if (f1) {
+                        if (tn1 <= step) cur = left;
+                        else use_stack = true;
+                    } else {
+                        if (f2 && tn2 <= step) cur = right;
+                        else use_stack = true;
+                    }
+                } else {
+                    use_stack = false;
+                    if (f1 && f2) {
+                        if (tn1 <= tn2) {
+                            cur = left;
+                            stack.push_back({right, tn2});
+                        } else {
+                            cur = right;
+                            stack.push_back({left, tn1});
+                        }
+                    } else if (f1) cur = left;
+                    else if (f2) cur = right;
+                    else use_stack = true;
+                }
+            }
+            if (use_stack) {
+                bool done = false;
+                pop(done);
+                if (done) return res;
+            }
+        }
+    }
+
+    // brute force over every triangle in (surface, triangle) order
+    Hit<R> brute(const V3<R>& o, const V3<R>& d, R step, bool limited) const {
+        Hit<R> res;
+        res.found = 0;
+        res.distance = Prec<R>::kMaxDistance;
+        res.surface = -1;
+        res.tri = -1;
+        res.normal = {0, 0, 0};
+        // visit in the caller's order, not bvh order
+        std::vector<int> pos_of;  // bvh position of (surface, index)
+        const xmc_surface* s = in->surface;
+        std::vector<int> base(in->numberOfSurfaces + 1, 0);
+        for (int i = 0; i < in->numberOfSurfaces; i++) base[i + 1] = base[i] + s[i].numberOfTriangles;
+        pos_of.assign(base.back(), 0);
+        for (size_t i = 0; i < bvh.tris.size(); i++) pos_of[base[bvh.tris[i].surf] + bvh.tris[i].index] = (int)i;
+        for (int g = 0; g < base.back(); g++) {
+            int ti = pos_of[g];
+            R t = tri_t(ti, o, d);
+            if (t >= (R)0.0 && t < res.distance) {
+                res.distance = t;
+                res.tri = ti;
+            }
+        }
+        if (res.tri >= 0 && (!limited || res.distance <= step)) {
+            res.found = 1;
+            res.surface = bvh.tris[res.tri].surf;
+            res.normal = tri_normal(res.tri);
+        }
+        return res;
+    }
+
+    // GetAreaIndexVector / GetAreaIndex (:190-227); returns -1 outside
+    int area_index(const V3<R>& p, int v[3]) const {
+        R ix = area_part[0] * (p.x - area_corner.x) / area_length.x;
+        R iy = area_part[1] * (p.y - area_corner.y) / area_length.y;
+        R iz = area_part[2] * (p.z - area_corner.z) / area_length.z;
+        bool inside = !((ix < (R)0.0 || ix >= area_part[0]) || (iy < (R)0.0 || iy >= area_part[1]) ||
+                        (iz < (R)0.0 || iz >= area_part[2]));
+        uint8_t bx = 255, by = 255, bz = 255;
+        if (inside) {
+            bx = (uint8_t)ix;
+            by = (uint8_t)iy;
+            bz = (uint8_t)iz;
+        }
+        v[0] = bx;
+        v[1] = by;
+        v[2] = bz;
+        if (bx == 255 && by == 255 && bz == 255) return -1;
+        return bx * part[1] * part[2] + by * part[2] + bz;
+    }
+
+    int find_layer(int surface, int current) const {  // :566-580
+        for (int i = 0; i < in->numberOfLayers; ++i)
+            for (int j = 0; j < in->layerInfo[i].numberOfSurfaces; ++j)
+                if (in->layerInfo[i].surfaceId[j] == surface && i != current) return i;
+        return -1;
+    }
+};
+
+// ----------------------------------------------------------------------------
+// physics helpers (mcml_kernel.cpp)
+// ----------------------------------------------------------------------------
+template <class R> R transmit_cos(R n1, R n2, R ci) {  // :497-529
+    if (n1 == n2) return ci;
+    if (ci > Prec<R>::kCosZero) return ci;
+    if (ci < Prec<R>::kCos90) return (R)0.0;
+    R si = std::sqrt(Prec<R>::one_minus_sq(ci));
+    R st = n1 * si / n2;
+    if (st >= (R)1.0) return (R)0.0;
+    return std::sqrt(Prec<R>::one_minus_sq(st));
+}
+
+template <class R> R fresnel(R n1, R n2, R ci, R ct) {  // :531-564
+    if (n1 == n2) return (R)0.0;
+    if (ci > Prec<R>::kCosZero) {
+        R r = (n2 - n1) / (n2 + n1);
+        return r * r;
+    }
+    if (ci < Prec<R>::kCos90) return (R)1.0;
+    R si = std::sqrt(Prec<R>::one_minus_sq(ci));
+    R st = std::sqrt(Prec<R>::one_minus_sq(ct));
+    R cs = ci * ct - si * st;
+    R cd = ci * ct + si * st;
+    R ss = si * ct + ci * st;
+    R sd = si * ct - ci * st;
+    return (R)0.5 * sd * sd * (cd * cd + cs * cs) / (ss * ss * cd * cd);
+}
+
+template <class R> R critical_cos(R n1, R n2) {  // :644-661
+    if (n1 > n2) return std::sqrt((R)1.0 - n2 * n2 / (n1 * n1));
+    return (R)0.0;
+}
+
+template <class R> V3<R> reflect(V3<R> i, V3<R> n) {  // :582-593
+    V3<R> sn = normalize(n);
+    V3<R> si = normalize(i);
+    R tmp = (R)2 * dot(si, sn);
+    sn.x *= tmp;
+    sn.y *= tmp;
+    sn.z *= tmp;
+    return normalize(sub(si, sn));
+}
+
+template <class R> V3<R> refract(R n1, R n2, V3<R> i, V3<R> n) {  // :595-616
+    V3<R> si = normalize(i);
+    si.x *= n1;
+    si.y *= n1;
+    si.z *= n1;
+    V3<R> sn = normalize(n);
+    R t1 = dot(si, sn);
+    R t2 = std::sqrt((n2 * n2 - n1 * n1) / (t1 * t1) + (R)1);
+    t2 = (t2 - (R)1) * t1;
+    V3<R> t = {si.x + t2 * sn.x, si.y + t2 * sn.y, si.z + t2 * sn.z};
+    return normalize(t);
+}
+
+template <class R> R cosine_theta(R g, Mcg59& rng) {  // :475-495
+    R u = Prec<R>::uniform(rng.next());
+    R c;
+    if (g != (R)0.0) {
+        R tmp = ((R)1 - g * g) / ((R)1 - g + (R)2 * g * u);
+        c = ((R)1 + g * g - tmp * tmp) / ((R)2 * g);
+    } else {
+        c = (R)2.0 * u - (R)1.0;
+    }
+    if (c > (R)1.0) c = (R)1.0;
+    else if (c < (R)-1.0) c = (R)-1.0;
+    return c;
+}
+
+template <class R> V3<R> new_direction(V3<R> d, R g, Mcg59& rng) {  // :442-473
+    V3<R> r;
+    R ux = d.x, uy = d.y, uz = d.z;
+    R ct = cosine_theta(g, rng);
+    R st = std::sqrt(Prec<R>::one_minus_sq(ct));
+    R sp, cp;
+    Prec<R>::sincos_2pi(Prec<R>::uniform(rng.next()), &sp, &cp);
+    if (std::fabs(uz) <= Prec<R>::kCosZero) {
+        R tmp = std::sqrt(Prec<R>::one_minus_sq(uz));
+        r.x = st * (ux * uz * cp - uy * sp) / tmp + ux * ct;
+        r.y = st * (uy * uz * cp + ux * sp) / tmp + uy * ct;
+        r.z = -st * cp * tmp + uz * ct;
+    } else {
+        R tmp = (uz >= (R)0.0) ? (R)1.0 : (R)-1.0;
+        r.x = st * cp;
+        r.y = st * sp;
+        r.z = tmp * ct;
+    }
+    return r;
+}
+
+template <class R> R step_size(R mua, R mus, Mcg59& rng) {  // :618-631
+    R u;
+    do {
+        u = Prec<R>::uniform(rng.next());
+    } while (u == (R)0.0);
+    return -Prec<R>::log_(u) / (mua + mus);
+}
+
+// ----------------------------------------------------------------------------
+// per-thread tallies (launcher_omp.cpp:60-74 keeps absorption/scattering per
+// thread; detector fields are shared with omp atomic in the reference -- the
+// oracle keeps them per thread and sums at the end)
+// ----------------------------------------------------------------------------
+struct Tally {
+    std::vector<double> absorption, scattering;
+    std::vector<double> det_weight, det_target;
+    std::vector<uint64_t> det_count;
+    std::vector<std::vector<uint64_t>> det_traj;
+    std::vector<double> bin_weight, bin_target;  // ndet * T
+    std::vector<uint64_t> bin_count;
+    uint64_t scatter = 0, boundary = 0, roulette = 0, draws = 0, photons = 0;
+    void init(int grid, int ndet, int T, bool abs, bool scat) {
+        if (abs) absorption.assign(grid, 0.0);
+        if (scat) {
+            scattering.assign(grid, 0.0);
+            det_traj.assign(ndet, std::vector<uint64_t>(grid, 0));
+        }
+        det_weight.assign(ndet, 0.0);
+        det_target.assign(ndet, 0.0);
+        det_count.assign(ndet, 0);
+        bin_weight.assign((size_t)ndet * T, 0.0);
+        bin_target.assign((size_t)ndet * T, 0.0);
+        bin_count.assign((size_t)ndet * T, 0);
+    }
+};
+
+template <class R> struct Photon {
+    V3<R> pos, dir;
+    R step, weight, time, target_range, other_range;
+    int layer;
+    bool visited[XMC_MAX_LAYERS];
+    bool dead;
+};
+
+struct Traj {  // PhotonTrajectory, grown on demand
+    std::vector<uint8_t> x, y, z;
+    void clear() { x.clear(); y.clear(); z.clear(); }
+};
+
+template <class R> struct Walker {
+    const Scene<R>& sc;
+    Tally& tl;
+    bool scat;
+    Traj traj;
+    xmc_trace_event* ev = nullptr;
+    int max_ev = 0, n_ev = 0;
+
+    Walker(const Scene<R>& s, Tally& t, bool sc_) : sc(s), tl(t), scat(sc_) {}
+
+    void record(int type, const Photon<R>& p, int aux, const Mcg59& rng) {
+        if (ev) {
+            if (n_ev < max_ev) {
+                xmc_trace_event& e = ev[n_ev];
+                e.type = type;
+                e.layer = p.layer;
+                e.aux = aux;
+                e.draws = rng.draws;
+                e.pos[0] = p.pos.x; e.pos[1] = p.pos.y; e.pos[2] = p.pos.z;
+                e.dir[0] = p.dir.x; e.dir[1] = p.dir.y; e.dir[2] = p.dir.z;
+                e.weight = p.weight;
+                e.time = p.time;
+            }
+            n_ev++;
+        }
+    }
+
+    // UpdatePhotonTrajectory (:750-792), double precision only
+    void update_trajectory(const Photon<R>& p, V3<R> prev) {
+        if (!scat) return;
+        V3<double> cur = {(double)p.pos.x, (double)p.pos.y, (double)p.pos.z};
+        V3<double> pv = {(double)prev.x, (double)prev.y, (double)prev.z};
+        V3<double> a, b;
+        if (cur.z > pv.z) { a = pv; b = cur; }
+        else if (cur.z < pv.z) { a = cur; b = pv; }
+        else return;
+        const xmc_area* ar = sc.in->area;
+        double step = ar->length.z / ar->partitionNumber.z;
+        int plane_index = (int)((a.z - ar->corner.z + step / 2.0) / step);
+        double plane = ar->corner.z + plane_index * step + step / 2.0;
+        while ((plane <= b.z) && (plane < ar->corner.z + ar->length.z)) {
+            V3<double> q = plane_segment_point(a, b, plane);
+            int v[3];
+            int idx = area_index_d(q, v);
+            if ((v[0] == 255 && v[1] == 255 && v[2] == 255) || idx == -1) break;
+            traj.x.push_back((uint8_t)v[0]);
+            traj.y.push_back((uint8_t)v[1]);
+            traj.z.push_back((uint8_t)v[2]);
+            tl.scattering[idx] += (double)p.weight;
+            plane += step;
+        }
+    }
+
+    int area_index_d(const V3<double>& p, int v[3]) const {
+        const xmc_area* a = sc.in->area;
+        double ix = a->partitionNumber.x * (p.x - a->corner.x) / a->length.x;
+        double iy = a->partitionNumber.y * (p.y - a->corner.y) / a->length.y;
+        double iz = a->partitionNumber.z * (p.z - a->corner.z) / a->length.z;
+        bool inside = !((ix < 0.0 || ix >= a->partitionNumber.x) || (iy < 0.0 || iy >= a->partitionNumber.y) ||
+                        (iz < 0.0 || iz >= a->partitionNumber.z));
+        uint8_t bx = 255, by = 255, bz = 255;
+        if (inside) { bx = (uint8_t)ix; by = (uint8_t)iy; bz = (uint8_t)iz; }
+        v[0] = bx; v[1] = by; v[2] = bz;
+        if (bx == 255 && by == 255 && bz == 255) return -1;
+        return bx * a->partitionNumber.y * a->partitionNumber.z + by * a->partitionNumber.z + bz;
+    }
+
+    void move(Photon<R>& p) {  // MovePhotonAndUpdateTrajectory (:158-174)
+        V3<R> prev = p.pos;
+        p.time += p.step * sc.layers[p.layer].n;
+        if (sc.in->targetRangeLayers[p.layer]) p.target_range += p.step;
+        else p.other_range += p.step;
+        p.pos.x += p.step * p.dir.x;
+        p.pos.y += p.step * p.dir.y;
+        p.pos.z += p.step * p.dir.z;
+        update_trajectory(p, prev);
+    }
+
+    // MovePhotonOnMinDistanceAndUpdateTrajectory (:176-188) in double; in
+    // float the photon is placed delta off the hit point `hit` on the side
+    // its (new) direction points to (see Prec<float>).
+    void nudge(Photon<R>& p, const V3<R>& n, const V3<R>& hit) {
+        if constexpr (sizeof(R) == 8) {
+            V3<R> prev = p.pos;
+            const R md = Prec<R>::kMinDistance;
+            p.pos.x += md * p.dir.x;
+            p.pos.y += md * p.dir.y;
+            p.pos.z += md * p.dir.z;
+            update_trajectory(p, prev);
+        } else {
+            R d = offset_scale(hit);
+            if (dot(p.dir, n) < (R)0) d = -d;
+            p.pos.x = hit.x + d * n.x;
+            p.pos.y = hit.y + d * n.y;
+            p.pos.z = hit.z + d * n.z;
+        }
+    }
+
+    static R offset_scale(const V3<R>& p) {
+        R m = std::fmax(std::fabs(p.x), std::fmax(std::fabs(p.y), std::fabs(p.z)));
+        return std::fmaf(m, 0x1p-20f, 0x1p-22f);
+    }
+
+    int detector_id(const Photon<R>& p) const {  // :663-742
+        int nc = (int)sc.cubes.size();
+        for (int i = 0; i < nc; ++i) {
+            const auto& c = sc.cubes[i];
+            V3<R> h = {c.length.x / (R)2.0, c.length.y / (R)2.0, c.length.z / (R)2.0};
+            bool in = (p.pos.x >= (c.center.x - h.x)) && (p.pos.x < (c.center.x + h.x)) &&
+                      (p.pos.y >= (c.center.y - h.y)) && (p.pos.y < (c.center.y + h.y)) &&
+                      (p.pos.z >= (c.center.z - h.z)) && (p.pos.z < (c.center.z + h.z));
+            if (in && visited_filter(p, c.filter)) {
+                R len = p.dir.x * p.dir.x + p.dir.y * p.dir.y + p.dir.z * p.dir.z;
+                R ac = (-p.dir.z) / len;
+                if (ac > c.cos_target) return i;
+            }
+        }
+        for (int i = 0; i < (int)sc.rings.size(); ++i) {
+            const auto& r = sc.rings[i];
+            R dist = std::sqrt(((p.pos.x - r.center.x) * (p.pos.x - r.center.x)) +
+                               ((p.pos.y - r.center.y) * (p.pos.y - r.center.y)));
+            bool in = (dist >= r.rs) && (dist < r.rb);
+            if (in && visited_filter(p, r.filter)) {
+                R len = p.dir.x * p.dir.x + p.dir.y * p.dir.y + p.dir.z * p.dir.z;
+                R ac = (-p.dir.z) / len;
+                if (ac > r.cos_target) return nc + i;
+            }
+        }
+        return -1;
+    }
+
+    static bool visited_filter(const Photon<R>& p, const uint8_t* f) {
+        for (int j = 0; j < XMC_MAX_LAYERS; ++j)
+            if (p.visited[j] && f[j]) return true;
+        return false;
+    }
+
+    void detect(const Photon<R>& p, int id) {  // :744-858
+        double w = (double)p.weight;
+        tl.det_weight[id] += w;
+        if (scat) {
+            const xmc_area* a = sc.in->area;
+            for (size_t i = 0; i < traj.x.size(); ++i) {
+                int idx = traj.x[i] * a->partitionNumber.y * a->partitionNumber.z +
+                          traj.y[i] * a->partitionNumber.z + traj.z[i];
+                tl.det_traj[id][idx]++;
+            }
+        }
+        tl.det_count[id]++;
+        int T = sc.in->timeScaleSize;
+        // ranges and weights in precision R, summed in double
+        R total = p.target_range + p.other_range;
+        R tw = p.target_range * (p.weight / total);
+        double t = (double)p.time;
+        size_t base = (size_t)id * T;
+        bool binned = false;
+        for (int i = 0; i < T - 1; ++i) {
+            if ((t >= sc.t_start[i]) && (t < sc.t_finish[i])) {
+                tl.bin_target[base + i] += (double)tw;
+                tl.bin_count[base + i]++;
+                tl.bin_weight[base + i] += w;
+                binned = true;
+                break;
+            }
+        }
+        if (!binned && T > 0 && t >= sc.t_start[T - 1]) {
+            tl.bin_target[base + T - 1] += (double)tw;
+            tl.bin_count[base + T - 1]++;
+            tl.bin_weight[base + T - 1] += w;
+        }
+        tl.det_target[id] += (double)tw;
+    }
+
+    // CrossBoundary (:229-284)
+    void cross(Photon<R>& p, const Hit<R>& h, const V3<R>& hit, Mcg59& rng) {
+        R ci = dot(p.dir, h.normal);
+        if (ci < (R)0.0) ci = -ci;
+        int cur = p.layer;
+        int nxt = sc.find_layer(h.surface, cur);
+        if (nxt < 0) {  // surface not shared with another layer: undefined in the reference
+            p.dead = true;
+            return;
+        }
+        R n1 = sc.layers[cur].n, n2 = sc.layers[nxt].n;
+        R cc = critical_cos(n1, n2);
+        R refl;
+        if (ci <= cc) refl = (R)1.0;
+        else {
+            R ct = transmit_cos(n1, n2, ci);
+            refl = fresnel(n1, n2, ci, ct);
+        }
+        tl.boundary++;
+        if (Prec<R>::uniform(rng.next()) > refl) {
+            if (nxt == 0) {
+                int id = detector_id(p);
+                if (id >= 0) detect(p, id);
+                p.dead = true;
+                record(XMC_EV_ESCAPE, p, id, rng);
+            } else {
+                p.dir = refract(n1, n2, p.dir, h.normal);
+                p.layer = nxt;
+                p.visited[nxt] = true;
+                nudge(p, h.normal, hit);
+                record(XMC_EV_TRANSMIT, p, h.surface, rng);
+            }
+        } else {
+            p.dir = reflect(p.dir, h.normal);
+            nudge(p, h.normal, hit);
+            record(XMC_EV_REFLECT, p, h.surface, rng);
+        }
+    }
+
+    // ComputePhoton (:39-126) with the termination policy
+    void photon(Mcg59& rng) {
+        Photon<R> p;
+        const xmc_input* in = sc.in;
+        traj.clear();
+        p.weight = sc.w0;  // LaunchPhoton (:128-149)
+        p.dead = false;
+        p.layer = 1;
+        p.step = 0;
+        p.time = 0;
+        p.target_range = 0;
+        p.other_range = 0;
+        p.pos = sc.start_pos;
+        if constexpr (sizeof(R) == 8) p.pos.z = in->startPosition.z + 1.0e-8;
+        else p.pos.z = sc.start_pos.z + offset_scale(sc.start_pos);
+        p.dir = sc.start_dir;
+        for (int i = 0; i < XMC_MAX_LAYERS; ++i) p.visited[i] = false;
+        p.visited[p.layer] = true;
+        record(XMC_EV_LAUNCH, p, 0, rng);
+        const R step_unlimited = Prec<R>::kMaxDistance;
+        while (!p.dead) {
+            const auto& L = sc.layers[p.layer];
+            Hit<R> h;
+            if (L.glass) {
+                h = sc.intersect(p.pos, p.dir, step_unlimited, false);
+            } else {
+                p.step = step_size(L.mua, L.mus, rng);
+                h = sc.intersect(p.pos, p.dir, p.step, true);
+            }
+            int v[3];
+            if (h.found) {
+                p.step = h.distance - (R)0.01 * Prec<R>::kMinDistance;
+                move(p);
+                V3<R> hit = p.pos;
+                if constexpr (sizeof(R) == 4) {
+                    R d = offset_scale(hit);
+                    if (dot(p.dir, h.normal) < (R)0) d = -d;
+                    p.pos.x = hit.x - d * h.normal.x;
+                    p.pos.y = hit.y - d * h.normal.y;
+                    p.pos.z = hit.z - d * h.normal.z;
+                }
+                if (sc.area_index(p.pos, v) >= 0) {
+                    cross(p, h, hit, rng);
+                } else {
+                    p.dead = true;
+                    record(XMC_EV_DIE_AREA, p, 0, rng);
+                }
+            } else if (L.glass) {
+                p.dead = true;
+                record(XMC_EV_GLASS_MISS, p, 0, rng);
+            } else {
+                move(p);
+                int idx = sc.area_index(p.pos, v);
+                if (idx >= 0) {
+                    R dw = p.weight * L.mua / (L.mua + L.mus);  // :633-642
+                    p.weight -= dw;
+                    if (!tl.absorption.empty()) tl.absorption[idx] += (double)dw;
+                    p.dir = new_direction(p.dir, L.g, rng);
+                    tl.scatter++;
+                    record(XMC_EV_SCATTER, p, idx, rng);
+                } else {
+                    p.dead = true;
+                    record(XMC_EV_DIE_AREA, p, 0, rng);
+                }
+            }
+            if (in->terminationMode == XMC_TERMINATE_ROULETTE) {
+                if (p.weight < sc.min_weight && !p.dead) {
+                    tl.roulette++;
+                    if (p.weight == (R)0.0) {
+                        p.dead = true;
+                        record(XMC_EV_ROULETTE_DIE, p, 0, rng);
+                    } else if (Prec<R>::uniform(rng.next()) < sc.chance) {
+                        p.weight /= sc.chance;
+                        record(XMC_EV_ROULETTE_LIVE, p, 0, rng);
+                    } else {
+                        p.dead = true;
+                        record(XMC_EV_ROULETTE_DIE, p, 0, rng);
+                    }
+                }
+            } else if (p.weight < sc.min_weight) {
+                if (!p.dead) record(XMC_EV_DIE_WEIGHT, p, 0, rng);
+                p.dead = true;
+            }
+        }
+        tl.draws += rng.draws;
+        tl.photons++;
+    }
+};
+
+Mcg59 stream_for(uint64_t seed, uint64_t photon) {
+    Mcg59 g;
+    g.value = photon_stream(seed, photon);
+    g.offset = kMcgC;
+    g.draws = 0;
+    return g;
+}
+
+void add_into(xmc_output* out, const Tally& t, int T, bool scat) {
+    if (out->absorption && !t.absorption.empty())
+        for (int j = 0; j < out->gridSize; ++j) out->absorption[j] += t.absorption[j];
+    if (scat && out->scattering && !t.scattering.empty())
+        for (int j = 0; j < out->gridSize; ++j) out->scattering[j] += t.scattering[j];
+    for (int d = 0; d < out->numberOfDetectors; ++d) {
+        xmc_detector_info& di = out->detectorInfo[d];
+        di.weight += t.det_weight[d];
+        di.targetWeight += t.det_target[d];
+        di.numberOfPhotons += t.det_count[d];
+        if (scat && di.trajectory && !t.det_traj.empty())
+            for (int j = 0; j < out->gridSize; ++j) di.trajectory[j] += t.det_traj[d][j];
+        for (int b = 0; b < T && b < di.timeScaleSize; ++b) {
+            size_t k = (size_t)d * T + b;
+            di.timeScale[b].weight += t.bin_weight[k];
+            di.timeScale[b].targetWeight += t.bin_target[k];
+            di.timeScale[b].numberOfPhotons += t.bin_count[k];
+        }
+    }
+}
+
+template <class R>
+int run_impl(const xmc_input* in, xmc_output* out, uint64_t seed, uint64_t first, uint64_t n,
+             int nthreads, bool scat, xmco_counters* c) {
+    if (sizeof(R) == 4) scat = false;  // float mode mirrors the GPU, which has no scattering grid
+    Scene<R> sc;
+    sc.init(in);
+    int T = in->timeScaleSize;
+    int ndet = in->numberOfCubeDetectors + in->numberOfRingDetectors;
+    if (out->numberOfDetectors != ndet) return XMC_ERR_ARG;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    std::vector<Tally> tallies(nthreads);
+    for (auto& t : tallies) t.init(out->gridSize, ndet, T, out->absorption != nullptr, scat);
+    auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel num_threads(nthreads)
+    {
+        int tid = omp_get_thread_num();
+        Walker<R> w(sc, tallies[tid], scat);
+#pragma omp for schedule(dynamic, 16)
+        for (long long i = 0; i < (long long)n; ++i) {
+            Mcg59 g = stream_for(seed, first + (uint64_t)i);
+            w.photon(g);
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    out->specularReflectance = sc.rsp;
+    for (auto& t : tallies) add_into(out, t, T, scat);
+    if (c) {
+        std::memset(c, 0, sizeof(*c));
+        for (auto& t : tallies) {
+            c->photons += t.photons;
+            c->scatterEvents += t.scatter;
+            c->boundaryEvents += t.boundary;
+            c->rouletteEvents += t.roulette;
+            c->draws += t.draws;
+        }
+        c->seconds = std::chrono::duration<double>(t1 - t0).count();
+        c->threads = nthreads;
+    }
+    return XMC_OK;
+}
+
+template <class R>
+int trace_impl(const xmc_input* in, uint64_t seed, uint64_t photon, xmc_trace_event* ev, int max_ev) {
+    Scene<R> sc;
+    sc.init(in);
+    int ndet = in->numberOfCubeDetectors + in->numberOfRingDetectors;
+    Tally t;
+    t.init(in->area->partitionNumber.x * in->area->partitionNumber.y * in->area->partitionNumber.z, ndet,
+           in->timeScaleSize, false, false);
+    Walker<R> w(sc, t, false);
+    w.ev = ev;
+    w.max_ev = max_ev;
+    Mcg59 g = stream_for(seed, photon);
+    w.photon(g);
+    return w.n_ev <= max_ev ? w.n_ev : -w.n_ev;
+}
+
+template <class R>
+int query_impl(const xmc_input* in, const double* o, const double* d, const double* steps, int nr,
+               int mode, int* found, double* dist, int* surf, int* tri, double* normal) {
+    Scene<R> sc;
+    sc.init(in);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nr; ++i) {
+        V3<R> oo = {(R)o[3 * i], (R)o[3 * i + 1], (R)o[3 * i + 2]};
+        V3<R> dd = {(R)d[3 * i], (R)d[3 * i + 1], (R)d[3 * i + 2]};
+        bool limited = steps[i] > 0.0;
+        R st = limited ? (R)steps[i] : Prec<R>::kMaxDistance;
+        Hit<R> h = mode == 0 ? sc.intersect(oo, dd, st, limited) : sc.brute(oo, dd, st, limited);
+        found[i] = h.found;
+        dist[i] = h.found ? (double)h.distance : 0.0;
+        surf[i] = h.found ? h.surface : -1;
+        tri[i] = h.found ? sc.bvh.tris[h.tri].index : -1;
+        normal[3 * i] = h.found ? (double)h.normal.x : 0.0;
+        normal[3 * i + 1] = h.found ? (double)h.normal.y : 0.0;
+        normal[3 * i + 2] = h.found ? (double)h.normal.z : 0.0;
+    }
+    return XMC_OK;
+}
+
+V3<double> d3(const double* p) { return {p[0], p[1], p[2]}; }
+
+}  // namespace
+
+extern "C" {
+
+uint64_t xmco_photon_stream(uint64_t seed, uint64_t photon) { return photon_stream(seed, photon); }
+
+int xmco_run(const xmc_input* input, xmc_output* output, uint64_t seed, uint64_t first, uint64_t n,
+             int precision, int nthreads, int scattering, xmco_counters* counters) {
+    if (!input || !output) return XMC_ERR_ARG;
+    if (precision == 32)
+        return run_impl<float>(input, output, seed, first, n, nthreads, scattering != 0, counters);
+    return run_impl<double>(input, output, seed, first, n, nthreads, scattering != 0, counters);
+}
+
+int xmco_trace(const xmc_input* input, uint64_t seed, uint64_t photon, int precision,
+               xmc_trace_event* events, int maxEvents) {
+    if (precision == 32) return trace_impl<float>(input, seed, photon, events, maxEvents);
+    return trace_impl<double>(input, seed, photon, events, maxEvents);
+}
+
+int xmco_bvh(const xmc_input* input, int* nn, int* root, int* nt, double* boxes, int* links,
+             int* tsurf, int* tidx) {
+    Bvh b;
+    build_bvh(b, input->surface, input->numberOfSurfaces);
+    *nn = (int)b.nodes.size();
+    *root = b.root;
+    *nt = (int)b.tris.size();
+    if (boxes) {
+        for (size_t i = 0; i < b.nodes.size(); i++) {
+            const Node& n = b.nodes[i];
+            double* bx = boxes + 6 * i;
+            bx[0] = n.box.lo.x; bx[1] = n.box.lo.y; bx[2] = n.box.lo.z;
+            bx[3] = n.box.hi.x; bx[4] = n.box.hi.y; bx[5] = n.box.hi.z;
+            links[4 * i] = n.left;
+            links[4 * i + 1] = n.right;
+            links[4 * i + 2] = n.offset;
+            links[4 * i + 3] = n.count;
+        }
+        for (size_t i = 0; i < b.tris.size(); i++) {
+            tsurf[i] = b.tris[i].surf;
+            tidx[i] = b.tris[i].index;
+        }
+    }
+    return XMC_OK;
+}
+
+int xmco_query(const xmc_input* input, const double* o, const double* d, const double* steps, int nr,
+               int precision, int mode, int* found, double* dist, int* surf, int* tri, double* normal) {
+    if (precision == 32) return query_impl<float>(input, o, d, steps, nr, mode, found, dist, surf, tri, normal);
+    return query_impl<double>(input, o, d, steps, nr, mode, found, dist, surf, tri, normal);
+}
+
+uint64_t xmco_mcg59_init(uint64_t seed, uint32_t id, uint32_t step, uint64_t* offset) {
+    uint64_t value = 2 * seed + 1;  // InitMCG59, mcml_mcg59.cpp:28-35
+    value = (value * mcg_pow(kMcgC, id)) & kMcgMask;
+    *offset = mcg_pow(kMcgC, step);
+    return value;
+}
+
+double xmco_mcg59_next(uint64_t* value, uint64_t offset) {
+    Mcg59 g{*value, offset, 0};
+    double u = Prec<double>::uniform(g.next());
+    *value = g.value;
+    return u;
+}
+
+double xmco_specular(const xmc_layer* layer) {  // :16-37
+    double t = (layer[0].refractiveIndex - layer[1].refractiveIndex) /
+               (layer[0].refractiveIndex + layer[1].refractiveIndex);
+    double r1 = t * t;
+    if (layer[1].absorptionCoefficient == 0.0 && layer[1].scatteringCoefficient == 0.0) {
+        t = (layer[1].refractiveIndex - layer[2].refractiveIndex) /
+            (layer[1].refractiveIndex + layer[2].refractiveIndex);
+        double r2 = t * t;
+        t = 1 - r1;
+        r1 = r1 + t * t * r2 / (1 - r1 * r2);
+    }
+    return r1;
+}
+
+double xmco_cosine_theta(double g, uint64_t* value, uint64_t offset) {
+    Mcg59 r{*value, offset, 0};
+    double c = cosine_theta<double>(g, r);
+    *value = r.value;
+    return c;
+}
+
+void xmco_new_direction(const double* dir, double g, uint64_t* value, uint64_t offset, double* out) {
+    Mcg59 r{*value, offset, 0};
+    V3<double> v = new_direction<double>(d3(dir), g, r);
+    *value = r.value;
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+}
+
+double xmco_transmit_cos(double n1, double n2, double ci) { return transmit_cos<double>(n1, n2, ci); }
+double xmco_fresnel(double n1, double n2, double ci, double ct) { return fresnel<double>(n1, n2, ci, ct); }
+
+int xmco_find_layer(const xmc_input* input, int surfaceId, int layer) {
+    for (int i = 0; i < input->numberOfLayers; ++i)
+        for (int j = 0; j < input->layerInfo[i].numberOfSurfaces; ++j)
+            if (input->layerInfo[i].surfaceId[j] == surfaceId && i != layer) return i;
+    return -1;
+}
+
+void xmco_reflect(const double* i, const double* n, double* out) {
+    V3<double> r = reflect<double>(d3(i), d3(n));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+void xmco_refract(double n1, double n2, const double* i, const double* n, double* out) {
+    V3<double> r = refract<double>(n1, n2, d3(i), d3(n));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+double xmco_step(double mua, double mus, uint64_t* value, uint64_t offset) {
+    Mcg59 r{*value, offset, 0};
+    double s = step_size<double>(mua, mus, r);
+    *value = r.value;
+    return s;
+}
+
+double xmco_critical_cos(double n1, double n2) { return critical_cos<double>(n1, n2); }
+
+int xmco_area_index(const xmc_area* a, const double* p) {
+    double ix = a->partitionNumber.x * (p[0] - a->corner.x) / a->length.x;
+    double iy = a->partitionNumber.y * (p[1] - a->corner.y) / a->length.y;
+    double iz = a->partitionNumber.z * (p[2] - a->corner.z) / a->length.z;
+    bool inside = !((ix < 0.0 || ix >= a->partitionNumber.x) || (iy < 0.0 || iy >= a->partitionNumber.y) ||
+                    (iz < 0.0 || iz >= a->partitionNumber.z));
+    uint8_t bx = 255, by = 255, bz = 255;
+    if (inside) { bx = (uint8_t)ix; by = (uint8_t)iy; bz = (uint8_t)iz; }
+    if (bx == 255 && by == 255 && bz == 255) return -1;
+    return bx * a->partitionNumber.y * a->partitionNumber.z + by * a->partitionNumber.z + bz;
+}
+
+int xmco_detector_id(const xmc_input* input, const double* pos, const double* dir, const uint8_t* visited) {
+    Scene<double> sc;
+    sc.in = input;
+    for (int i = 0; i < input->numberOfCubeDetectors; i++) {
+        const xmc_cube_detector& c = input->cubeDetector[i];
+        Scene<double>::Cube q;
+        q.center = cvt<double>(c.center);
+        q.length = cvt<double>(c.length);
+        std::memcpy(q.filter, c.filterLayers, XMC_MAX_LAYERS);
+        q.cos_target = cos(c.targetAngle * kPi / 180.0);
+        sc.cubes.push_back(q);
+    }
+    for (int i = 0; i < input->numberOfRingDetectors; i++) {
+        const xmc_ring_detector& c = input->ringDetector[i];
+        Scene<double>::Ring q;
+        q.center = cvt<double>(c.center);
+        q.rs = c.smallRadius;
+        q.rb = c.bigRadius;
+        std::memcpy(q.filter, c.filterLayers, XMC_MAX_LAYERS);
+        q.cos_target = cos(c.targetAngle * kPi / 180.0);
+        sc.rings.push_back(q);
+    }
+    Tally t;
+    Walker<double> w(sc, t, false);
+    Photon<double> p;
+    p.pos = d3(pos);
+    p.dir = d3(dir);
+    for (int i = 0; i < XMC_MAX_LAYERS; i++) p.visited[i] = visited[i] != 0;
+    return w.detector_id(p);
+}
+
+double xmco_triangle_distance(const double* o, const double* d, const double* a, const double* b,
+                              const double* c) {
+    return triangle_distance<double>(d3(o), d3(d), d3(a), d3(b), d3(c));
+}
+
+float xmco_logf(float x) { return f_logf(x); }
+void xmco_sincos2pi(float u, float* s, float* c) { f_sincos2pi(u, s, c); }
+
+}  // extern "C"

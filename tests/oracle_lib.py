@@ -1,0 +1,137 @@
+"""ctypes bindings of the CPU oracle (oracle/liboracle.so) and of the
+reference build (oracle/_ref/libxmcml_ref.so).  Test infrastructure."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libxmcml_ref.so"
+
+P = C.c_void_p
+D = C.c_double
+I = C.c_int
+U64 = C.c_uint64
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int)
+PU64 = C.POINTER(C.c_uint64)
+
+
+class Counters(C.Structure):
+    _fields_ = [("photons", U64), ("scatterEvents", U64), ("boundaryEvents", U64),
+                ("rouletteEvents", U64), ("draws", U64), ("seconds", D), ("threads", I)]
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_SO.exists():
+            raise RuntimeError(f"{ORACLE_SO} is not built (run __graft_entry__.build())")
+        lib = C.CDLL(str(ORACLE_SO))
+        _sig(lib, "xmco_photon_stream", U64, [U64, U64])
+        _sig(lib, "xmco_run", I, [P, P, U64, U64, U64, I, I, I, C.POINTER(Counters)])
+        _sig(lib, "xmco_trace", I, [P, U64, U64, I, P, I])
+        _sig(lib, "xmco_bvh", I, [P, PI, PI, PI, PD, PI, PI, PI])
+        _sig(lib, "xmco_query", I, [P, PD, PD, PD, I, I, I, PI, PD, PI, PI, PD])
+        _sig(lib, "xmco_mcg59_init", U64, [U64, C.c_uint32, C.c_uint32, PU64])
+        _sig(lib, "xmco_mcg59_next", D, [PU64, U64])
+        _sig(lib, "xmco_specular", D, [P])
+        _sig(lib, "xmco_cosine_theta", D, [D, PU64, U64])
+        _sig(lib, "xmco_new_direction", None, [PD, D, PU64, U64, PD])
+        _sig(lib, "xmco_transmit_cos", D, [D, D, D])
+        _sig(lib, "xmco_fresnel", D, [D, D, D, D])
+        _sig(lib, "xmco_find_layer", I, [P, I, I])
+        _sig(lib, "xmco_reflect", None, [PD, PD, PD])
+        _sig(lib, "xmco_refract", None, [D, D, PD, PD, PD])
+        _sig(lib, "xmco_step", D, [D, D, PU64, U64])
+        _sig(lib, "xmco_critical_cos", D, [D, D])
+        _sig(lib, "xmco_area_index", I, [P, PD])
+        _sig(lib, "xmco_detector_id", I, [P, PD, PD, C.POINTER(C.c_uint8)])
+        _sig(lib, "xmco_triangle_distance", D, [PD, PD, PD, PD, PD])
+        _sig(lib, "xmco_logf", C.c_float, [C.c_float])
+        _sig(lib, "xmco_sincos2pi", None, [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)])
+        _oracle = lib
+    return _oracle
+
+
+def ref_available() -> bool:
+    return REF_SO.exists()
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not REF_SO.exists():
+            raise RuntimeError(f"{REF_SO} is not built")
+        lib = C.CDLL(str(REF_SO))
+        _sig(lib, "xref_run", I, [P, P, U64, U64, U64, I])
+        _sig(lib, "xref_launch_omp", D, [P, P, U64, I])
+        _sig(lib, "xref_trace", I, [P, U64, U64, P, I])
+        _sig(lib, "xref_bvh", I, [P, PI, PI, PI, PD, PI, PI, PI])
+        _sig(lib, "xref_query", I, [P, PD, PD, PD, I, PI, PD, PI, PD])
+        _sig(lib, "xref_specular", D, [P])
+        _sig(lib, "xref_cosine_theta", D, [D, PU64, U64])
+        _sig(lib, "xref_new_direction", None, [PD, D, PU64, U64, PD])
+        _sig(lib, "xref_transmit_cos", D, [D, D, D])
+        _sig(lib, "xref_fresnel", D, [D, D, D, D])
+        _sig(lib, "xref_reflect", None, [PD, PD, PD])
+        _sig(lib, "xref_refract", None, [D, D, PD, PD, PD])
+        _sig(lib, "xref_step", D, [D, D, PU64, U64])
+        _sig(lib, "xref_triangle_distance", D, [PD, PD, PD, PD, PD])
+        _sig(lib, "xref_detector_id", I, [P, PD, PD, C.POINTER(C.c_uint8)])
+        _ref = lib
+    return _ref
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(PD)
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(PI)
+
+
+def trace(lib_fn, inp, seed, photon, max_ev=4096, precision=None):
+    """Run a trace function and return the events as a structured array."""
+    from xmcml_amd.abi import TRACE_DTYPE
+    buf = np.zeros(max_ev, dtype=TRACE_DTYPE)
+    if precision is None:
+        n = lib_fn(inp.ptr, seed, photon, buf.ctypes.data, max_ev)
+    else:
+        n = lib_fn(inp.ptr, seed, photon, precision, buf.ctypes.data, max_ev)
+    if n < 0:
+        raise RuntimeError(f"trace longer than {max_ev} events ({-n})")
+    return buf[:n]
+
+
+def bvh_arrays(fn, inp):
+    nn, root, nt = C.c_int(), C.c_int(), C.c_int()
+    fn(inp.ptr, C.byref(nn), C.byref(root), C.byref(nt), None, None, None, None)
+    boxes = np.zeros((nn.value, 6))
+    links = np.zeros((nn.value, 4), np.int32)
+    ts = np.zeros(nt.value, np.int32)
+    ti = np.zeros(nt.value, np.int32)
+    fn(inp.ptr, C.byref(nn), C.byref(root), C.byref(nt), dptr(boxes), iptr(links), iptr(ts), iptr(ti))
+    return dict(root=root.value, boxes=boxes, links=links, tsurf=ts, tidx=ti)
+
+
+def run_oracle(spec, inp, out, seed, first, n, precision=64, nthreads=1, scattering=False):
+    c = Counters()
+    rc = oracle().xmco_run(inp.ptr, out.ptr, seed, first, n, precision, nthreads, int(scattering), C.byref(c))
+    assert rc == 0, rc
+    return c

@@ -1,0 +1,154 @@
+"""Synthetic inputs of the documented workloads, as xmcml scenes.
+
+xmcml describes every boundary as a triangulated surface (Surface,
+base/xmcml/xmcml/mcml_types.h:33-39) and every layer by the list of surfaces
+that bound it (LayerInfo, mcml_kernel_types.h:54-62); layer 0 is the ambient
+medium.  The generators here build those meshes:
+
+* ``plane`` -- a 2-triangle square in the vertex/triangle order of the
+  reference's surface generator (base/tools/xmcmlSurfaceGenerator/
+  xmcmlSurfaceGenerator/plane.cpp:3-27).
+* ``heightfield`` -- z = z0 + h(x, y) on a regular grid, two triangles per cell.
+* ``ellipsoid`` -- an icosphere scaled to an ellipsoid with seeded radial noise.
+
+The heightfield workload uses one shared perturbation h for the three
+perturbed boundaries (conformal layers): layer 1 is 0.1 mm thick while the sum
+of the four amplitudes can reach 1.2 mm, so independent surfaces would cut
+through each other.
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .abi import (CubeSpec, LayerSpec, RingSpec, SceneSpec, TERMINATE_KILL, TERMINATE_ROULETTE)
+
+SPEED_OF_LIGHT_MM_PER_NS = 299.792458
+
+# The three tissue layers of the slab and heightfield workloads: (n, mua, mus, g).
+SLAB_LAYERS = [(1.4, 0.3, 20.0, 0.8), (1.4, 0.05, 15.0, 0.9), (1.37, 0.02, 10.0, 0.9)]
+SLAB_DEPTHS = [0.0, 0.1, 2.1, 22.1]
+
+
+def plane(z: float, side: float = 2000.0, cx: float = 0.0, cy: float = 0.0) -> Tuple[np.ndarray, np.ndarray]:
+    h = side / 2.0
+    v = np.array([[cx - h, cy - h, z], [cx - h, cy + h, z], [cx + h, cy - h, z], [cx + h, cy + h, z]],
+                 dtype=np.float64)
+    t = np.array([[0, 1, 2], [1, 2, 3]], dtype=np.int32)
+    return v, t
+
+
+def grid_triangles(n: int) -> np.ndarray:
+    """Two triangles per cell of an (n+1) x (n+1) vertex grid (row-major in y)."""
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    v00 = (i * (n + 1) + j).ravel()
+    v10 = ((i + 1) * (n + 1) + j).ravel()
+    v01 = (i * (n + 1) + j + 1).ravel()
+    v11 = ((i + 1) * (n + 1) + j + 1).ravel()
+    t = np.empty((2 * n * n, 3), dtype=np.int32)
+    t[0::2] = np.stack([v00, v10, v01], 1)
+    t[1::2] = np.stack([v10, v11, v01], 1)
+    return t
+
+
+class SineHeight:
+    """h(x, y) = sum_k A_k sin(kx_k x + ky_k y + phi_k), drawn from a seed:
+    A_k ~ U[0.05, 0.3] mm, |k| ~ U[0.2, 2] /mm, direction and phase ~ U[0, 2 pi)."""
+
+    def __init__(self, seed: int, terms: int = 4):
+        rng = np.random.default_rng(seed)
+        self.A = rng.uniform(0.05, 0.3, terms)
+        kmag = rng.uniform(0.2, 2.0, terms)
+        ang = rng.uniform(0.0, 2 * math.pi, terms)
+        self.kx = kmag * np.cos(ang)
+        self.ky = kmag * np.sin(ang)
+        self.phi = rng.uniform(0.0, 2 * math.pi, terms)
+
+    def __call__(self, x, y):
+        x = np.asarray(x, dtype=np.float64)
+        y = np.asarray(y, dtype=np.float64)
+        out = np.zeros(np.broadcast(x, y).shape)
+        for a, kx, ky, p in zip(self.A, self.kx, self.ky, self.phi):
+            out = out + a * np.sin(kx * x + ky * y + p)
+        return out
+
+
+def heightfield(z0: float, height: SineHeight, cells: int = 128, half: float = 20.0):
+    xs = np.linspace(-half, half, cells + 1)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    Z = z0 + height(X, Y)
+    v = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float64)
+    return v, grid_triangles(cells)
+
+
+def rings(count: int, width: float, center=(0.0, 0.0, 0.0)) -> List[RingSpec]:
+    return [RingSpec(i * width, (i + 1) * width, center) for i in range(count)]
+
+
+def _slab_layers(layers=SLAB_LAYERS) -> List[LayerSpec]:
+    n = len(layers)
+    out = [LayerSpec(1.0, 0.0, 0.0, 0.0, [0, n])]
+    for i, (ri, mua, mus, g) in enumerate(layers):
+        out.append(LayerSpec(ri, mua, mus, g, [i, i + 1]))
+    return out
+
+
+def slab_scene(photons: int = 1_000_000, time_bins: int = 50, time_finish: float = 100.0) -> SceneSpec:
+    """3-layer planar slab, pencil beam, 20 rings of 0.5 mm, 100^3 grid over
+    20 x 20 x 20 mm, roulette at 1e-4 with survival 0.1.  Time is optical
+    path in mm (mcml_kernel.cpp:161)."""
+    surfaces = [plane(z) for z in SLAB_DEPTHS]
+    return SceneSpec(
+        layers=_slab_layers(), surfaces=surfaces, area_corner=(-10.0, -10.0, 0.0),
+        area_length=(20.0, 20.0, 20.0), area_partitions=(100, 100, 100), rings=rings(20, 0.5),
+        min_weight=1e-4, termination=TERMINATE_ROULETTE, roulette_chance=0.1, time_start=0.0,
+        time_finish=time_finish, time_bins=time_bins, photons=photons, name="slab-3layer")
+
+
+def heightfield_scene(photons: int = 100_000_000, seed: int = 2024, cells: int = 128,
+                      time_bins: int = 256, time_ns: float = 2.0) -> SceneSpec:
+    """Three conformal seeded heightfield boundaries (z0 = 0, 0.1, 2.1 mm) on a
+    cells x cells grid over 40 x 40 mm plus a flat bottom at 22.1 mm; 20 rings
+    of 0.5 mm; a time-of-flight histogram of time_bins bins over 0..time_ns
+    at speed c/n, i.e. over optical path 0..c*time_ns mm."""
+    h = SineHeight(seed)
+    surfaces = [heightfield(z, h, cells) for z in SLAB_DEPTHS[:3]]
+    surfaces.append(plane(SLAB_DEPTHS[3], side=40.0))
+    z_top = float(h(0.0, 0.0))
+    return SceneSpec(
+        layers=_slab_layers(), surfaces=surfaces, area_corner=(-20.0, -20.0, -2.0),
+        area_length=(40.0, 40.0, 24.2), area_partitions=(1, 1, 1), rings=rings(20, 0.5),
+        min_weight=1e-4, termination=TERMINATE_ROULETTE, roulette_chance=0.1, time_start=0.0,
+        time_finish=SPEED_OF_LIGHT_MM_PER_NS * time_ns, time_bins=time_bins,
+        start_position=(0.0, 0.0, z_top), photons=photons, name="heightfield-3layer")
+
+
+def small_scene(kind: str, photons: int = 2000) -> SceneSpec:
+    """Small cases that exercise rarely taken branches."""
+    if kind == "glass":
+        # a non-scattering glass layer on top of tissue (mcml_kernel.cpp:50-77)
+        layers = [(1.5, 0.0, 0.0, 0.0), (1.4, 0.1, 10.0, 0.85), (1.4, 0.02, 5.0, 0.9)]
+        s = slab_scene(photons)
+        s.layers = _slab_layers(layers)
+        s.surfaces = [plane(z) for z in [0.0, 1.0, 3.0, 10.0]]
+        s.name = "glass-top"
+        return s
+    if kind == "mismatch":
+        # strong index steps: total internal reflection inside the slab
+        layers = [(1.6, 0.2, 8.0, 0.7), (1.1, 0.05, 5.0, 0.9), (1.45, 0.02, 10.0, 0.0)]
+        s = slab_scene(photons)
+        s.layers = _slab_layers(layers)
+        s.cubes = [CubeSpec((0.0, 0.0, 0.0), (2.0, 2.0, 0.01), filter_layers=(2,), target_angle=60.0)]
+        s.target_range_layers = (2,)
+        s.termination = TERMINATE_KILL
+        s.min_weight = 1e-3
+        s.name = "index-mismatch"
+        return s
+    if kind == "heightfield":
+        s = heightfield_scene(photons, cells=32, time_bins=64)
+        s.area_partitions = (40, 40, 20)
+        s.name = "heightfield-small"
+        return s
+    raise ValueError(kind)
