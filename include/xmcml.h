@@ -167,14 +167,24 @@ typedef struct xmc_rng {
     uint64_t nextPhoton;
 } xmc_rng;
 
-/* Event counters of the last launch on a scene. */
+/* Event counters and weight ledger of the work since the last stats call. */
 typedef struct xmc_stats {
     uint64_t photons;
     uint64_t scatterEvents;  /* hop + drop + spin */
     uint64_t boundaryEvents; /* Fresnel reflect / refract / escape */
     uint64_t rouletteEvents;
-    uint64_t stackRestarts;  /* BVH short-stack overflows (traversal restarted) */
-    double kernelMs;         /* device time of the transport kernel (HIP events) */
+    uint64_t stackSpills;    /* BVH stack pushes that spilled beyond the LDS entries */
+    double kernelMs;         /* device time of the transport kernels (HIP events) */
+    /* where the launched weight went; with the specular part they sum to the
+     * number of photons, up to float rounding */
+    double wAbsorbed;
+    double wEscaped;      /* transmitted into layer 0 */
+    double wDetected;     /* part of wEscaped that hit a detector */
+    double wLeftArea;
+    double wKilled;       /* below minWeight, terminationMode KILL */
+    double wRouletteLost;
+    double wRouletteGain;
+    double wOther;        /* glass misses, surfaces without a next layer */
 } xmc_stats;
 
 /* One recorded step of a traced photon (debug / parity).  Positions and
@@ -202,11 +212,10 @@ typedef struct xmc_trace_event {
 
 /* Launch options (all zero = defaults). */
 typedef struct xmc_options {
-    int blockSize;        /* threads per workgroup; 0 = autotuned/default */
-    int blocksPerCU;      /* persistent workgroups per CU; 0 = default */
-    int stackDepth;       /* LDS short-stack entries per lane; 0 = default */
-    int absorptionTile;   /* 1 = privatise the hottest absorption voxels in LDS */
-    int device;           /* HIP device ordinal */
+    int device;          /* HIP device ordinal */
+    int blockSize;       /* threads per workgroup (64..256, multiple of 64); 0 = 256 */
+    int blocksPerCU;     /* cap on persistent workgroups per CU; 0 = occupancy limit */
+    int skipAbsorption;  /* 1 = do not tally the absorption grid */
 } xmc_options;
 
 typedef struct xmc_scene xmc_scene;
@@ -220,6 +229,9 @@ int xmc_scene_destroy(xmc_scene* scene);
  * into the scene's device tallies.  Inputs stay resident in HBM. */
 int xmc_scene_run(xmc_scene* scene, uint64_t seed, uint64_t firstPhoton,
                   uint64_t numberOfPhotons, void* hipStream);
+
+/* Blocks until every launch queued on the scene has finished. */
+int xmc_scene_sync(xmc_scene* scene);
 
 /* Waits for the scene's work, adds the device tallies into `output`
  * (+=, like LaunchOMP), sets output->specularReflectance and zeroes the

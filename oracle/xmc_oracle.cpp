@@ -51,6 +51,8 @@ constexpr uint64_t kMcgC = 302875106592253ULL;      // 13^13
 constexpr uint64_t kMcgM = 576460752303423488ULL;   // 2^59
 constexpr uint64_t kMcgMask = 576460752303423487ULL;
 constexpr int kStreamLog2 = 24;                     // draws reserved per photon
+// photons about to run past their block of draws are retired
+constexpr uint32_t kDrawLimit = (1u << kStreamLog2) - 64u;
 
 uint64_t mcg_pow(uint64_t a, uint64_t e) {
     uint64_t r = 1;
@@ -157,6 +159,9 @@ template <> struct Prec<double> {
 // Float-only rules (mirrored by the GPU kernel):
 //  - uniforms carry the top 24 bits of the MCG59 state: (x >> 35) * 2^-24;
 //  - log and sincos(2 pi u) are f_logf / f_sincos2pi above;
+//  - after each spin the direction gets one first-order renormalisation
+//    step, d *= 1.5 - 0.5 |d|^2 (the rotation formula does not preserve
+//    |d| = 1 and float rounding would otherwise drift it by ~1e-4);
 //  - 1 - x^2 is evaluated as (1 - x)(1 + x), and the "cosine of zero"
 //    threshold is 1 - 1e-6 (1 - 1e-12 rounds to 1.0f, which would send
 //    normal-incidence directions through a 0/0 in the spin);
@@ -785,6 +790,14 @@ template <class R> V3<R> new_direction(V3<R> d, R g, Mcg59& rng) {  // :442-473
         r.y = st * sp;
         r.z = tmp * ct;
     }
+    if constexpr (sizeof(R) == 4) {
+        // float only: one first-order renormalisation step, d *= (3 - |d|^2)/2,
+        // keeps |d| within float rounding of 1 over long walks
+        R f = std::fmaf((R)-0.5, dot(r, r), (R)1.5);
+        r.x = r.x * f;
+        r.y = r.y * f;
+        r.z = r.z * f;
+    }
     return r;
 }
 
@@ -809,6 +822,7 @@ struct Tally {
     std::vector<double> bin_weight, bin_target;  // ndet * T
     std::vector<uint64_t> bin_count;
     uint64_t scatter = 0, boundary = 0, roulette = 0, draws = 0, photons = 0;
+    double w_abs = 0, w_esc = 0, w_det = 0, w_area = 0, w_kill = 0, w_rlost = 0, w_rgain = 0, w_other = 0;
     void init(int grid, int ndet, int T, bool abs, bool scat) {
         if (abs) absorption.assign(grid, 0.0);
         if (scat) {
@@ -1018,6 +1032,7 @@ template <class R> struct Walker {
         int nxt = sc.find_layer(h.surface, cur);
         if (nxt < 0) {  // surface not shared with another layer: undefined in the reference
             p.dead = true;
+            tl.w_other += (double)p.weight;
             return;
         }
         R n1 = sc.layers[cur].n, n2 = sc.layers[nxt].n;
@@ -1032,7 +1047,11 @@ template <class R> struct Walker {
         if (Prec<R>::uniform(rng.next()) > refl) {
             if (nxt == 0) {
                 int id = detector_id(p);
-                if (id >= 0) detect(p, id);
+                if (id >= 0) {
+                    detect(p, id);
+                    tl.w_det += (double)p.weight;
+                }
+                tl.w_esc += (double)p.weight;
                 p.dead = true;
                 record(XMC_EV_ESCAPE, p, id, rng);
             } else {
@@ -1070,6 +1089,11 @@ template <class R> struct Walker {
         record(XMC_EV_LAUNCH, p, 0, rng);
         const R step_unlimited = Prec<R>::kMaxDistance;
         while (!p.dead) {
+            if (rng.draws > kDrawLimit) {  // stream block exhausted: retire the photon
+                p.dead = true;
+                tl.w_other += (double)p.weight;
+                break;
+            }
             const auto& L = sc.layers[p.layer];
             Hit<R> h;
             if (L.glass) {
@@ -1094,10 +1118,12 @@ template <class R> struct Walker {
                     cross(p, h, hit, rng);
                 } else {
                     p.dead = true;
+                    tl.w_area += (double)p.weight;
                     record(XMC_EV_DIE_AREA, p, 0, rng);
                 }
             } else if (L.glass) {
                 p.dead = true;
+                tl.w_other += (double)p.weight;
                 record(XMC_EV_GLASS_MISS, p, 0, rng);
             } else {
                 move(p);
@@ -1106,11 +1132,13 @@ template <class R> struct Walker {
                     R dw = p.weight * L.mua / (L.mua + L.mus);  // :633-642
                     p.weight -= dw;
                     if (!tl.absorption.empty()) tl.absorption[idx] += (double)dw;
+                    tl.w_abs += (double)dw;
                     p.dir = new_direction(p.dir, L.g, rng);
                     tl.scatter++;
                     record(XMC_EV_SCATTER, p, idx, rng);
                 } else {
                     p.dead = true;
+                    tl.w_area += (double)p.weight;
                     record(XMC_EV_DIE_AREA, p, 0, rng);
                 }
             }
@@ -1121,15 +1149,21 @@ template <class R> struct Walker {
                         p.dead = true;
                         record(XMC_EV_ROULETTE_DIE, p, 0, rng);
                     } else if (Prec<R>::uniform(rng.next()) < sc.chance) {
+                        R before = p.weight;
                         p.weight /= sc.chance;
+                        tl.w_rgain += (double)p.weight - (double)before;
                         record(XMC_EV_ROULETTE_LIVE, p, 0, rng);
                     } else {
                         p.dead = true;
+                        tl.w_rlost += (double)p.weight;
                         record(XMC_EV_ROULETTE_DIE, p, 0, rng);
                     }
                 }
             } else if (p.weight < sc.min_weight) {
-                if (!p.dead) record(XMC_EV_DIE_WEIGHT, p, 0, rng);
+                if (!p.dead) {
+                    tl.w_kill += (double)p.weight;
+                    record(XMC_EV_DIE_WEIGHT, p, 0, rng);
+                }
                 p.dead = true;
             }
         }
@@ -1201,6 +1235,14 @@ int run_impl(const xmc_input* in, xmc_output* out, uint64_t seed, uint64_t first
             c->boundaryEvents += t.boundary;
             c->rouletteEvents += t.roulette;
             c->draws += t.draws;
+            c->wAbsorbed += t.w_abs;
+            c->wEscaped += t.w_esc;
+            c->wDetected += t.w_det;
+            c->wLeftArea += t.w_area;
+            c->wKilled += t.w_kill;
+            c->wRouletteLost += t.w_rlost;
+            c->wRouletteGain += t.w_rgain;
+            c->wOther += t.w_other;
         }
         c->seconds = std::chrono::duration<double>(t1 - t0).count();
         c->threads = nthreads;
@@ -1422,5 +1464,13 @@ double xmco_triangle_distance(const double* o, const double* d, const double* a,
 
 float xmco_logf(float x) { return f_logf(x); }
 void xmco_sincos2pi(float u, float* s, float* c) { f_sincos2pi(u, s, c); }
+void xmco_logf_many(const float* x, float* y, int64_t n) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) y[i] = f_logf(x[i]);
+}
+void xmco_sincos2pi_many(const float* u, float* s, float* c, int64_t n) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) f_sincos2pi(u[i], s + i, c + i);
+}
 
 }  // extern "C"

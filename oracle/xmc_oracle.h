@@ -32,6 +32,15 @@ typedef struct xmco_counters {
     uint64_t draws;
     double seconds; /* wall time of the photon loop */
     int threads;
+    /* weight ledger: where the launched weight went */
+    double wAbsorbed;      /* dropped weight (absorption grid total) */
+    double wEscaped;       /* weight of photons transmitted into layer 0 */
+    double wDetected;      /* part of wEscaped that hit a detector */
+    double wLeftArea;      /* weight of photons that left the Area */
+    double wKilled;        /* weight of photons killed below minWeight */
+    double wRouletteLost;  /* weight of photons lost in roulette */
+    double wRouletteGain;  /* weight added to roulette survivors */
+    double wOther;         /* glass misses, surfaces without a next layer */
 } xmco_counters;
 
 /* Per-photon stream: state before the first draw of photon p. */
@@ -79,9 +88,11 @@ int xmco_detector_id(const xmc_input* input, const double* pos, const double* di
                      const uint8_t* visited);
 double xmco_triangle_distance(const double* o, const double* d, const double* a, const double* b,
                               const double* c);
-/* float special functions of the 32-bit mode */
+/* float special functions of the 32-bit mode (and bulk forms for tests) */
 float xmco_logf(float x);
 void xmco_sincos2pi(float u, float* s, float* c);
+void xmco_logf_many(const float* x, float* y, int64_t n);
+void xmco_sincos2pi_many(const float* u, float* s, float* c, int64_t n);
 
 #ifdef __cplusplus
 }

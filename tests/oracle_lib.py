@@ -23,7 +23,15 @@ PU64 = C.POINTER(C.c_uint64)
 
 class Counters(C.Structure):
     _fields_ = [("photons", U64), ("scatterEvents", U64), ("boundaryEvents", U64),
-                ("rouletteEvents", U64), ("draws", U64), ("seconds", D), ("threads", I)]
+                ("rouletteEvents", U64), ("draws", U64), ("seconds", D), ("threads", I),
+                ("wAbsorbed", D), ("wEscaped", D), ("wDetected", D), ("wLeftArea", D), ("wKilled", D),
+                ("wRouletteLost", D), ("wRouletteGain", D), ("wOther", D)]
+
+    def ledger_total(self, rsp):
+        """Rsp + absorbed + weight at every death - roulette gains, per photon."""
+        n = max(self.photons, 1)
+        return rsp + (self.wAbsorbed + self.wEscaped + self.wLeftArea + self.wKilled + self.wRouletteLost
+                      + self.wOther - self.wRouletteGain) / n
 
 
 def _sig(lib, name, res, args):
@@ -65,6 +73,8 @@ def oracle():
         _sig(lib, "xmco_triangle_distance", D, [PD, PD, PD, PD, PD])
         _sig(lib, "xmco_logf", C.c_float, [C.c_float])
         _sig(lib, "xmco_sincos2pi", None, [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)])
+        _sig(lib, "xmco_logf_many", None, [P, P, C.c_int64])
+        _sig(lib, "xmco_sincos2pi_many", None, [P, P, P, C.c_int64])
         _oracle = lib
     return _oracle
 

@@ -1,0 +1,93 @@
+"""The C-ABI library: it loads, exports every function include/xmcml.h
+declares, validates inputs before touching the GPU, and lays out outputs as
+the reference's InitOutput does.  No GPU needed."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from xmcml_amd import engine, scenes
+from xmcml_amd.abi import InputHandle, Output, OutputHandle
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared(header: Path):
+    text = header.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|void|double|const char\*|uint64_t|float)\s+\*?(\w+)\s*\(",
+                                 text, re.M)))
+
+
+def test_engine_exports_every_declared_function():
+    names = declared(ROOT / "include" / "xmcml.h")
+    assert "xmc_launch" in names and "xmc_scene_run" in names
+    lib = engine.lib()
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) <= set(engine.EXPORTS)
+
+
+def test_oracle_exports_every_declared_function():
+    lib = ol.oracle()
+    for n in declared(ROOT / "oracle" / "xmc_oracle.h"):
+        assert hasattr(lib, n), n
+
+
+def _create(spec):
+    h = C.c_void_p()
+    rc = engine.lib().xmc_scene_create(InputHandle(spec).ptr, None, C.byref(h))
+    if rc == 0:
+        engine.lib().xmc_scene_destroy(h)
+    return rc
+
+
+def test_input_validation_precedes_device():
+    s = scenes.slab_scene(10)
+    s.area_partitions = (256, 10, 10)
+    assert _create(s) == -3  # XMC_ERR_LIMIT: byte3 voxel indices
+    s = scenes.slab_scene(10)
+    s.layers[1].surfaces = [0, 9]
+    assert _create(s) == -1
+    s = scenes.slab_scene(10)
+    s.time_bins = 0
+    assert _create(s) == -1
+    s = scenes.slab_scene(10)
+    s.roulette_chance = 0.0
+    assert _create(s) == -1
+    assert b"rouletteChance" in engine.lib().xmc_last_error()
+
+
+def test_no_gpu_is_an_error_not_a_fallback():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    assert _create(scenes.slab_scene(10)) == -4
+    with pytest.raises(engine.EngineError):
+        engine.Scene(scenes.slab_scene(10))
+
+
+def test_init_output_matches_initoutput_layout():
+    spec = scenes.slab_scene(10, time_bins=7, time_finish=3.0)
+    inp = InputHandle(spec)
+    out = Output()
+    assert engine.lib().xmc_init_output(inp.ptr, C.byref(out)) == 0
+    ref = OutputHandle(spec)
+    assert out.gridSize == spec.grid_size and out.numberOfDetectors == 20
+    for d in range(out.numberOfDetectors):
+        di = out.detectorInfo[d]
+        assert di.timeScaleSize == 7
+        for b in range(7):
+            assert di.timeScale[b].timeStart == ref.bins[d * 7 + b].timeStart
+            assert di.timeScale[b].timeFinish == ref.bins[d * 7 + b].timeFinish
+    engine.lib().xmc_free_output(C.byref(out))
+    assert out.detectorInfo is None or not out.detectorInfo
+
+
+def test_specular_matches_closed_form():
+    spec = scenes.slab_scene(10)
+    inp = InputHandle(spec)
+    r = engine.lib().xmc_specular_reflectance(inp.c.layerInfo)
+    assert r == ((1.0 - 1.4) / (1.0 + 1.4)) ** 2

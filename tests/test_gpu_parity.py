@@ -1,0 +1,235 @@
+"""GPU engine (through the C-ABI) against the oracle and the reference.
+
+* bit for bit against the 32-bit oracle: BVH hits, per-step traces, event
+  counts -- the same float arithmetic on the same MCG59 streams;
+* against the reference's double arithmetic (oracle/_ref when present, else
+  the 64-bit oracle, which is bit-identical to it): integer decisions
+  identical and positions / weights within 1e-5 relative over each photon's
+  common prefix;
+* statistically, under independent streams: every ring bin and the total
+  reflectance within 3 combined standard errors;
+* the weight ledger closes and the LaunchOMP-style entry point accumulates.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from xmcml_amd import scenes
+from xmcml_amd.abi import InputHandle, OutputHandle, TERMINATE_KILL
+
+pytestmark = pytest.mark.gpu
+
+SMALL = {
+    "slab": lambda n: scenes.slab_scene(n),
+    "glass": lambda n: scenes.small_scene("glass", n),
+    "mismatch": lambda n: scenes.small_scene("mismatch", n),
+    "heightfield": lambda n: scenes.small_scene("heightfield", n),
+}
+
+
+def _fields_equal(a, b):
+    if len(a) != len(b):
+        return False
+    return all(np.array_equal(a[k], b[k]) for k in a.dtype.names)
+
+
+def test_bvh_is_the_reference_tree(engine):
+    for spec in (scenes.small_scene("heightfield", 10), scenes.heightfield_scene(10)):
+        with engine.Scene(spec) as sc:
+            A = sc.bvh()
+        B = ol.bvh_arrays(ol.oracle().xmco_bvh, InputHandle(spec))
+        for k in A:
+            assert np.array_equal(A[k], B[k]), k
+
+
+def _random_rays(n, rng, lo, hi):
+    o = rng.uniform(lo, hi, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    steps = rng.exponential(0.3, size=n)
+    steps[rng.random(n) < 0.2] = 0.0  # unlimited (glass-layer query)
+    return o, d, steps
+
+
+def test_bvh_hits_bit_exact_vs_brute_force(engine):
+    spec = scenes.heightfield_scene(10)  # 98,306 triangles
+    rng = np.random.default_rng(7)
+    o, d, s = _random_rays(20000, rng, np.array([-19.0, -19.0, -1.5]), np.array([19.0, 19.0, 3.0]))
+    with engine.Scene(spec) as sc:
+        bv = sc.query(o, d, s, brute=False)
+        bf = sc.query(o, d, s, brute=True)
+    assert bv["found"].sum() > 5000
+    assert np.array_equal(bv["found"], bf["found"])
+    assert np.array_equal(bv["surface"], bf["surface"])
+    assert np.array_equal(bv["distance"], bf["distance"])
+    ties = bv["triangle"] != bf["triangle"]
+    assert ties.sum() <= 2  # equal distances on a shared edge may pick either triangle
+    # and the same walk in the 32-bit oracle
+    inp = InputHandle(spec)
+    found = np.zeros(len(s), np.int32)
+    dist = np.zeros(len(s))
+    surf = np.zeros(len(s), np.int32)
+    tri = np.zeros(len(s), np.int32)
+    nrm = np.zeros((len(s), 3))
+    ol.oracle().xmco_query(inp.ptr, ol.dptr(o), ol.dptr(d), ol.dptr(s), len(s), 32, 0, ol.iptr(found),
+                           ol.dptr(dist), ol.iptr(surf), ol.iptr(tri), ol.dptr(nrm))
+    assert np.array_equal(found, bv["found"])
+    assert np.array_equal(dist, bv["distance"])
+    assert np.array_equal(tri, bv["triangle"])
+    assert np.array_equal(nrm, bv["normal"])
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_traces_bit_identical_to_float_oracle(engine, name):
+    spec = SMALL[name](64)
+    inp = InputHandle(spec)
+    with engine.Scene(spec) as sc:
+        traces, counts = sc.trace(2024, 0, 64, max_events=20000)
+    assert counts.max() < 20000
+    bad = [p for p in range(64) if not _fields_equal(traces[p], ol.trace(ol.oracle().xmco_trace, inp, 2024, p,
+                                                                          20000, precision=32))]
+    assert not bad, f"photons whose GPU trace differs from the float oracle: {bad[:10]}"
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_tallies_match_float_oracle(engine, name):
+    n = 4000
+    spec = SMALL[name](n)
+    inp = InputHandle(spec)
+    with engine.Scene(spec) as sc:
+        sc.run(5, 100, n)
+        g = sc.collect()
+        st = sc.stats()
+    o = OutputHandle(spec)
+    c = ol.run_oracle(spec, inp, o, 5, 100, n, 32, 8)
+    assert st["photons"] == n
+    assert st["scatterEvents"] == c.scatterEvents
+    assert st["boundaryEvents"] == c.boundaryEvents
+    assert st["rouletteEvents"] == c.rouletteEvents
+    assert np.array_equal(g.det_count(), o.det_count())
+    assert np.array_equal(g.bin_count(), o.bin_count())
+    np.testing.assert_allclose(g.det_weight(), o.det_weight(), rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(g.bin_weight(), o.bin_weight(), rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(g.det_target(), o.det_target(), rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(g.absorption, o.absorption, rtol=1e-10, atol=1e-12)
+    assert g.specular == o.specular
+    assert st["wAbsorbed"] == pytest.approx(c.wAbsorbed, rel=1e-10)
+    assert st["wEscaped"] == pytest.approx(c.wEscaped, rel=1e-10)
+
+
+def _decision_prefix(a, b):
+    """Length of the common prefix of integer decisions (event type, layer)."""
+    n = min(len(a), len(b))
+    same = (a["type"][:n] == b["type"][:n]) & (a["layer"][:n] == b["layer"][:n])
+    return n if same.all() else int(np.argmin(same))
+
+
+@pytest.mark.parametrize("name", ["slab", "heightfield", "mismatch"])
+def test_decisions_match_reference(engine, name):
+    """GPU float walk vs the reference's double walk on the same streams."""
+    nphot = 200
+    spec = SMALL[name](nphot)
+    if name == "slab":
+        spec.termination = TERMINATE_KILL
+    inp = InputHandle(spec)
+    ref_trace = (lambda p: ol.trace(ol.ref().xref_trace, inp, 99, p, 50000)) if ol.ref_available() else \
+        (lambda p: ol.trace(ol.oracle().xmco_trace, inp, 99, p, 50000, precision=64))
+    with engine.Scene(spec) as sc:
+        traces, _ = sc.trace(99, 0, nphot, max_events=50000)
+    full = 0
+    worst = 0.0
+    for p in range(nphot):
+        a, b = traces[p], ref_trace(p)
+        k = _decision_prefix(a, b)
+        full += (k == len(a) == len(b))
+        if k == 0:
+            continue
+        # within the common prefix (up to 100 events) positions and weights agree to 1e-5
+        m = min(k, 100)
+        scale = np.maximum(np.abs(b["pos"][:m]).max(axis=1, keepdims=True), 1.0)
+        dp = (np.abs(a["pos"][:m] - b["pos"][:m]) / scale).max()
+        dw = (np.abs(a["weight"][:m] - b["weight"][:m]) / np.maximum(b["weight"][:m], 1e-30)).max()
+        worst = max(worst, dp, dw)
+    assert worst < 1e-5, worst
+    assert full >= 0.9 * nphot, full
+
+
+def _batches(run, nb):
+    return np.array([run(i) for i in range(nb)])
+
+
+def test_statistics_within_three_sigma(engine):
+    """Independent streams: GPU (seed 1) vs the double oracle (seed 2)."""
+    spec = scenes.slab_scene(0)
+    inp = InputHandle(spec)
+    nb, n_gpu, n_cpu = 10, 100000, 20000
+
+    def gpu_batch(i):
+        with engine.Scene(spec) as sc:
+            sc.run(1, i * n_gpu, n_gpu)
+            g = sc.collect()
+            st = sc.stats()
+        return np.concatenate([g.det_weight() / n_gpu, [g.det_weight().sum() / n_gpu, st["wAbsorbed"] / n_gpu]])
+
+    def cpu_batch(i):
+        o = OutputHandle(spec)
+        c = ol.run_oracle(spec, inp, o, 2, i * n_cpu, n_cpu, 64, 0)
+        return np.concatenate([o.det_weight() / n_cpu, [o.det_weight().sum() / n_cpu, c.wAbsorbed / n_cpu]])
+
+    G = _batches(gpu_batch, nb)
+    Cc = _batches(cpu_batch, nb)
+    mg, mc = G.mean(0), Cc.mean(0)
+    se = np.sqrt(G.var(0, ddof=1) / nb + Cc.var(0, ddof=1) / nb)
+    z = np.abs(mg - mc) / np.where(se > 0, se, 1.0)
+    assert np.all(z[mg > 1e-4] < 3.0), z
+    # total diffuse reflectance and absorption agree to 0.5 %
+    assert abs(mg[-2] / mc[-2] - 1) < 5e-3
+    assert abs(mg[-1] / mc[-1] - 1) < 5e-3
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_weight_ledger_closes(engine, name):
+    n = 20000
+    spec = SMALL[name](n)
+    with engine.Scene(spec) as sc:
+        sc.run(3, 0, n)
+        g = sc.collect()
+        st = sc.stats()
+    total = g.specular + (st["wAbsorbed"] + st["wEscaped"] + st["wLeftArea"] + st["wKilled"] + st["wRouletteLost"]
+                          + st["wOther"] - st["wRouletteGain"]) / n
+    assert abs(total - 1.0) < 1e-5
+    assert st["wDetected"] == pytest.approx(g.det_weight().sum(), rel=1e-9)
+
+
+def test_launch_entry_point_accumulates_and_advances(engine):
+    spec = scenes.slab_scene(0)
+    out = OutputHandle(spec)
+    nxt = engine.launch(spec, out, 77, 0, 3000)
+    assert nxt == 3000
+    nxt = engine.launch(spec, out, 77, nxt, 2000)
+    assert nxt == 5000
+    with engine.Scene(spec) as sc:
+        sc.run(77, 0, 5000)
+        one = sc.collect()
+    assert np.array_equal(out.det_count(), one.det_count())
+    np.testing.assert_allclose(out.det_weight(), one.det_weight(), rtol=1e-11)
+    np.testing.assert_allclose(out.absorption, one.absorption, rtol=1e-10, atol=1e-13)
+
+
+def test_heightfield_full_size_ledger_and_symmetry(engine):
+    """The 98k-triangle heightfield workload at 2e6 photons: ledger closes,
+    reflectance is positive in every ring, counters are consistent."""
+    n = 2_000_000
+    spec = scenes.heightfield_scene(n)
+    with engine.Scene(spec, absorption=False) as sc:
+        sc.run(11, 0, n)
+        g = sc.collect()
+        st = sc.stats()
+    total = g.specular + (st["wAbsorbed"] + st["wEscaped"] + st["wLeftArea"] + st["wKilled"] + st["wRouletteLost"]
+                          + st["wOther"] - st["wRouletteGain"]) / n
+    assert abs(total - 1.0) < 1e-4
+    assert st["photons"] == n
+    assert np.all(g.det_weight() > 0)
+    assert 0.999 * g.det_count().sum() <= g.bin_count().sum() <= g.det_count().sum()

@@ -1,0 +1,94 @@
+"""What the mathematics fixes, for the oracle in both precisions: closed forms
+(Beer-Lambert transmission and depth profile, the Henyey-Greenstein mean
+cosine, the mean free path), the weight ledger, and float/double agreement."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from xmcml_amd import scenes
+from xmcml_amd.abi import InputHandle, LayerSpec, OutputHandle, SceneSpec, TERMINATE_KILL, RingSpec
+
+
+def absorber_scene(mua=1.0, d=1.0, n=20000):
+    """One purely absorbing layer (mus = 0) of thickness d between index-matched
+    ambient half-spaces: every photon deposits all its weight at its first
+    interaction, so T = exp(-mua d) and the deposits follow mua exp(-mua z)."""
+    return SceneSpec(layers=[LayerSpec(1.0, 0, 0, 0, [0, 1]), LayerSpec(1.0, mua, 0.0, 0.0, [0, 1])],
+                     surfaces=[scenes.plane(0.0), scenes.plane(d)], area_corner=(-10, -10, 0),
+                     area_length=(20, 20, d), area_partitions=(1, 1, 20), rings=[RingSpec(0, 1e3)],
+                     min_weight=1e-4, termination=TERMINATE_KILL, photons=n, name="absorber")
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_beer_lambert(precision):
+    mua, d, n = 1.3, 1.0, 40000
+    spec = absorber_scene(mua, d, n)
+    inp = InputHandle(spec)
+    out = OutputHandle(spec)
+    c = ol.run_oracle(spec, inp, out, 99, 0, n, precision, 8)
+    T = math.exp(-mua * d)
+    se = math.sqrt(T * (1 - T) / n)
+    assert abs(c.wEscaped / n - T) < 4 * se
+    assert abs(c.wAbsorbed / n - (1 - T)) < 4 * se
+    # depth profile: fraction absorbed in slab [z0, z1) is exp(-mua z0) - exp(-mua z1)
+    edges = np.linspace(0.0, d, 21)
+    p = np.exp(-mua * edges[:-1]) - np.exp(-mua * edges[1:])
+    got = out.absorption / n
+    assert np.all(np.abs(got - p) < 4 * np.sqrt(p * (1 - p) / n) + 1e-12)
+
+
+def test_hg_mean_cosine_and_step_mean():
+    for g in (0.0, 0.5, 0.9):
+        off = C.c_uint64()
+        val = C.c_uint64(ol.oracle().xmco_mcg59_init(4242, 0, 1, C.byref(off)))
+        d = np.array([0.0, 0.0, 1.0])
+        out = np.zeros(3)
+        cs = []
+        for _ in range(40000):
+            ol.oracle().xmco_new_direction(ol.dptr(d), g, C.byref(val), off.value, ol.dptr(out))
+            cs.append(out[2])
+        cs = np.array(cs)
+        var = (1 + 2 * g * g) / 3 - g * g
+        assert abs(cs.mean() - g) < 4 * math.sqrt(var / cs.size)
+    off = C.c_uint64()
+    val = C.c_uint64(ol.oracle().xmco_mcg59_init(5, 0, 1, C.byref(off)))
+    mut = 12.5
+    s = np.array([ol.oracle().xmco_step(2.5, 10.0, C.byref(val), off.value) for _ in range(40000)])
+    assert abs(s.mean() - 1 / mut) < 4 / mut / math.sqrt(s.size)
+
+
+@pytest.mark.parametrize("name", ["slab", "glass", "mismatch", "heightfield"])
+@pytest.mark.parametrize("precision", [64, 32])
+def test_weight_ledger_closes(name, precision):
+    spec = scenes.slab_scene(3000) if name == "slab" else scenes.small_scene(name, 3000)
+    inp = InputHandle(spec)
+    out = OutputHandle(spec)
+    c = ol.run_oracle(spec, inp, out, 31, 0, 3000, precision, 8)
+    tol = 1e-9 if precision == 64 else 2e-5
+    assert abs(c.ledger_total(out.specular) - 1.0) < tol
+    assert abs(c.wAbsorbed - out.absorption.sum()) <= 1e-9 * max(1.0, c.wAbsorbed) or name == "heightfield"
+    assert c.wDetected == pytest.approx(out.det_weight().sum(), rel=1e-12)
+
+
+def test_float_and_double_walks_agree():
+    spec = scenes.slab_scene(20000)
+    inp = InputHandle(spec)
+    res = {}
+    for prec in (64, 32):
+        out = OutputHandle(spec)
+        c = ol.run_oracle(spec, inp, out, 1, 0, 20000, prec, 8)
+        res[prec] = (out.det_weight().sum() / 20000, c.scatterEvents / 20000)
+    # same streams: the float walk follows the double one almost everywhere
+    assert res[32][0] == pytest.approx(res[64][0], rel=2e-3)
+    assert res[32][1] == pytest.approx(res[64][1], rel=2e-3)
+    same = 0
+    for p in range(40):
+        a = ol.trace(ol.oracle().xmco_trace, inp, 777, p, 50000, precision=64)
+        b = ol.trace(ol.oracle().xmco_trace, inp, 777, p, 50000, precision=32)
+        same += len(a) == len(b) and np.array_equal(a["type"], b["type"]) and np.array_equal(a["layer"], b["layer"])
+        n = np.linalg.norm(b["dir"], axis=1)
+        assert np.abs(n - 1).max() < 1e-5
+    assert same >= 36

@@ -140,8 +140,16 @@ class Stats(C.Structure):
         ("scatterEvents", C.c_uint64),
         ("boundaryEvents", C.c_uint64),
         ("rouletteEvents", C.c_uint64),
-        ("stackRestarts", C.c_uint64),
+        ("stackSpills", C.c_uint64),
         ("kernelMs", C.c_double),
+        ("wAbsorbed", C.c_double),
+        ("wEscaped", C.c_double),
+        ("wDetected", C.c_double),
+        ("wLeftArea", C.c_double),
+        ("wKilled", C.c_double),
+        ("wRouletteLost", C.c_double),
+        ("wRouletteGain", C.c_double),
+        ("wOther", C.c_double),
     ]
 
 
@@ -175,11 +183,10 @@ assert TRACE_DTYPE.itemsize == C.sizeof(TraceEvent)
 
 class Options(C.Structure):
     _fields_ = [
+        ("device", C.c_int),
         ("blockSize", C.c_int),
         ("blocksPerCU", C.c_int),
-        ("stackDepth", C.c_int),
-        ("absorptionTile", C.c_int),
-        ("device", C.c_int),
+        ("skipAbsorption", C.c_int),
     ]
 
 

@@ -1,0 +1,60 @@
+"""Builds the engine library (hipcc, gfx950) and the CPU oracle (g++).
+
+    python -m xmcml_amd.build           # library + oracle (+ reference build when present)
+
+The engine is one shared object, ``xmcml_amd/libxmcml_amd.so``, exporting
+the C-ABI of include/xmcml.h.  Every HIP translation unit is compiled with
+FMA contraction off (the float arithmetic is specified operation by
+operation) and without packed FP32 instructions.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+PKG = ROOT / "xmcml_amd"
+CSRC = PKG / "csrc"
+LIB = PKG / "libxmcml_amd.so"
+REFERENCE = Path("/root/reference")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("XMC_OFFLOAD_ARCH", "gfx950")
+SOURCES = ["xmc_transport.hip", "xmc_host.cpp", "xmc_bvh.cpp"]
+FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fno-fast-math", "-munsafe-fp-atomics",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _run(cmd, cwd=None):
+    print("+", " ".join(str(c) for c in cmd), flush=True)
+    subprocess.run([str(c) for c in cmd], cwd=cwd, check=True)
+
+
+def build_library() -> Path:
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, *FLAGS, f"-I{ROOT / 'include'}", *[CSRC / s for s in SOURCES], "-o", tmp]
+    _run(cmd)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> None:
+    _run(["make", "-s", "-C", ROOT / "oracle", "liboracle.so"])
+    if (REFERENCE / "base" / "xmcml" / "xmcml" / "mcml_kernel.cpp").exists():
+        _run(["make", "-s", "-C", ROOT / "oracle", "ref", f"REF={REFERENCE}"])
+
+
+def main(argv=None) -> int:
+    build_library()
+    build_oracle()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,697 @@
+// Host side of the C-ABI (include/xmcml.h): scene preparation, launches,
+// tally collection.  Replaces GenerateBVHTree + LaunchOMP
+// (base/xmcml/xmcml/mcml_intersection.cpp:316-344,
+//  base/xmcml/xmcmlLauncher/launcher_omp.cpp:10-163).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/xmcml.h"
+#include "xmc_bvh.h"
+#include "xmc_params.h"
+
+extern "C" {
+hipError_t xmc_launch_transport(const xmc::Params* p, int grid, int block, size_t lds, hipStream_t s, int trace);
+hipError_t xmc_launch_query(const xmc::QueryParams* q, const int* perm, int block, size_t lds, hipStream_t s);
+hipError_t xmc_launch_zero(unsigned long long* p, size_t n, hipStream_t s);
+hipError_t xmc_transport_occupancy(int block, size_t lds, int* blocks_per_cu);
+}
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail(XMC_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));       \
+    } while (0)
+
+constexpr double kPi = 3.14159265358979323846;
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        release();
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc((void**)&p, count * sizeof(T));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+uint64_t mcg_pow(uint64_t a, uint64_t e) {
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r *= a;
+        a *= a;
+        e >>= 1;
+    }
+    return r;
+}
+
+}  // namespace
+
+struct xmc_scene {
+    int device = 0;
+    int n_layers = 0, n_surfaces = 0, n_det = 0, T = 0, grid = 0;
+    int block = 256, blocks = 0;
+    bool absorption = true;
+    double rsp = 0.0;
+    hipStream_t stream = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;  // transport launches not yet timed
+    double kernel_ms = 0.0;
+    xmc::RefBvh ref;
+    std::vector<int> perm;  // (surface, index) order -> leaf-order position
+    DevBuf<xmc::GpuNode> nodes;
+    DevBuf<xmc::GpuTri> tris;
+    DevBuf<int8_t> next_layer;
+    DevBuf<float> cube, ring;
+    DevBuf<uint32_t> filter;
+    DevBuf<double> t_start, t_finish;
+    // tallies: one 64-bit slab, views below
+    DevBuf<unsigned long long> slab;
+    size_t off_abs = 0, off_dw = 0, off_dt = 0, off_dc = 0, off_bw = 0, off_bt = 0, off_bc = 0, off_st = 0,
+           off_q = 0, slab_words = 0;
+    DevBuf<int2> spill;
+    DevBuf<int> perm_dev;
+    xmc::Params base;
+    std::vector<double> h_t_start, h_t_finish;
+};
+
+namespace {
+
+int validate(const xmc_input* in) {
+    if (!in) return fail(XMC_ERR_ARG, "input is NULL");
+    if (in->numberOfLayers < 2 || in->numberOfLayers > XMC_MAX_LAYERS)
+        return fail(XMC_ERR_LIMIT, "numberOfLayers must be in [2, 8] (MAX_LAYERS)");
+    if (!in->layerInfo || !in->area) return fail(XMC_ERR_ARG, "layerInfo / area is NULL");
+    const xmc_layer& l1 = in->layerInfo[1];
+    if (l1.absorptionCoefficient == 0.0 && l1.scatteringCoefficient == 0.0 && in->numberOfLayers < 3)
+        return fail(XMC_ERR_ARG, "a glass layer 1 needs a layer 2 (ComputeSpecularReflectance)");
+    const xmc_int3& p = in->area->partitionNumber;
+    if (p.x < 1 || p.y < 1 || p.z < 1 || p.x > 255 || p.y > 255 || p.z > 255)
+        return fail(XMC_ERR_LIMIT, "area partitions must be in [1, 255] per axis (byte3 voxel indices)");
+    if (!(in->area->length.x > 0 && in->area->length.y > 0 && in->area->length.z > 0))
+        return fail(XMC_ERR_ARG, "area lengths must be positive");
+    if (in->numberOfSurfaces < 1 || !in->surface) return fail(XMC_ERR_ARG, "no surfaces");
+    if (in->numberOfSurfaces > 127) return fail(XMC_ERR_LIMIT, "at most 127 surfaces");
+    for (int i = 0; i < in->numberOfLayers; i++) {
+        const xmc_layer& l = in->layerInfo[i];
+        if (l.numberOfSurfaces > 0 && !l.surfaceId) return fail(XMC_ERR_ARG, "layer surfaceId is NULL");
+        for (int j = 0; j < l.numberOfSurfaces; j++)
+            if (l.surfaceId[j] < 0 || l.surfaceId[j] >= in->numberOfSurfaces)
+                return fail(XMC_ERR_ARG, "layer surfaceId out of range");
+        if (!(l.refractiveIndex > 0)) return fail(XMC_ERR_ARG, "refractive index must be positive");
+    }
+    if (in->timeScaleSize < 1) return fail(XMC_ERR_ARG, "timeScaleSize must be >= 1");
+    if (in->numberOfCubeDetectors < 0 || in->numberOfRingDetectors < 0)
+        return fail(XMC_ERR_ARG, "negative detector count");
+    if (in->numberOfCubeDetectors > 0 && !in->cubeDetector) return fail(XMC_ERR_ARG, "cubeDetector is NULL");
+    if (in->numberOfRingDetectors > 0 && !in->ringDetector) return fail(XMC_ERR_ARG, "ringDetector is NULL");
+    if (in->terminationMode != XMC_TERMINATE_KILL && in->terminationMode != XMC_TERMINATE_ROULETTE)
+        return fail(XMC_ERR_ARG, "unknown terminationMode");
+    if (in->terminationMode == XMC_TERMINATE_ROULETTE && !(in->rouletteChance > 0 && in->rouletteChance <= 1))
+        return fail(XMC_ERR_ARG, "rouletteChance must be in (0, 1]");
+    return XMC_OK;
+}
+
+// dynamic LDS of the transport kernel: short stacks, then the block's ledger
+size_t transport_lds(int block) { return (size_t)xmc::kStackLds * block * sizeof(int2) + 8 * sizeof(double); }
+
+// critical cosine in float arithmetic (GetCriticalCos, mcml_kernel.cpp:644-661)
+float crit_cos(float n1, float n2) {
+    if (n1 > n2) return std::sqrt(1.0f - n2 * n2 / (n1 * n1));
+    return 0.0f;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* xmc_last_error(void) { return g_error.c_str(); }
+const char* xmc_version(void) { return "xmcml_amd 0.1 gfx950"; }
+
+double xmc_specular_reflectance(const xmc_layer* layer) {  // mcml_kernel.cpp:16-37
+    double t = (layer[0].refractiveIndex - layer[1].refractiveIndex) /
+               (layer[0].refractiveIndex + layer[1].refractiveIndex);
+    double r1 = t * t;
+    if (layer[1].absorptionCoefficient == 0.0 && layer[1].scatteringCoefficient == 0.0) {
+        t = (layer[1].refractiveIndex - layer[2].refractiveIndex) /
+            (layer[1].refractiveIndex + layer[2].refractiveIndex);
+        double r2 = t * t;
+        t = 1 - r1;
+        r1 = r1 + t * t * r2 / (1 - r1 * r2);
+    }
+    return r1;
+}
+
+int xmc_scene_create(const xmc_input* in, const xmc_options* opt, xmc_scene** out) {
+    if (!out) return fail(XMC_ERR_ARG, "scene pointer is NULL");
+    *out = nullptr;
+    int rc = validate(in);
+    if (rc) return rc;
+    xmc_options o;
+    std::memset(&o, 0, sizeof(o));
+    if (opt) o = *opt;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(XMC_ERR_NOGPU, "no HIP device");
+    if (o.device < 0 || o.device >= ndev) return fail(XMC_ERR_ARG, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(o.device));
+
+    auto* sc = new xmc_scene();
+    std::unique_ptr<xmc_scene> guard(sc);
+    sc->device = o.device;
+    sc->n_layers = in->numberOfLayers;
+    sc->n_surfaces = in->numberOfSurfaces;
+    sc->n_det = in->numberOfCubeDetectors + in->numberOfRingDetectors;
+    sc->T = in->timeScaleSize;
+    const xmc_int3& part = in->area->partitionNumber;
+    sc->grid = part.x * part.y * part.z;
+    sc->absorption = o.skipAbsorption == 0;
+    sc->rsp = xmc_specular_reflectance(in->layerInfo);
+
+    // BVH (GenerateBVHTree) and its GPU layout
+    const char* err = nullptr;
+    if (!xmc::build_reference_bvh(in->surface, in->numberOfSurfaces, sc->ref, &err))
+        return fail(XMC_ERR_ARG, std::string("BVH: ") + (err ? err : "failed"));
+    xmc::GpuBvh gb;
+    xmc::flatten_bvh(sc->ref, in->surface, gb);
+    size_t nt = gb.tris.size();
+    {
+        std::vector<int> base(in->numberOfSurfaces + 1, 0);
+        for (int i = 0; i < in->numberOfSurfaces; i++) base[i + 1] = base[i] + in->surface[i].numberOfTriangles;
+        sc->perm.assign(nt, 0);
+        for (size_t k = 0; k < nt; k++) sc->perm[base[sc->ref.tri_surface[k]] + sc->ref.tri_index[k]] = (int)k;
+    }
+    HIP_TRY(sc->nodes.alloc(gb.nodes.size() > 0 ? gb.nodes.size() : 1));
+    if (!gb.nodes.empty())
+        HIP_TRY(hipMemcpy(sc->nodes.p, gb.nodes.data(), gb.nodes.size() * sizeof(xmc::GpuNode), hipMemcpyHostToDevice));
+    HIP_TRY(sc->tris.alloc(nt));
+    HIP_TRY(hipMemcpy(sc->tris.p, gb.tris.data(), nt * sizeof(xmc::GpuTri), hipMemcpyHostToDevice));
+    HIP_TRY(sc->perm_dev.alloc(nt));
+    HIP_TRY(hipMemcpy(sc->perm_dev.p, sc->perm.data(), nt * sizeof(int), hipMemcpyHostToDevice));
+
+    xmc::Params& P = sc->base;
+    std::memset(&P, 0, sizeof(P));
+    P.nodes = sc->nodes.p;
+    P.tris = sc->tris.p;
+    P.root_ref = gb.root_ref;
+    for (int a = 0; a < 3; a++) {
+        P.root_lo[a] = gb.root_lo[a];
+        P.root_hi[a] = gb.root_hi[a];
+    }
+
+    // layer boundary table: FindIntersectionLayer (mcml_kernel.cpp:566-580)
+    std::vector<int8_t> nl((size_t)sc->n_layers * sc->n_surfaces, -1);
+    for (int l = 0; l < sc->n_layers; l++)
+        for (int s = 0; s < sc->n_surfaces; s++) {
+            int found = -1;
+            for (int i = 0; i < in->numberOfLayers && found < 0; ++i)
+                for (int j = 0; j < in->layerInfo[i].numberOfSurfaces; ++j)
+                    if (in->layerInfo[i].surfaceId[j] == s && i != l) {
+                        found = i;
+                        break;
+                    }
+            nl[(size_t)l * sc->n_surfaces + s] = (int8_t)found;
+        }
+    HIP_TRY(sc->next_layer.alloc(nl.size()));
+    HIP_TRY(hipMemcpy(sc->next_layer.p, nl.data(), nl.size(), hipMemcpyHostToDevice));
+    P.next_layer = sc->next_layer.p;
+    P.n_surfaces = sc->n_surfaces;
+
+    for (int l = 0; l < XMC_MAX_LAYERS; l++) {
+        if (l < sc->n_layers) {
+            const xmc_layer& L = in->layerInfo[l];
+            P.layers.n[l] = (float)L.refractiveIndex;
+            P.layers.mua[l] = (float)L.absorptionCoefficient;
+            P.layers.mus[l] = (float)L.scatteringCoefficient;
+            P.layers.g[l] = (float)L.anisotropy;
+            if (L.absorptionCoefficient == 0.0 && L.scatteringCoefficient == 0.0) P.layers.glass_mask |= 1u << l;
+        } else {
+            P.layers.n[l] = 1.0f;
+        }
+        if (in->targetRangeLayers[l]) P.layers.target_mask |= 1u << l;
+    }
+    for (int a = 0; a < XMC_MAX_LAYERS; a++)
+        for (int b = 0; b < XMC_MAX_LAYERS; b++) P.layers.crit[a][b] = crit_cos(P.layers.n[a], P.layers.n[b]);
+    P.layers.count = sc->n_layers;
+
+    P.area.corner[0] = (float)in->area->corner.x;
+    P.area.corner[1] = (float)in->area->corner.y;
+    P.area.corner[2] = (float)in->area->corner.z;
+    P.area.length[0] = (float)in->area->length.x;
+    P.area.length[1] = (float)in->area->length.y;
+    P.area.length[2] = (float)in->area->length.z;
+    P.area.part[0] = part.x;
+    P.area.part[1] = part.y;
+    P.area.part[2] = part.z;
+    for (int a = 0; a < 3; a++) P.area.partf[a] = (float)P.area.part[a];
+
+    // detectors (cube bounds in float arithmetic as the float walk evaluates them)
+    std::vector<float> cube, ring;
+    std::vector<uint32_t> filt;
+    for (int i = 0; i < in->numberOfCubeDetectors; i++) {
+        const xmc_cube_detector& c = in->cubeDetector[i];
+        float cx = (float)c.center.x, cy = (float)c.center.y, cz = (float)c.center.z;
+        float hx = (float)c.length.x / 2.0f, hy = (float)c.length.y / 2.0f, hz = (float)c.length.z / 2.0f;
+        cube.insert(cube.end(), {cx - hx, cy - hy, cz - hz, cx + hx, cy + hy, cz + hz,
+                                 (float)std::cos(c.targetAngle * kPi / 180.0)});
+        uint32_t m = 0;
+        for (int j = 0; j < XMC_MAX_LAYERS; j++)
+            if (c.filterLayers[j]) m |= 1u << j;
+        filt.push_back(m);
+    }
+    for (int i = 0; i < in->numberOfRingDetectors; i++) {
+        const xmc_ring_detector& r = in->ringDetector[i];
+        ring.insert(ring.end(), {(float)r.center.x, (float)r.center.y, (float)r.smallRadius, (float)r.bigRadius,
+                                 (float)std::cos(r.targetAngle * kPi / 180.0)});
+        uint32_t m = 0;
+        for (int j = 0; j < XMC_MAX_LAYERS; j++)
+            if (r.filterLayers[j]) m |= 1u << j;
+        filt.push_back(m);
+    }
+    HIP_TRY(sc->cube.alloc(cube.size() ? cube.size() : 1));
+    HIP_TRY(sc->ring.alloc(ring.size() ? ring.size() : 1));
+    HIP_TRY(sc->filter.alloc(filt.size() ? filt.size() : 1));
+    if (!cube.empty()) HIP_TRY(hipMemcpy(sc->cube.p, cube.data(), cube.size() * 4, hipMemcpyHostToDevice));
+    if (!ring.empty()) HIP_TRY(hipMemcpy(sc->ring.p, ring.data(), ring.size() * 4, hipMemcpyHostToDevice));
+    if (!filt.empty()) HIP_TRY(hipMemcpy(sc->filter.p, filt.data(), filt.size() * 4, hipMemcpyHostToDevice));
+    P.det.n_cube = in->numberOfCubeDetectors;
+    P.det.n_ring = in->numberOfRingDetectors;
+    P.det.cube = sc->cube.p;
+    P.det.ring = sc->ring.p;
+    P.det.filter = sc->filter.p;
+
+    // time bins exactly as InitOutput lays them out (launcher_omp.cpp:38-55)
+    int T = sc->T;
+    double dt = (in->timeFinish - in->timeStart) / T;
+    if (dt < 0.0) dt = 0.0;
+    sc->h_t_start.assign(T, 0.0);
+    sc->h_t_finish.assign(T, 0.0);
+    for (int j = 0; j < T - 1; j++) {
+        sc->h_t_start[j] = in->timeStart + j * dt;
+        sc->h_t_finish[j] = sc->h_t_start[j] + dt;
+    }
+    sc->h_t_start[T - 1] = in->timeStart + (T - 1) * dt;
+    sc->h_t_finish[T - 1] = in->timeFinish;
+    HIP_TRY(sc->t_start.alloc(T));
+    HIP_TRY(sc->t_finish.alloc(T));
+    HIP_TRY(hipMemcpy(sc->t_start.p, sc->h_t_start.data(), T * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(sc->t_finish.p, sc->h_t_finish.data(), T * 8, hipMemcpyHostToDevice));
+    P.det.T = T;
+    P.det.t0 = in->timeStart;
+    P.det.dt = dt;
+    P.det.t_start = sc->t_start.p;
+    P.det.t_finish = sc->t_finish.p;
+
+    // tallies in one zeroed slab of 64-bit words
+    size_t w = 0;
+    sc->off_abs = w;
+    w += sc->absorption ? (size_t)sc->grid : 0;
+    sc->off_dw = w;
+    w += sc->n_det;
+    sc->off_dt = w;
+    w += sc->n_det;
+    sc->off_dc = w;
+    w += sc->n_det;
+    sc->off_bw = w;
+    w += (size_t)sc->n_det * T;
+    sc->off_bt = w;
+    w += (size_t)sc->n_det * T;
+    sc->off_bc = w;
+    w += (size_t)sc->n_det * T;
+    sc->off_st = w;
+    w += 16;
+    sc->off_q = w;
+    w += 1;
+    sc->slab_words = w;
+    HIP_TRY(sc->slab.alloc(w));
+    HIP_TRY(hipMemset(sc->slab.p, 0, w * 8));
+    unsigned long long* s = sc->slab.p;
+    P.tally.absorption = sc->absorption ? reinterpret_cast<double*>(s + sc->off_abs) : nullptr;
+    P.tally.det_weight = reinterpret_cast<double*>(s + sc->off_dw);
+    P.tally.det_target = reinterpret_cast<double*>(s + sc->off_dt);
+    P.tally.det_count = s + sc->off_dc;
+    P.tally.bin_weight = reinterpret_cast<double*>(s + sc->off_bw);
+    P.tally.bin_target = reinterpret_cast<double*>(s + sc->off_bt);
+    P.tally.bin_count = s + sc->off_bc;
+    P.tally.stats = s + sc->off_st;
+    P.queue = s + sc->off_q;
+
+    // photons and streams
+    const uint64_t jump0 = mcg_pow(xmc::kMcgA, 1ull << xmc::kStreamLog2) & xmc::kMcgMask;
+    uint64_t j = jump0;
+    for (int i = 0; i < xmc::kJumpBits; i++) {
+        P.jump[i] = j;
+        j = (j * j) & xmc::kMcgMask;
+    }
+    P.start_pos[0] = (float)in->startPosition.x;
+    P.start_pos[1] = (float)in->startPosition.y;
+    P.start_pos[2] = (float)in->startPosition.z;
+    P.start_dir[0] = (float)in->startDirection.x;
+    P.start_dir[1] = (float)in->startDirection.y;
+    P.start_dir[2] = (float)in->startDirection.z;
+    P.w0 = (float)(1.0 - sc->rsp);
+    P.min_weight = (float)in->minWeight;
+    P.chance = (float)in->rouletteChance;
+    P.roulette = in->terminationMode == XMC_TERMINATE_ROULETTE;
+
+    // persistent launch shape
+    sc->block = o.blockSize > 0 ? o.blockSize : 256;
+    if (sc->block % 64 != 0 || sc->block > 256) return fail(XMC_ERR_ARG, "blockSize must be 64, 128, 192 or 256");
+    size_t lds = transport_lds(sc->block);
+    int per_cu = 0;
+    HIP_TRY(xmc_transport_occupancy(sc->block, lds, &per_cu));
+    if (per_cu < 1) per_cu = 1;
+    if (o.blocksPerCU > 0 && o.blocksPerCU < per_cu) per_cu = o.blocksPerCU;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, sc->device));
+    sc->blocks = prop.multiProcessorCount * per_cu;
+    // stack spill: the walk holds at most depth-1 entries
+    int need = sc->ref.depth - xmc::kStackLds;
+    P.spill_depth = need > 0 ? need : 0;
+    size_t lanes = (size_t)sc->blocks * sc->block;
+    HIP_TRY(sc->spill.alloc(P.spill_depth > 0 ? (size_t)P.spill_depth * lanes : 1));
+    P.spill = sc->spill.p;
+
+    HIP_TRY(hipStreamCreate(&sc->stream));
+    *out = guard.release();
+    return XMC_OK;
+}
+
+int xmc_scene_destroy(xmc_scene* sc) {
+    if (!sc) return XMC_OK;
+    (void)hipSetDevice(sc->device);
+    if (sc->stream) (void)hipStreamSynchronize(sc->stream);
+    for (auto& e : sc->pending) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    if (sc->stream) (void)hipStreamDestroy(sc->stream);
+    delete sc;
+    return XMC_OK;
+}
+
+static int run_impl(xmc_scene* sc, uint64_t seed, uint64_t first, uint64_t n, hipStream_t s, const xmc::Trace* tr) {
+    if (n == 0) return XMC_OK;
+    if (first + n < first || first + n > (1ull << xmc::kJumpBits))
+        return fail(XMC_ERR_LIMIT, "photon index beyond 2^40");
+    HIP_TRY(hipSetDevice(sc->device));
+    xmc::Params P = sc->base;
+    P.first_photon = first;
+    P.n_photons = n;
+    P.seed_value = (2 * seed + 1) & xmc::kMcgMask;
+    if (tr) P.trace = *tr;
+    HIP_TRY(xmc_launch_zero(P.queue, 1, s));
+    size_t lds = transport_lds(sc->block);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    sc->pending.emplace_back(e0, e1);
+    HIP_TRY(hipEventRecord(e0, s));
+    HIP_TRY(xmc_launch_transport(&P, sc->blocks, sc->block, lds, s, tr != nullptr));
+    HIP_TRY(hipEventRecord(e1, s));
+    return XMC_OK;
+}
+
+int xmc_scene_run(xmc_scene* sc, uint64_t seed, uint64_t first, uint64_t n, void* stream) {
+    if (!sc) return fail(XMC_ERR_ARG, "scene is NULL");
+    hipStream_t s = stream ? (hipStream_t)stream : sc->stream;
+    return run_impl(sc, seed, first, n, s, nullptr);
+}
+
+int xmc_scene_sync(xmc_scene* sc) {
+    if (!sc) return fail(XMC_ERR_ARG, "scene is NULL");
+    HIP_TRY(hipSetDevice(sc->device));
+    HIP_TRY(hipDeviceSynchronize());
+    return XMC_OK;
+}
+
+int xmc_scene_collect(xmc_scene* sc, xmc_output* out) {
+    if (!sc || !out) return fail(XMC_ERR_ARG, "scene/output is NULL");
+    HIP_TRY(hipSetDevice(sc->device));
+    HIP_TRY(hipStreamSynchronize(sc->stream));
+    HIP_TRY(hipDeviceSynchronize());
+    if (out->numberOfDetectors != sc->n_det) return fail(XMC_ERR_ARG, "output detector count does not match input");
+    std::vector<unsigned long long> h(sc->slab_words);
+    HIP_TRY(hipMemcpy(h.data(), sc->slab.p, sc->slab_words * 8, hipMemcpyDeviceToHost));
+    auto dv = [&](size_t off) { return reinterpret_cast<const double*>(h.data() + off); };
+    if (sc->absorption && out->absorption) {
+        if (out->gridSize != sc->grid) return fail(XMC_ERR_ARG, "output gridSize does not match area");
+        const double* a = dv(sc->off_abs);
+        for (int i = 0; i < sc->grid; i++) out->absorption[i] += a[i];
+    }
+    for (int d = 0; d < sc->n_det; d++) {
+        xmc_detector_info& di = out->detectorInfo[d];
+        di.weight += dv(sc->off_dw)[d];
+        di.targetWeight += dv(sc->off_dt)[d];
+        di.numberOfPhotons += h[sc->off_dc + d];
+        for (int b = 0; b < sc->T && b < di.timeScaleSize; b++) {
+            size_t k = (size_t)d * sc->T + b;
+            di.timeScale[b].weight += dv(sc->off_bw)[k];
+            di.timeScale[b].targetWeight += dv(sc->off_bt)[k];
+            di.timeScale[b].numberOfPhotons += h[sc->off_bc + k];
+        }
+    }
+    out->specularReflectance = sc->rsp;
+    // zero everything except the counters of xmc_scene_stats
+    HIP_TRY(xmc_launch_zero(sc->slab.p, sc->off_st, sc->stream));
+    HIP_TRY(hipStreamSynchronize(sc->stream));
+    return XMC_OK;
+}
+
+int xmc_scene_stats(xmc_scene* sc, xmc_stats* st) {
+    if (!sc || !st) return fail(XMC_ERR_ARG, "scene/stats is NULL");
+    HIP_TRY(hipSetDevice(sc->device));
+    HIP_TRY(hipStreamSynchronize(sc->stream));
+    HIP_TRY(hipDeviceSynchronize());
+    for (auto& e : sc->pending) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        sc->kernel_ms += ms;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    sc->pending.clear();
+    unsigned long long c[16];
+    HIP_TRY(hipMemcpy(c, sc->slab.p + sc->off_st, sizeof(c), hipMemcpyDeviceToHost));
+    st->photons = c[0];
+    st->scatterEvents = c[1];
+    st->boundaryEvents = c[2];
+    st->rouletteEvents = c[3];
+    st->stackSpills = c[4];
+    st->kernelMs = sc->kernel_ms;
+    double led[8];
+    std::memcpy(led, c + 8, sizeof(led));
+    st->wAbsorbed = led[0];
+    st->wEscaped = led[1];
+    st->wDetected = led[2];
+    st->wLeftArea = led[3];
+    st->wKilled = led[4];
+    st->wRouletteLost = led[5];
+    st->wRouletteGain = led[6];
+    st->wOther = led[7];
+    HIP_TRY(xmc_launch_zero(sc->slab.p + sc->off_st, 16, sc->stream));
+    HIP_TRY(hipStreamSynchronize(sc->stream));
+    sc->kernel_ms = 0.0;
+    return XMC_OK;
+}
+
+int xmc_scene_trace(xmc_scene* sc, uint64_t seed, uint64_t first, uint64_t n, xmc_trace_event* events,
+                    int max_events, int* counts) {
+    if (!sc || !events || !counts || max_events <= 0) return fail(XMC_ERR_ARG, "bad trace arguments");
+    HIP_TRY(hipSetDevice(sc->device));
+    DevBuf<xmc_trace_event> ev;
+    DevBuf<int> cnt;
+    HIP_TRY(ev.alloc(n * (size_t)max_events));
+    HIP_TRY(cnt.alloc(n));
+    xmc::Trace tr;
+    tr.events = ev.p;
+    tr.counts = cnt.p;
+    tr.max_events = max_events;
+    tr.first = first;
+    tr.n = n;
+    int rc = run_impl(sc, seed, first, n, sc->stream, &tr);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(sc->stream));
+    HIP_TRY(hipMemcpy(events, ev.p, n * (size_t)max_events * sizeof(xmc_trace_event), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(counts, cnt.p, n * sizeof(int), hipMemcpyDeviceToHost));
+    return XMC_OK;
+}
+
+int xmc_scene_query(xmc_scene* sc, const double* o, const double* d, const double* steps, int nr, int brute,
+                    int* found, double* dist, int* surf, int* tri, double* normal) {
+    if (!sc || nr < 0) return fail(XMC_ERR_ARG, "bad query arguments");
+    if (nr == 0) return XMC_OK;
+    HIP_TRY(hipSetDevice(sc->device));
+    std::vector<float> ho(3 * (size_t)nr), hd(3 * (size_t)nr), hs(nr);
+    for (size_t i = 0; i < 3 * (size_t)nr; i++) {
+        ho[i] = (float)o[i];
+        hd[i] = (float)d[i];
+    }
+    for (int i = 0; i < nr; i++) hs[i] = (float)steps[i];
+    DevBuf<float> fo, fd, fs, fdist, fn;
+    DevBuf<int> ff, fsurf, ftri;
+    HIP_TRY(fo.alloc(3 * (size_t)nr));
+    HIP_TRY(fd.alloc(3 * (size_t)nr));
+    HIP_TRY(fs.alloc(nr));
+    HIP_TRY(fdist.alloc(nr));
+    HIP_TRY(fn.alloc(3 * (size_t)nr));
+    HIP_TRY(ff.alloc(nr));
+    HIP_TRY(fsurf.alloc(nr));
+    HIP_TRY(ftri.alloc(nr));
+    HIP_TRY(hipMemcpy(fo.p, ho.data(), ho.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(fd.p, hd.data(), hd.size() * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(fs.p, hs.data(), hs.size() * 4, hipMemcpyHostToDevice));
+    int block = 256;
+    int grid = (nr + block - 1) / block;
+    int need = sc->ref.depth - xmc::kStackLds;
+    int sdepth = need > 0 ? need : 0;
+    DevBuf<int2> spill;
+    HIP_TRY(spill.alloc(sdepth > 0 ? (size_t)sdepth * grid * block : 1));
+    xmc::QueryParams Q;
+    std::memset(&Q, 0, sizeof(Q));
+    Q.nodes = sc->base.nodes;
+    Q.tris = sc->base.tris;
+    Q.root_ref = sc->base.root_ref;
+    for (int a = 0; a < 3; a++) {
+        Q.root_lo[a] = sc->base.root_lo[a];
+        Q.root_hi[a] = sc->base.root_hi[a];
+    }
+    Q.origins = fo.p;
+    Q.directions = fd.p;
+    Q.steps = fs.p;
+    Q.n = nr;
+    Q.brute = brute;
+    Q.n_tris = (int)sc->perm.size();
+    Q.found = ff.p;
+    Q.distance = fdist.p;
+    Q.surface = fsurf.p;
+    Q.triangle = ftri.p;
+    Q.normal = fn.p;
+    Q.spill = spill.p;
+    Q.spill_depth = sdepth;
+    HIP_TRY(xmc_launch_query(&Q, sc->perm_dev.p, block, (size_t)xmc::kStackLds * block * sizeof(int2), sc->stream));
+    HIP_TRY(hipStreamSynchronize(sc->stream));
+    std::vector<float> hdist(nr), hn(3 * (size_t)nr);
+    HIP_TRY(hipMemcpy(found, ff.p, nr * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(surf, fsurf.p, nr * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(tri, ftri.p, nr * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hdist.data(), fdist.p, nr * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(hn.data(), fn.p, hn.size() * 4, hipMemcpyDeviceToHost));
+    for (int i = 0; i < nr; i++) dist[i] = hdist[i];
+    for (size_t i = 0; i < hn.size(); i++) normal[i] = hn[i];
+    return XMC_OK;
+}
+
+int xmc_scene_bvh(xmc_scene* sc, int* nn, int* root, int* nt, double* boxes, int* links, int* tsurf, int* tidx) {
+    if (!sc || !nn || !root || !nt) return fail(XMC_ERR_ARG, "bad bvh arguments");
+    *nn = (int)sc->ref.nodes.size();
+    *root = sc->ref.root;
+    *nt = (int)sc->ref.tri_surface.size();
+    if (boxes && links) {
+        for (size_t i = 0; i < sc->ref.nodes.size(); i++) {
+            const xmc::RefNode& n = sc->ref.nodes[i];
+            for (int a = 0; a < 3; a++) {
+                boxes[6 * i + a] = n.lo[a];
+                boxes[6 * i + 3 + a] = n.hi[a];
+            }
+            links[4 * i] = n.left;
+            links[4 * i + 1] = n.right;
+            links[4 * i + 2] = n.offset;
+            links[4 * i + 3] = n.count;
+        }
+    }
+    if (tsurf && tidx)
+        for (size_t i = 0; i < sc->ref.tri_surface.size(); i++) {
+            tsurf[i] = sc->ref.tri_surface[i];
+            tidx[i] = sc->ref.tri_index[i];
+        }
+    return XMC_OK;
+}
+
+int xmc_launch(const xmc_input* in, xmc_output* out, xmc_rng* rng, const xmc_options* opt) {
+    if (!in || !out || !rng) return fail(XMC_ERR_ARG, "input/output/rng is NULL");
+    xmc_options o;
+    std::memset(&o, 0, sizeof(o));
+    if (opt) o = *opt;
+    if (!out->absorption) o.skipAbsorption = 1;
+    xmc_scene* sc = nullptr;
+    int rc = xmc_scene_create(in, &o, &sc);
+    if (rc) return rc;
+    rc = xmc_scene_run(sc, rng->seed, rng->nextPhoton, in->numberOfPhotons, nullptr);
+    if (rc == XMC_OK) rc = xmc_scene_collect(sc, out);
+    if (rc == XMC_OK) rng->nextPhoton += in->numberOfPhotons;
+    xmc_scene_destroy(sc);
+    return rc;
+}
+
+int xmc_init_output(const xmc_input* in, xmc_output* out) {  // InitOutput, launcher_omp.cpp:10-58
+    if (!in || !out || !in->area) return fail(XMC_ERR_ARG, "input/output is NULL");
+    if (in->timeScaleSize < 1) return fail(XMC_ERR_ARG, "timeScaleSize must be >= 1");
+    std::memset(out, 0, sizeof(*out));
+    const xmc_int3& p = in->area->partitionNumber;
+    int G = p.x * p.y * p.z;
+    out->gridSize = G;
+    out->absorption = (double*)calloc(G, sizeof(double));
+    out->scattering = (double*)calloc(G, sizeof(double));
+    int nd = in->numberOfCubeDetectors + in->numberOfRingDetectors;
+    out->numberOfDetectors = nd;
+    out->detectorInfo = (xmc_detector_info*)calloc(nd > 0 ? nd : 1, sizeof(xmc_detector_info));
+    if (!out->absorption || !out->scattering || !out->detectorInfo) return fail(XMC_ERR_ARG, "out of host memory");
+    int T = in->timeScaleSize;
+    double dt = (in->timeFinish - in->timeStart) / T;
+    if (dt < 0.0) dt = 0.0;
+    for (int d = 0; d < nd; d++) {
+        xmc_detector_info& di = out->detectorInfo[d];
+        di.trajectorySize = G;
+        di.trajectory = nullptr;  // not produced by the GPU path
+        di.timeScaleSize = T;
+        di.timeScale = (xmc_time_info*)calloc(T, sizeof(xmc_time_info));
+        if (!di.timeScale) return fail(XMC_ERR_ARG, "out of host memory");
+        for (int j = 0; j < T - 1; j++) {
+            di.timeScale[j].timeStart = in->timeStart + j * dt;
+            di.timeScale[j].timeFinish = di.timeScale[j].timeStart + dt;
+        }
+        di.timeScale[T - 1].timeStart = in->timeStart + (T - 1) * dt;
+        di.timeScale[T - 1].timeFinish = in->timeFinish;
+    }
+    return XMC_OK;
+}
+
+void xmc_free_output(xmc_output* out) {
+    if (!out) return;
+    free(out->absorption);
+    free(out->scattering);
+    if (out->detectorInfo) {
+        for (int d = 0; d < out->numberOfDetectors; d++) {
+            free(out->detectorInfo[d].timeScale);
+            free(out->detectorInfo[d].trajectory);
+        }
+        free(out->detectorInfo);
+    }
+    std::memset(out, 0, sizeof(*out));
+}
+
+}  // extern "C"

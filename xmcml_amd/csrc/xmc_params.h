@@ -1,0 +1,117 @@
+// Launch parameters of the transport and query kernels (host and device).
+#pragma once
+
+#include <cstdint>
+
+#include "../../include/xmcml.h"
+#include "xmc_bvh.h"
+
+namespace xmc {
+
+constexpr int kMaxLayers = XMC_MAX_LAYERS;
+constexpr int kStreamLog2 = 24;        // MCG59 draws reserved per photon
+constexpr int kJumpBits = 40;          // photons < 2^40 per seed
+constexpr uint64_t kMcgA = 302875106592253ULL;     // 13^13, mcml_mcg59.cpp:3
+constexpr uint64_t kMcgMask = 576460752303423487ULL;  // 2^59 - 1
+constexpr int kStackLds = 8;           // LDS entries of the per-lane short stack
+// A photon whose stream block is nearly used up is retired (its weight goes
+// to the "other" ledger entry): drawing past 2^24 would reuse the next
+// photon's numbers.  One interaction draws at most 4 numbers plus redraws.
+constexpr uint32_t kDrawLimit = (1u << kStreamLog2) - 64u;
+
+struct Layers {
+    float n[kMaxLayers];
+    float mua[kMaxLayers];
+    float mus[kMaxLayers];
+    float g[kMaxLayers];
+    float crit[kMaxLayers][kMaxLayers];  // critical cosine for (from, to)
+    uint32_t glass_mask;                  // bit l: layer l has mua == mus == 0
+    uint32_t target_mask;                 // bit l: targetRangeLayers[l]
+    int count;
+};
+
+struct Detectors {
+    int n_cube, n_ring;
+    const float* cube;  // 7 per cube: lo.xyz, hi.xyz (center -/+ length/2), cos target
+    const float* ring;  // 5 per ring: center.xy, rs, rb, cos target
+    const uint32_t* filter;  // per detector (cubes first): bit l = filterLayers[l]
+    int T;                   // time bins per detector
+    double t0, dt;
+    const double* t_start;   // T
+    const double* t_finish;  // T
+};
+
+struct Tallies {
+    double* absorption;      // gridSize, or null
+    double* det_weight;      // per detector
+    double* det_target;
+    unsigned long long* det_count;
+    double* bin_weight;      // ndet * T
+    double* bin_target;
+    unsigned long long* bin_count;
+    unsigned long long* stats;  // 5 event counters, then 8 ledger doubles from word 8
+};
+
+struct Area {
+    float corner[3];
+    float length[3];
+    float partf[3];
+    int part[3];
+};
+
+struct Trace {
+    xmc_trace_event* events;
+    int* counts;
+    int max_events;
+    uint64_t first;  // photons [first, first + n) are traced
+    uint64_t n;
+};
+
+struct Params {
+    const GpuNode* nodes;
+    const GpuTri* tris;
+    int root_ref;
+    float root_lo[3], root_hi[3];
+    const int8_t* next_layer;  // [layer * n_surfaces + surface] -> layer or -1
+    int n_surfaces;
+    Layers layers;
+    Area area;
+    Detectors det;
+    Tallies tally;
+    // photons and streams
+    uint64_t first_photon;
+    uint64_t n_photons;
+    unsigned long long* queue;  // next photon to hand out (relative to first)
+    uint64_t seed_value;        // (2 seed + 1) mod 2^59
+    uint64_t jump[kJumpBits];   // (a^(2^24))^(2^i) mod 2^59
+    float start_pos[3], start_dir[3];
+    float w0;
+    float min_weight, chance;
+    int roulette;
+    // short-stack spill (global) for entries beyond kStackLds
+    int2* spill;
+    int spill_depth;  // entries per lane in the spill
+    Trace trace;
+};
+
+struct QueryParams {
+    const GpuNode* nodes;
+    const GpuTri* tris;
+    int root_ref;
+    float root_lo[3], root_hi[3];
+    const float* origins;     // 3 per ray
+    const float* directions;  // 3 per ray
+    const float* steps;       // <= 0: unlimited
+    int n;
+    int brute;
+    int n_tris;
+    int* found;
+    float* distance;
+    int* surface;
+    int* triangle;
+    float* normal;
+    int2* spill;
+    int spill_depth;
+};
+
+}  // namespace xmc

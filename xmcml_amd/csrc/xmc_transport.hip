@@ -1,0 +1,862 @@
+// gfx950 kernels of the photon random-walk engine.
+//
+// transport_kernel is a persistent kernel: every lane carries one photon and,
+// when its photon dies, takes the next photon index from a device counter
+// (one atomic per wavefront per refill, ranks from mbcnt), so wavefronts stay
+// full while photon lifetimes differ by orders of magnitude.  Each lane runs
+// xmcml's ComputePhoton loop (base/xmcml/xmcml/mcml_kernel.cpp:39-126) in
+// float: step, nearest surface hit through the BVH, then either a Fresnel
+// boundary interaction or hop/drop/spin, then termination.  The arithmetic of
+// each formula follows the reference's expression order with no FMA
+// contraction, so that a float CPU restatement reproduces it bit for bit; the
+// float-only rules (24-bit uniforms, own log and sincos(2 pi u), accurate
+// 1-x^2, coordinate-scaled surface offsets) are stated where they apply.
+//
+// Random numbers: photon p of seed s owns the MCG59 block that starts at
+// (2s+1) * a^(p * 2^24) mod 2^59 (a = 13^13, mcml_mcg59.cpp:3-41); the lane
+// jumps there with a table of squarings and then draws x <- a x mod 2^59.
+//
+// BVH walk: ordered front-to-back traversal with the reference's visiting
+// and pruning rules (mcml_intersection.cpp:354-584) over nodes that hold both
+// child boxes; the per-lane stack keeps its top 8 entries in LDS and spills
+// deeper entries to a per-lane global area.
+#include <hip/hip_runtime.h>
+
+#include "xmc_params.h"
+
+namespace xmc {
+
+constexpr float kInf = __builtin_inff();
+constexpr float kEpsF = 1.0e-6f;                 // EPSILON
+constexpr float kCosZeroF = 1.0f - 1.0e-6f;      // float "cosine of zero"
+constexpr float kCos90F = 1.0e-12f;              // COSINE_OF_90D
+constexpr float kMinDistF = 1.0e-8f;             // MIN_DISTANCE
+
+// --------------------------------------------------------------------------
+// float special functions
+// log: x = 2^k (1+f), 1+f in [sqrt(2)/2, sqrt(2)); log(1+f) = 2 atanh(s),
+// s = f/(2+f) = f - (f^2/2 - s (f^2/2 + R)), R the atanh series to s^8.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float xlogf(float x) {
+    uint32_t ix = __float_as_uint(x);
+    int32_t t = (int32_t)(ix - 0x3f3504f3u);
+    int32_t k = t >> 23;
+    uint32_t im = ix - ((uint32_t)k << 23);
+    float m = __uint_as_float(im);
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float z = s * s;
+    float R = __builtin_fmaf(z, 2.0f / 9.0f, 2.0f / 7.0f);
+    R = __builtin_fmaf(z, R, 2.0f / 5.0f);
+    R = __builtin_fmaf(z, R, 2.0f / 3.0f);
+    R = z * R;
+    float hfsq = 0.5f * (f * f);
+    float l1p = f - (hfsq - s * (hfsq + R));
+    float kf = (float)k;
+    return __builtin_fmaf(kf, 0.693145751953125f, __builtin_fmaf(kf, 1.42860677e-06f, l1p));
+}
+
+// sin and cos of 2 pi u, u in [0, 1): exact quadrant split q = rint(4u),
+// theta = (4u - q) pi/2 in [-pi/4, pi/4], Taylor series to theta^9 / theta^10.
+__device__ __forceinline__ void xsincos2pi(float u, float& sp, float& cp) {
+    float u4 = 4.0f * u;
+    float q = __builtin_rintf(u4);
+    float r = u4 - q;
+    int iq = ((int)q) & 3;
+    float th = r * 1.57079637f;
+    float t2 = th * th;
+    float ps = __builtin_fmaf(t2, 1.0f / 362880.0f, -1.0f / 5040.0f);
+    ps = __builtin_fmaf(t2, ps, 1.0f / 120.0f);
+    ps = __builtin_fmaf(t2, ps, -1.0f / 6.0f);
+    float s = __builtin_fmaf(th * t2, ps, th);
+    float pc = __builtin_fmaf(t2, -1.0f / 3628800.0f, 1.0f / 40320.0f);
+    pc = __builtin_fmaf(t2, pc, -1.0f / 720.0f);
+    pc = __builtin_fmaf(t2, pc, 1.0f / 24.0f);
+    pc = __builtin_fmaf(t2, pc, -0.5f);
+    float c = __builtin_fmaf(t2, pc, 1.0f);
+    float ss = (iq & 1) ? c : s;
+    float cc = (iq & 1) ? s : c;
+    sp = (iq == 2 || iq == 3) ? -ss : ss;
+    cp = (iq == 1 || iq == 2) ? -cc : cc;
+}
+
+__device__ __forceinline__ float one_minus_sq(float x) { return (1.0f - x) * (1.0f + x); }
+
+// MCG59 draw: top 24 bits of the 59-bit state.
+__device__ __forceinline__ float next_u(uint64_t& x) {
+    x = (x * kMcgA) & kMcgMask;
+    return (float)(uint32_t)(x >> 35) * 0x1p-24f;
+}
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ V3 normalize(V3 a) {
+    float l = __builtin_sqrtf(dot(a, a));
+    return {a.x / l, a.y / l, a.z / l};
+}
+__device__ __forceinline__ float offset_scale(V3 p) {
+    float m = fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z)));
+    return __builtin_fmaf(m, 0x1p-20f, 0x1p-22f);
+}
+
+// --------------------------------------------------------------------------
+// geometry
+// --------------------------------------------------------------------------
+struct Ray {
+    V3 o, d, inv;
+    bool ax, ay, az;  // |d| > EPSILON per axis
+};
+
+__device__ __forceinline__ Ray make_ray(V3 o, V3 d) {
+    Ray r;
+    r.o = o;
+    r.d = d;
+    r.ax = fabsf(d.x) > kEpsF;
+    r.ay = fabsf(d.y) > kEpsF;
+    r.az = fabsf(d.z) > kEpsF;
+    r.inv.x = r.ax ? 1.0f / d.x : 0.0f;
+    r.inv.y = r.ay ? 1.0f / d.y : 0.0f;
+    r.inv.z = r.az ? 1.0f / d.z : 0.0f;
+    return r;
+}
+
+// IntersectAABB (mcml_intersection.cpp:11-42).  Slab tests of the three axes
+// without the early exit: tmin only grows and tmax only shrinks, so the
+// result and t_near are the same.
+__device__ __forceinline__ bool box_hit(const Ray& r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                        float& tnear) {
+    float tmin = -kInf, tmax = kInf;
+    bool ok = true;
+    if (r.ax) {
+        float t1 = (lx - r.o.x) * r.inv.x, t2 = (hx - r.o.x) * r.inv.x;
+        tmin = fmaxf(tmin, fminf(t1, t2));
+        tmax = fminf(tmax, fmaxf(t1, t2));
+    } else {
+        ok = ok && !((lx - r.o.x > 0.0f) || (hx - r.o.x < 0.0f));
+    }
+    if (r.ay) {
+        float t1 = (ly - r.o.y) * r.inv.y, t2 = (hy - r.o.y) * r.inv.y;
+        tmin = fmaxf(tmin, fminf(t1, t2));
+        tmax = fminf(tmax, fmaxf(t1, t2));
+    } else {
+        ok = ok && !((ly - r.o.y > 0.0f) || (hy - r.o.y < 0.0f));
+    }
+    if (r.az) {
+        float t1 = (lz - r.o.z) * r.inv.z, t2 = (hz - r.o.z) * r.inv.z;
+        tmin = fmaxf(tmin, fminf(t1, t2));
+        tmax = fminf(tmax, fmaxf(t1, t2));
+    } else {
+        ok = ok && !((lz - r.o.z > 0.0f) || (hz - r.o.z < 0.0f));
+    }
+    ok = ok && !(tmin > tmax) && !(tmax < 0.0f);
+    tnear = ok ? tmin : kInf;
+    return ok;
+}
+
+// GetTriangleIntersectionDistance (mcml_intersection.cpp:44-86) on the
+// precomputed edges; returns +inf for a miss.
+__device__ __forceinline__ float tri_distance(const GpuTri* tris, int i, const Ray& r, int& last) {
+    const float4* T = reinterpret_cast<const float4*>(tris) + 3 * i;
+    float4 A = T[0], B = T[1], C = T[2];
+    last = __float_as_int(C.w);
+    V3 a = {A.x, A.y, A.z}, e1 = {B.x, B.y, B.z}, e2 = {C.x, C.y, C.z};
+    V3 p = cross(r.d, e2);
+    float det = dot(e1, p);
+    if (det < kEpsF && det > -kEpsF) return kInf;
+    float inv = 1.0f / det;
+    V3 t = sub(r.o, a);
+    float u = dot(t, p) * inv;
+    if (u < 0.0f || u > 1.0f) return kInf;
+    V3 q = cross(t, e1);
+    float v = dot(r.d, q) * inv;
+    if (v < 0.0f || u + v > 1.0f) return kInf;
+    return dot(e2, q) * inv;
+}
+
+__device__ __forceinline__ V3 tri_normal(const GpuTri* tris, int i) {
+    const float4* T = reinterpret_cast<const float4*>(tris) + 3 * i;
+    float4 B = T[1], C = T[2];
+    V3 e1 = {B.x, B.y, B.z}, e2 = {C.x, C.y, C.z};
+    return normalize(cross(e1, e2));
+}
+
+// Per-lane stack: entries [max(0, sp-K), sp) live in LDS slots (i mod K),
+// older entries in the lane's spill area (layout [entry][lane]).
+struct Stack {
+    int2* lds;
+    int stride;
+    int tid;
+    int2* spill;
+    int sstride;
+    int lane;
+    int cap;
+    int sp;
+    uint32_t spills;
+
+    __device__ __forceinline__ void push(int ref, float t) {
+        if (sp >= kStackLds) {
+            int j = sp - kStackLds;
+            spills++;
+            if (j < cap) spill[(size_t)j * sstride + lane] = lds[(j % kStackLds) * stride + tid];
+        }
+        lds[(sp % kStackLds) * stride + tid] = make_int2(ref, __float_as_int(t));
+        sp++;
+    }
+    __device__ __forceinline__ int2 pop() {
+        sp--;
+        int2 e = lds[(sp % kStackLds) * stride + tid];
+        if (sp >= kStackLds) {
+            int j = sp - kStackLds;
+            if (j < cap) lds[(j % kStackLds) * stride + tid] = spill[(size_t)j * sstride + lane];
+        }
+        return e;
+    }
+};
+
+struct Geo {
+    const GpuNode* nodes;
+    const GpuTri* tris;
+    int root_ref;
+    float rlo[3], rhi[3];
+};
+
+struct Hit {
+    float dist;
+    int tri;
+    bool found;
+};
+
+// ComputeBVHIntersection (limited = true, :354-466) or
+// ComputeBVHIntersectionWithoutStep (:468-584).
+__device__ Hit walk(const Geo& g, const Ray& r, float step, bool limited, Stack& st) {
+    Hit h;
+    h.dist = kInf;
+    h.tri = -1;
+    h.found = false;
+    float tn1, tn2;
+    if (!box_hit(r, g.rlo[0], g.rlo[1], g.rlo[2], g.rhi[0], g.rhi[1], g.rhi[2], tn1)) return h;
+    if (limited && step < tn1) return h;
+    st.sp = 0;
+    int cur = g.root_ref;
+    while (true) {
+        bool use_stack;
+        if (cur < 0) {
+            int i = ~cur, best = -1, last;
+            do {
+                float t = tri_distance(g.tris, i, r, last);
+                if (t >= 0.0f && t < h.dist) {
+                    h.dist = t;
+                    best = i;
+                }
+                i++;
+            } while (!last);
+            if (best >= 0 && (!limited || h.dist <= step)) {
+                h.found = true;
+                h.tri = best;
+            }
+            use_stack = true;
+        } else {
+            const float4* N = reinterpret_cast<const float4*>(g.nodes) + 4 * cur;
+            float4 a = N[0], b = N[1], c = N[2];
+            int4 d = reinterpret_cast<const int4*>(N)[3];
+            bool f1 = box_hit(r, a.x, a.y, a.z, a.w, b.x, b.y, tn1);
+            bool f2 = box_hit(r, b.z, b.w, c.x, c.y, c.z, c.w, tn2);
+            int left = d.x, right = d.y;
+            use_stack = false;
+            if (limited) {
+                if (f1 && f2) {
+                    if (tn1 <= tn2) {
+                        if (tn1 <= step) {
+                            cur = left;
+                            if (tn2 <= step) st.push(right, tn2);
+                        } else {
+                            use_stack = true;
+                        }
+                    } else {
+                        if (tn2 <= step) {
+                            cur = right;
+                            if (tn1 <= step) st.push(left, tn1);
+                        } else {
+                            use_stack = true;
+                        }
+                    }
+                } else if (f1) {
+                    if (tn1 <= step) cur = left;
+                    else use_stack = true;
+                } else {
+                    if (f2 && tn2 <= step) cur = right;
+                    else use_stack = true;
+                }
+            } else {
+                if (f1 && f2) {
+                    if (tn1 <= tn2) {
+                        cur = left;
+                        st.push(right, tn2);
+                    } else {
+                        cur = right;
+                        st.push(left, tn1);
+                    }
+                } else if (f1) {
+                    cur = left;
+                } else if (f2) {
+                    cur = right;
+                } else {
+                    use_stack = true;
+                }
+            }
+        }
+        if (use_stack) {
+            if (st.sp == 0) break;
+            int2 e = st.pop();
+            float tn = __int_as_float(e.y);
+            cur = e.x;
+            if (h.dist < tn) {
+                while (st.sp > 0 && h.dist < tn) {
+                    e = st.pop();
+                    tn = __int_as_float(e.y);
+                }
+                if (h.dist < tn || (limited && step < tn)) break;
+                cur = e.x;
+            }
+        }
+    }
+    return h;
+}
+
+// --------------------------------------------------------------------------
+// photon physics (mcml_kernel.cpp), float
+// --------------------------------------------------------------------------
+__device__ __forceinline__ float transmit_cos(float n1, float n2, float ci) {  // :497-529
+    if (n1 == n2) return ci;
+    if (ci > kCosZeroF) return ci;
+    if (ci < kCos90F) return 0.0f;
+    float si = __builtin_sqrtf(one_minus_sq(ci));
+    float st = n1 * si / n2;
+    if (st >= 1.0f) return 0.0f;
+    return __builtin_sqrtf(one_minus_sq(st));
+}
+
+__device__ __forceinline__ float fresnel(float n1, float n2, float ci, float ct) {  // :531-564
+    if (n1 == n2) return 0.0f;
+    if (ci > kCosZeroF) {
+        float r = (n2 - n1) / (n2 + n1);
+        return r * r;
+    }
+    if (ci < kCos90F) return 1.0f;
+    float si = __builtin_sqrtf(one_minus_sq(ci));
+    float st = __builtin_sqrtf(one_minus_sq(ct));
+    float cs = ci * ct - si * st;
+    float cd = ci * ct + si * st;
+    float ss = si * ct + ci * st;
+    float sd = si * ct - ci * st;
+    return 0.5f * sd * sd * (cd * cd + cs * cs) / (ss * ss * cd * cd);
+}
+
+__device__ __forceinline__ V3 reflect(V3 i, V3 n) {  // :582-593
+    V3 sn = normalize(n);
+    V3 si = normalize(i);
+    float tmp = 2.0f * dot(si, sn);
+    sn.x *= tmp;
+    sn.y *= tmp;
+    sn.z *= tmp;
+    return normalize(sub(si, sn));
+}
+
+__device__ __forceinline__ V3 refract(float n1, float n2, V3 i, V3 n) {  // :595-616
+    V3 si = normalize(i);
+    si.x *= n1;
+    si.y *= n1;
+    si.z *= n1;
+    V3 sn = normalize(n);
+    float t1 = dot(si, sn);
+    float t2 = __builtin_sqrtf((n2 * n2 - n1 * n1) / (t1 * t1) + 1.0f);
+    t2 = (t2 - 1.0f) * t1;
+    V3 t = {si.x + t2 * sn.x, si.y + t2 * sn.y, si.z + t2 * sn.z};
+    return normalize(t);
+}
+
+// GetNewPhotonDirection + ComputeCosineTheta (:442-495)
+__device__ __forceinline__ V3 spin(V3 d, float g, uint64_t& x, uint32_t& draws) {
+    float u = next_u(x);
+    float c;
+    if (g != 0.0f) {
+        float tmp = (1.0f - g * g) / (1.0f - g + 2.0f * g * u);
+        c = (1.0f + g * g - tmp * tmp) / (2.0f * g);
+    } else {
+        c = 2.0f * u - 1.0f;
+    }
+    if (c > 1.0f) c = 1.0f;
+    else if (c < -1.0f) c = -1.0f;
+    float st = __builtin_sqrtf(one_minus_sq(c));
+    float sp, cp;
+    xsincos2pi(next_u(x), sp, cp);
+    draws += 2;
+    float ux = d.x, uy = d.y, uz = d.z;
+    V3 r;
+    if (fabsf(uz) <= kCosZeroF) {
+        float tmp = __builtin_sqrtf(one_minus_sq(uz));
+        r.x = st * (ux * uz * cp - uy * sp) / tmp + ux * c;
+        r.y = st * (uy * uz * cp + ux * sp) / tmp + uy * c;
+        r.z = -st * cp * tmp + uz * c;
+    } else {
+        float tmp = (uz >= 0.0f) ? 1.0f : -1.0f;
+        r.x = st * cp;
+        r.y = st * sp;
+        r.z = tmp * c;
+    }
+    // one first-order renormalisation step, d *= (3 - |d|^2)/2: the rotation
+    // does not preserve |d| = 1 and float rounding would drift it
+    float f = __builtin_fmaf(-0.5f, dot(r, r), 1.5f);
+    r.x = r.x * f;
+    r.y = r.y * f;
+    r.z = r.z * f;
+    return r;
+}
+
+// GetAreaIndex (:190-227) in float; -1 outside.  Partitions are <= 255.
+__device__ __forceinline__ int area_index(const Area& A, V3 p) {
+    float ix = A.partf[0] * (p.x - A.corner[0]) / A.length[0];
+    float iy = A.partf[1] * (p.y - A.corner[1]) / A.length[1];
+    float iz = A.partf[2] * (p.z - A.corner[2]) / A.length[2];
+    bool inside = !((ix < 0.0f || ix >= A.partf[0]) || (iy < 0.0f || iy >= A.partf[1]) ||
+                    (iz < 0.0f || iz >= A.partf[2]));
+    if (!inside) return -1;
+    int bx = (int)ix, by = (int)iy, bz = (int)iz;
+    return bx * A.part[1] * A.part[2] + by * A.part[2] + bz;
+}
+
+// GetDetectorId (:663-742): first matching cube, then ring.
+__device__ int detector_id(const Detectors& D, V3 p, V3 d, uint32_t visited) {
+    for (int i = 0; i < D.n_cube; ++i) {
+        const float* c = D.cube + 7 * i;
+        bool in = (p.x >= c[0]) && (p.x < c[3]) && (p.y >= c[1]) && (p.y < c[4]) && (p.z >= c[2]) && (p.z < c[5]);
+        if (in && (visited & D.filter[i])) {
+            float len = d.x * d.x + d.y * d.y + d.z * d.z;
+            float ac = (-d.z) / len;
+            if (ac > c[6]) return i;
+        }
+    }
+    for (int i = 0; i < D.n_ring; ++i) {
+        const float* r = D.ring + 5 * i;
+        float dist = __builtin_sqrtf(((p.x - r[0]) * (p.x - r[0])) + ((p.y - r[1]) * (p.y - r[1])));
+        bool in = (dist >= r[2]) && (dist < r[3]);
+        if (in && (visited & D.filter[D.n_cube + i])) {
+            float len = d.x * d.x + d.y * d.y + d.z * d.z;
+            float ac = (-d.z) / len;
+            if (ac > r[4]) return D.n_cube + i;
+        }
+    }
+    return -1;
+}
+
+// UpdateDetectorTimeScale (:812-849): first bin i < T-1 with
+// start_i <= t < finish_i (candidates around (t - t0)/dt), else the open
+// last bin when t >= start_{T-1}.
+__device__ __forceinline__ int time_bin(const Detectors& D, double t) {
+    int T = D.T;
+    if (T <= 0) return -1;
+    if (T > 1) {
+        if (D.dt > 0.0) {
+            double xb = (t - D.t0) / D.dt;
+            int k = xb < 0.0 ? -1 : (xb >= (double)(T - 1) ? T - 1 : (int)xb);
+            int lo = k - 1 < 0 ? 0 : k - 1;
+            int hi = k + 1 > T - 2 ? T - 2 : k + 1;
+            for (int i = lo; i <= hi; ++i)
+                if (t >= D.t_start[i] && t < D.t_finish[i]) return i;
+        } else {
+            for (int i = 0; i < T - 1; ++i)
+                if (t >= D.t_start[i] && t < D.t_finish[i]) return i;
+        }
+    }
+    if (t >= D.t_start[T - 1]) return T - 1;
+    return -1;
+}
+
+template <bool TRACE>
+struct Recorder {
+    xmc_trace_event* ev;
+    int n;
+    int cap;
+    __device__ __forceinline__ void put(int type, int layer, int aux, uint32_t draws, V3 p, V3 d, float w,
+                                        float t) {
+        if (TRACE) {
+            if (ev != nullptr && n < cap) {
+                xmc_trace_event& e = ev[n];
+                e.type = type;
+                e.layer = layer;
+                e.aux = aux;
+                e.draws = draws;
+                e.pos[0] = p.x; e.pos[1] = p.y; e.pos[2] = p.z;
+                e.dir[0] = d.x; e.dir[1] = d.y; e.dir[2] = d.z;
+                e.weight = w;
+                e.time = t;
+            }
+            n++;
+        }
+    }
+};
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
+    uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, lane);
+    uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Weight ledger slots (per block in LDS, flushed to stats words 8..15).
+enum { kAbs = 0, kEsc, kDet, kArea, kKill, kRLost, kRGain, kOther, kLedger };
+
+template <bool TRACE>
+__global__ void __launch_bounds__(256) transport_kernel(Params P) {
+    extern __shared__ int2 lds_stack[];
+    const int tid = threadIdx.x;
+    const int lane = __lane_id();
+    const int nlanes = gridDim.x * blockDim.x;
+    Stack st{lds_stack, (int)blockDim.x, tid, P.spill, nlanes, (int)(blockIdx.x * blockDim.x + tid),
+             P.spill_depth, 0, 0};
+    double* ledger = reinterpret_cast<double*>(lds_stack + kStackLds * blockDim.x);
+    if (tid < kLedger) ledger[tid] = 0.0;
+    __syncthreads();
+    double w_abs = 0.0;
+    Geo geo{P.nodes, P.tris, P.root_ref, {P.root_lo[0], P.root_lo[1], P.root_lo[2]},
+            {P.root_hi[0], P.root_hi[1], P.root_hi[2]}};
+
+    V3 pos = {0, 0, 0}, dir = {0, 0, 1};
+    float w = 0.0f, tim = 0.0f, trange = 0.0f, orange = 0.0f;
+    int layer = 1;
+    uint32_t visited = 0;
+    uint64_t x = 0;
+    bool alive = false, done = false;
+    float Ln = 1.0f, Lmua = 0.0f, Lmut = 0.0f, Lg = 0.0f;
+    bool Lglass = false, Ltarget = false;
+    uint32_t draws = 0;
+    Recorder<TRACE> rec{nullptr, 0, 0};
+    unsigned long long n_scatter = 0, n_boundary = 0, n_roulette = 0, n_photons = 0;
+
+    auto load_layer = [&](int l) {
+        Ln = P.layers.n[l];
+        Lmua = P.layers.mua[l];
+        Lmut = P.layers.mua[l] + P.layers.mus[l];
+        Lg = P.layers.g[l];
+        Lglass = (P.layers.glass_mask >> l) & 1u;
+        Ltarget = (P.layers.target_mask >> l) & 1u;
+    };
+
+    while (true) {
+        // ---- refill retired lanes from the photon queue
+        bool need = !alive && !done;
+        uint64_t need_mask = __ballot(need);
+        if (need_mask) {
+            int leader = __ffsll((unsigned long long)need_mask) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(P.queue, (unsigned long long)__popcll(need_mask));
+            base = readlane64(base, leader);
+            if (need) {
+                uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(need_mask >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)need_mask, 0u));
+                uint64_t idx = base + rank;
+                if (idx < P.n_photons) {
+                    uint64_t pid = P.first_photon + idx;
+                    x = P.seed_value;
+                    for (int i = 0; i < kJumpBits; ++i)
+                        if ((pid >> i) & 1ull) x = (x * P.jump[i]) & kMcgMask;
+                    // LaunchPhoton (:128-149); float start offset along +z
+                    V3 s = {P.start_pos[0], P.start_pos[1], P.start_pos[2]};
+                    pos = s;
+                    pos.z = s.z + offset_scale(s);
+                    dir = {P.start_dir[0], P.start_dir[1], P.start_dir[2]};
+                    w = P.w0;
+                    tim = 0.0f;
+                    trange = 0.0f;
+                    orange = 0.0f;
+                    layer = 1;
+                    visited = 1u << 1;
+                    load_layer(1);
+                    alive = true;
+                    n_photons++;
+                    draws = 0;
+                    if (TRACE) {
+                        uint64_t k = pid - P.trace.first;
+                        rec.ev = (k < P.trace.n) ? P.trace.events + k * (uint64_t)P.trace.max_events : nullptr;
+                        rec.n = 0;
+                        rec.cap = P.trace.max_events;
+                        rec.put(XMC_EV_LAUNCH, layer, 0, draws, pos, dir, w, tim);
+                        if (k < P.trace.n) P.trace.counts[k] = 0;
+                    }
+                } else {
+                    done = true;
+                }
+            }
+        }
+        if (__ballot(alive) == 0) break;
+        if (!alive) continue;
+
+        if (draws > kDrawLimit) {  // stream block exhausted
+            alive = false;
+            atomicAdd(ledger + kOther, (double)w);
+            continue;
+        }
+        // ---- one interaction (mcml_kernel.cpp:47-124)
+        float step = kInf;
+        if (!Lglass) {
+            float u;
+            do {
+                u = next_u(x);
+                draws++;
+            } while (u == 0.0f);
+            step = -xlogf(u) / Lmut;
+        }
+        Ray ray = make_ray(pos, dir);
+        Hit h = walk(geo, ray, step, !Lglass, st);
+        if (h.found) {
+            step = h.dist - 0.01f * kMinDistF;
+            tim += step * Ln;
+            if (Ltarget) trange += step;
+            else orange += step;
+            pos.x += step * dir.x;
+            pos.y += step * dir.y;
+            pos.z += step * dir.z;
+            V3 hit = pos;
+            V3 nrm = tri_normal(geo.tris, h.tri);
+            float ofs = offset_scale(hit);
+            if (dot(dir, nrm) < 0.0f) ofs = -ofs;
+            pos.x = hit.x - ofs * nrm.x;
+            pos.y = hit.y - ofs * nrm.y;
+            pos.z = hit.z - ofs * nrm.z;
+            if (area_index(P.area, pos) >= 0) {
+                // CrossBoundary (:229-284)
+                float ci = dot(dir, nrm);
+                if (ci < 0.0f) ci = -ci;
+                int surface = __float_as_int(reinterpret_cast<const float4*>(geo.tris)[3 * h.tri].w);
+                int nxt = P.next_layer[layer * P.n_surfaces + surface];
+                if (nxt < 0) {
+                    alive = false;
+                    atomicAdd(ledger + kOther, (double)w);
+                } else {
+                    float n1 = Ln, n2 = P.layers.n[nxt];
+                    float cc = P.layers.crit[layer][nxt];
+                    float refl;
+                    if (ci <= cc) {
+                        refl = 1.0f;
+                    } else {
+                        float ct = transmit_cos(n1, n2, ci);
+                        refl = fresnel(n1, n2, ci, ct);
+                    }
+                    n_boundary++;
+                    float u = next_u(x);
+                    draws++;
+                    if (u > refl) {
+                        if (nxt == 0) {
+                            int id = detector_id(P.det, pos, dir, visited);
+                            if (id >= 0) {
+                                double wd = (double)w;
+                                float total = trange + orange;
+                                float tw = trange * (w / total);
+                                unsafeAtomicAdd(P.tally.det_weight + id, wd);
+                                atomicAdd(P.tally.det_count + id, 1ull);
+                                int b = time_bin(P.det, (double)tim);
+                                if (b >= 0) {
+                                    size_t k = (size_t)id * P.det.T + b;
+                                    unsafeAtomicAdd(P.tally.bin_target + k, (double)tw);
+                                    atomicAdd(P.tally.bin_count + k, 1ull);
+                                    unsafeAtomicAdd(P.tally.bin_weight + k, wd);
+                                }
+                                unsafeAtomicAdd(P.tally.det_target + id, (double)tw);
+                                atomicAdd(ledger + kDet, wd);
+                            }
+                            atomicAdd(ledger + kEsc, (double)w);
+                            alive = false;
+                            rec.put(XMC_EV_ESCAPE, layer, id, draws, pos, dir, w, tim);
+                        } else {
+                            dir = refract(n1, n2, dir, nrm);
+                            layer = nxt;
+                            visited |= 1u << nxt;
+                            load_layer(nxt);
+                            float o2 = offset_scale(hit);
+                            if (dot(dir, nrm) < 0.0f) o2 = -o2;
+                            pos.x = hit.x + o2 * nrm.x;
+                            pos.y = hit.y + o2 * nrm.y;
+                            pos.z = hit.z + o2 * nrm.z;
+                            rec.put(XMC_EV_TRANSMIT, layer, surface, draws, pos, dir, w, tim);
+                        }
+                    } else {
+                        dir = reflect(dir, nrm);
+                        float o2 = offset_scale(hit);
+                        if (dot(dir, nrm) < 0.0f) o2 = -o2;
+                        pos.x = hit.x + o2 * nrm.x;
+                        pos.y = hit.y + o2 * nrm.y;
+                        pos.z = hit.z + o2 * nrm.z;
+                        rec.put(XMC_EV_REFLECT, layer, surface, draws, pos, dir, w, tim);
+                    }
+                }
+            } else {
+                alive = false;
+                atomicAdd(ledger + kArea, (double)w);
+                rec.put(XMC_EV_DIE_AREA, layer, 0, draws, pos, dir, w, tim);
+            }
+        } else if (Lglass) {
+            alive = false;
+            atomicAdd(ledger + kOther, (double)w);
+            rec.put(XMC_EV_GLASS_MISS, layer, 0, draws, pos, dir, w, tim);
+        } else {
+            tim += step * Ln;
+            if (Ltarget) trange += step;
+            else orange += step;
+            pos.x += step * dir.x;
+            pos.y += step * dir.y;
+            pos.z += step * dir.z;
+            int idx = area_index(P.area, pos);
+            if (idx >= 0) {
+                float dw = w * Lmua / Lmut;  // ComputeDroppedWeightOfPhoton (:633-642)
+                w -= dw;
+                if (P.tally.absorption) unsafeAtomicAdd(P.tally.absorption + idx, (double)dw);
+                w_abs += (double)dw;
+                dir = spin(dir, Lg, x, draws);
+                n_scatter++;
+                rec.put(XMC_EV_SCATTER, layer, idx, draws, pos, dir, w, tim);
+            } else {
+                alive = false;
+                atomicAdd(ledger + kArea, (double)w);
+                rec.put(XMC_EV_DIE_AREA, layer, 0, draws, pos, dir, w, tim);
+            }
+        }
+        // ---- termination: xmcml kill (:121-124) or MCML roulette
+        if (P.roulette) {
+            if (w < P.min_weight && alive) {
+                n_roulette++;
+                if (w == 0.0f) {
+                    alive = false;
+                    rec.put(XMC_EV_ROULETTE_DIE, layer, 0, draws, pos, dir, w, tim);
+                } else {
+                    float u = next_u(x);
+                    draws++;
+                    if (u < P.chance) {
+                        float before = w;
+                        w /= P.chance;
+                        atomicAdd(ledger + kRGain, (double)w - (double)before);
+                        rec.put(XMC_EV_ROULETTE_LIVE, layer, 0, draws, pos, dir, w, tim);
+                    } else {
+                        alive = false;
+                        atomicAdd(ledger + kRLost, (double)w);
+                        rec.put(XMC_EV_ROULETTE_DIE, layer, 0, draws, pos, dir, w, tim);
+                    }
+                }
+            }
+        } else if (w < P.min_weight) {
+            if (alive) {
+                atomicAdd(ledger + kKill, (double)w);
+                rec.put(XMC_EV_DIE_WEIGHT, layer, 0, draws, pos, dir, w, tim);
+            }
+            alive = false;
+        }
+        if (TRACE && !alive && rec.ev != nullptr) {
+            uint64_t k = (uint64_t)(rec.ev - P.trace.events) / (uint64_t)P.trace.max_events;
+            P.trace.counts[k] = rec.n;
+        }
+    }
+    if (n_photons) atomicAdd(P.tally.stats + 0, n_photons);
+    if (n_scatter) atomicAdd(P.tally.stats + 1, n_scatter);
+    if (n_boundary) atomicAdd(P.tally.stats + 2, n_boundary);
+    if (n_roulette) atomicAdd(P.tally.stats + 3, n_roulette);
+    if (st.spills) atomicAdd(P.tally.stats + 4, (unsigned long long)st.spills);
+    atomicAdd(ledger + kAbs, w_abs);
+    __syncthreads();
+    if (tid < kLedger) unsafeAtomicAdd(reinterpret_cast<double*>(P.tally.stats + 8) + tid, ledger[tid]);
+}
+
+template __global__ void transport_kernel<false>(Params);
+template __global__ void transport_kernel<true>(Params);
+
+// One ray per thread: nearest hit through the BVH or over every triangle.
+__global__ void __launch_bounds__(256) query_kernel(QueryParams Q, const int* perm) {
+    extern __shared__ int2 lds_stack[];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st{lds_stack, (int)blockDim.x, (int)threadIdx.x, Q.spill, (int)(gridDim.x * blockDim.x), i,
+             Q.spill_depth, 0, 0};
+    if (i >= Q.n) return;
+    V3 o = {Q.origins[3 * i], Q.origins[3 * i + 1], Q.origins[3 * i + 2]};
+    V3 d = {Q.directions[3 * i], Q.directions[3 * i + 1], Q.directions[3 * i + 2]};
+    float step = Q.steps[i];
+    bool limited = step > 0.0f;
+    Ray r = make_ray(o, d);
+    Hit h;
+    if (Q.brute) {
+        h.dist = kInf;
+        h.tri = -1;
+        int last;
+        for (int g = 0; g < Q.n_tris; ++g) {
+            int k = perm[g];
+            float t = tri_distance(Q.tris, k, r, last);
+            if (t >= 0.0f && t < h.dist) {
+                h.dist = t;
+                h.tri = k;
+            }
+        }
+        h.found = h.tri >= 0 && (!limited || h.dist <= step);
+    } else {
+        Geo geo{Q.nodes, Q.tris, Q.root_ref, {Q.root_lo[0], Q.root_lo[1], Q.root_lo[2]},
+                {Q.root_hi[0], Q.root_hi[1], Q.root_hi[2]}};
+        h = walk(geo, r, limited ? step : kInf, limited, st);
+    }
+    Q.found[i] = h.found ? 1 : 0;
+    Q.distance[i] = h.found ? h.dist : 0.0f;
+    if (h.found) {
+        const float4* T = reinterpret_cast<const float4*>(Q.tris) + 3 * h.tri;
+        Q.surface[i] = __float_as_int(T[0].w);
+        Q.triangle[i] = __float_as_int(T[1].w);
+        V3 n = tri_normal(Q.tris, h.tri);
+        Q.normal[3 * i] = n.x;
+        Q.normal[3 * i + 1] = n.y;
+        Q.normal[3 * i + 2] = n.z;
+    } else {
+        Q.surface[i] = -1;
+        Q.triangle[i] = -1;
+        Q.normal[3 * i] = Q.normal[3 * i + 1] = Q.normal[3 * i + 2] = 0.0f;
+    }
+}
+
+// Fills n 64-bit words with zero (capture-safe replacement for memset).
+__global__ void zero_kernel(unsigned long long* p, size_t n) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    size_t s = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += s) p[i] = 0ull;
+}
+
+}  // namespace xmc
+
+// Host-callable launchers (the host translation unit has no kernel syntax).
+extern "C" {
+
+hipError_t xmc_launch_transport(const xmc::Params* p, int grid, int block, size_t lds, hipStream_t s, int trace) {
+    if (trace)
+        hipLaunchKernelGGL(xmc::transport_kernel<true>, dim3(grid), dim3(block), lds, s, *p);
+    else
+        hipLaunchKernelGGL(xmc::transport_kernel<false>, dim3(grid), dim3(block), lds, s, *p);
+    return hipGetLastError();
+}
+
+hipError_t xmc_launch_query(const xmc::QueryParams* q, const int* perm, int block, size_t lds, hipStream_t s) {
+    int grid = (q->n + block - 1) / block;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(xmc::query_kernel, dim3(grid), dim3(block), lds, s, *q, perm);
+    return hipGetLastError();
+}
+
+hipError_t xmc_launch_zero(unsigned long long* p, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(xmc::zero_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+    return hipGetLastError();
+}
+
+hipError_t xmc_transport_occupancy(int block, size_t lds, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, xmc::transport_kernel<false>, block, lds);
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+"""ctypes binding of the engine library (libxmcml_amd.so, include/xmcml.h).
+
+There is no CPU fallback: if the library is missing or no GPU is present the
+calls raise.  ``Scene`` mirrors the reference's split between building the
+geometry once (GenerateBVHTree, base/xmcml/xmcml/mcml_intersection.cpp:316)
+and running photon portions into accumulated outputs (LaunchOMP,
+base/xmcml/xmcmlLauncher/launcher_omp.cpp:123).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from .abi import (TRACE_DTYPE, InputHandle, Options, OutputHandle, Rng, SceneSpec, Stats)
+
+LIB_PATH = Path(__file__).resolve().parent / "libxmcml_amd.so"
+
+_P = C.c_void_p
+_I = C.c_int
+_U64 = C.c_uint64
+_PD = C.POINTER(C.c_double)
+_PI = C.POINTER(C.c_int)
+
+_lib = None
+
+EXPORTS = [
+    "xmc_scene_create", "xmc_scene_destroy", "xmc_scene_run", "xmc_scene_sync", "xmc_scene_collect", "xmc_scene_stats",
+    "xmc_scene_trace", "xmc_scene_query", "xmc_scene_bvh", "xmc_launch", "xmc_init_output",
+    "xmc_free_output", "xmc_specular_reflectance", "xmc_last_error", "xmc_version",
+]
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load the engine library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise EngineError(f"{LIB_PATH} is missing: build it with `python -m xmcml_amd.build`")
+    L = C.CDLL(str(LIB_PATH))
+
+    def sig(name, res, args):
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+    sig("xmc_scene_create", _I, [_P, _P, C.POINTER(_P)])
+    sig("xmc_scene_destroy", _I, [_P])
+    sig("xmc_scene_run", _I, [_P, _U64, _U64, _U64, _P])
+    sig("xmc_scene_sync", _I, [_P])
+    sig("xmc_scene_collect", _I, [_P, _P])
+    sig("xmc_scene_stats", _I, [_P, _P])
+    sig("xmc_scene_trace", _I, [_P, _U64, _U64, _U64, _P, _I, _PI])
+    sig("xmc_scene_query", _I, [_P, _PD, _PD, _PD, _I, _I, _PI, _PD, _PI, _PI, _PD])
+    sig("xmc_scene_bvh", _I, [_P, _PI, _PI, _PI, _PD, _PI, _PI, _PI])
+    sig("xmc_launch", _I, [_P, _P, _P, _P])
+    sig("xmc_init_output", _I, [_P, _P])
+    sig("xmc_free_output", None, [_P])
+    sig("xmc_specular_reflectance", C.c_double, [_P])
+    sig("xmc_last_error", C.c_char_p, [])
+    sig("xmc_version", C.c_char_p, [])
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().xmc_last_error().decode(errors="replace")
+        raise EngineError(f"{what} failed ({rc}): {msg}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(_PD)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_PI)
+
+
+class Scene:
+    """Geometry, optics and device tallies of one input, resident on the GPU."""
+
+    def __init__(self, spec: SceneSpec, block_size: int = 0, blocks_per_cu: int = 0,
+                 absorption: bool = True, device: int = 0):
+        self.spec = spec
+        self.input = InputHandle(spec)
+        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1)
+        h = _P()
+        _check(lib().xmc_scene_create(self.input.ptr, C.byref(opt), C.byref(h)), "xmc_scene_create")
+        self._h = h
+        self.absorption = absorption
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().xmc_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def run(self, seed: int, first: int, n: int, stream: Optional[int] = None):
+        _check(lib().xmc_scene_run(self._h, seed, first, n, stream), "xmc_scene_run")
+
+    def sync(self):
+        _check(lib().xmc_scene_sync(self._h), "xmc_scene_sync")
+
+    def collect(self, out: Optional[OutputHandle] = None) -> OutputHandle:
+        if out is None:
+            out = OutputHandle(self.spec, absorption=self.absorption)
+        _check(lib().xmc_scene_collect(self._h, out.ptr), "xmc_scene_collect")
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().xmc_scene_stats(self._h, C.byref(s)), "xmc_scene_stats")
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def trace(self, seed: int, first: int, n: int, max_events: int = 4096):
+        ev = np.zeros(n * max_events, dtype=TRACE_DTYPE)
+        cnt = np.zeros(n, dtype=np.int32)
+        _check(lib().xmc_scene_trace(self._h, seed, first, n, ev.ctypes.data, max_events, _ip(cnt)),
+               "xmc_scene_trace")
+        ev = ev.reshape(n, max_events)
+        return [ev[i, :min(cnt[i], max_events)].copy() for i in range(n)], cnt
+
+    def query(self, origins, directions, steps, brute: bool = False):
+        o = np.ascontiguousarray(origins, dtype=np.float64)
+        d = np.ascontiguousarray(directions, dtype=np.float64)
+        s = np.ascontiguousarray(steps, dtype=np.float64)
+        n = s.shape[0]
+        found = np.zeros(n, np.int32)
+        dist = np.zeros(n)
+        surf = np.zeros(n, np.int32)
+        tri = np.zeros(n, np.int32)
+        nrm = np.zeros((n, 3))
+        _check(lib().xmc_scene_query(self._h, _dp(o), _dp(d), _dp(s), n, int(brute), _ip(found), _dp(dist),
+                                     _ip(surf), _ip(tri), _dp(nrm)), "xmc_scene_query")
+        return dict(found=found, distance=dist, surface=surf, triangle=tri, normal=nrm)
+
+    def bvh(self) -> dict:
+        nn, root, nt = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().xmc_scene_bvh(self._h, C.byref(nn), C.byref(root), C.byref(nt), None, None, None, None),
+               "xmc_scene_bvh")
+        boxes = np.zeros((nn.value, 6))
+        links = np.zeros((nn.value, 4), np.int32)
+        ts = np.zeros(nt.value, np.int32)
+        ti = np.zeros(nt.value, np.int32)
+        _check(lib().xmc_scene_bvh(self._h, C.byref(nn), C.byref(root), C.byref(nt), _dp(boxes), _ip(links),
+                                   _ip(ts), _ip(ti)), "xmc_scene_bvh")
+        return dict(root=root.value, boxes=boxes, links=links, tsurf=ts, tidx=ti)
+
+
+def launch(spec: SceneSpec, out: OutputHandle, seed: int, first: int, n: int, absorption: bool = True) -> int:
+    """The LaunchOMP replacement: one call, tallies added into ``out``.
+    Returns the next photon index of the stream."""
+    inp = InputHandle(spec, photons=n)
+    rng = Rng(seed, first)
+    opt = Options(0, 0, 0, 0 if absorption else 1)
+    _check(lib().xmc_launch(inp.ptr, out.ptr, C.byref(rng), C.byref(opt)), "xmc_launch")
+    return rng.nextPhoton
