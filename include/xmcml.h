@@ -264,6 +264,28 @@ int xmc_scene_bvh(xmc_scene* scene, int* numberOfNodes, int* root, int* numberOf
                   double* boxes /* 6 per node */, int* links /* left,right,offset,count */,
                   int* triangleSurface, int* triangleIndex);
 
+/* Launch shape of a scene (threads per block, resident blocks per CU cap);
+ * used by the autotuner.  get returns the block size and total blocks. */
+int xmc_scene_set_launch(xmc_scene* scene, int blockSize, int blocksPerCU);
+int xmc_scene_get_launch(xmc_scene* scene, int* blockSize, int* blocks);
+
+/* --- Batches: many configurations in one persistent launch. ---
+ * No counterpart in the reference (it runs one input per process).  configs[c]
+ * is a complete input; configs[c].numberOfPhotons is its photon budget.  A run
+ * processes every budget in one launch fed by a device work queue (photon p of
+ * configuration c draws the stream block sum_{c'<c} budget + p of `seed`),
+ * replayed from a captured HIP graph when useGraph is set (default). */
+typedef struct xmc_batch xmc_batch;
+int xmc_batch_create(const xmc_input* configs, int numberOfConfigs, const xmc_options* options,
+                     xmc_batch** batch);
+int xmc_batch_destroy(xmc_batch* batch);
+int xmc_batch_run(xmc_batch* batch, uint64_t seed, void* hipStream);
+/* outputs[c] receives configuration c's tallies (+=). */
+int xmc_batch_collect(xmc_batch* batch, xmc_output* outputs);
+int xmc_batch_stats(xmc_batch* batch, xmc_stats* stats);
+int xmc_batch_set_launch(xmc_batch* batch, int blockSize, int blocksPerCU, int useGraph);
+int xmc_batch_get_launch(xmc_batch* batch, int* blockSize, int* blocks);
+
 /* --- The LaunchOMP replacement. --- */
 /* Builds a scene, runs input->numberOfPhotons photons starting at
  * rng->nextPhoton, adds tallies into output, advances rng->nextPhoton. */

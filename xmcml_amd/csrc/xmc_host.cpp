@@ -22,6 +22,9 @@ hipError_t xmc_launch_transport(const xmc::Params* p, int grid, int block, size_
 hipError_t xmc_launch_query(const xmc::QueryParams* q, const int* perm, int block, size_t lds, hipStream_t s);
 hipError_t xmc_launch_zero(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t xmc_transport_occupancy(int block, size_t lds, int* blocks_per_cu);
+hipError_t xmc_launch_batch(const xmc::Params* p, const xmc::Params* cfgs, const uint64_t* prefix, int ncfg, int grid,
+                            int block, size_t lds, hipStream_t s);
+hipError_t xmc_batch_occupancy(int block, size_t lds, int* blocks_per_cu);
 }
 
 namespace {
@@ -41,6 +44,9 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr double kPi = 3.14159265358979323846;
+
+// dynamic LDS of the transport kernels: short stacks, then the block's ledger
+size_t transport_lds(int block) { return (size_t)xmc::kStackLds * block * sizeof(int2) + 8 * sizeof(double); }
 
 template <class T>
 struct DevBuf {
@@ -136,9 +142,6 @@ int validate(const xmc_input* in) {
     return XMC_OK;
 }
 
-// dynamic LDS of the transport kernel: short stacks, then the block's ledger
-size_t transport_lds(int block) { return (size_t)xmc::kStackLds * block * sizeof(int2) + 8 * sizeof(double); }
-
 // critical cosine in float arithmetic (GetCriticalCos, mcml_kernel.cpp:644-661)
 float crit_cos(float n1, float n2) {
     if (n1 > n2) return std::sqrt(1.0f - n2 * n2 / (n1 * n1));
@@ -166,7 +169,37 @@ double xmc_specular_reflectance(const xmc_layer* layer) {  // mcml_kernel.cpp:16
     return r1;
 }
 
+static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchable, xmc_scene** out);
+
 int xmc_scene_create(const xmc_input* in, const xmc_options* opt, xmc_scene** out) {
+    return scene_init(in, opt, true, out);
+}
+
+// Persistent launch shape and the stack spill it needs.  kernel_occupancy is
+// the occupancy query of the kernel that will run.
+static int configure_launch(int device, int depth, int block, int blocks_per_cu,
+                            hipError_t (*kernel_occupancy)(int, size_t, int*), int* blocks_out,
+                            DevBuf<int2>& spill, xmc::Params& P) {
+    if (block <= 0) block = 256;
+    if (block % 64 != 0 || block > 256) return fail(XMC_ERR_ARG, "blockSize must be 64, 128, 192 or 256");
+    size_t lds = transport_lds(block);
+    int per_cu = 0;
+    HIP_TRY(kernel_occupancy(block, lds, &per_cu));
+    if (per_cu < 1) per_cu = 1;
+    if (blocks_per_cu > 0 && blocks_per_cu < per_cu) per_cu = blocks_per_cu;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    *blocks_out = prop.multiProcessorCount * per_cu;
+    // the walk holds at most depth - 1 entries; the top kStackLds live in LDS
+    int need = depth - xmc::kStackLds;
+    P.spill_depth = need > 0 ? need : 0;
+    size_t lanes = (size_t)(*blocks_out) * block;
+    HIP_TRY(spill.alloc(P.spill_depth > 0 ? (size_t)P.spill_depth * lanes : 1));
+    P.spill = spill.p;
+    return XMC_OK;
+}
+
+static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchable, xmc_scene** out) {
     if (!out) return fail(XMC_ERR_ARG, "scene pointer is NULL");
     *out = nullptr;
     int rc = validate(in);
@@ -377,26 +410,34 @@ int xmc_scene_create(const xmc_input* in, const xmc_options* opt, xmc_scene** ou
     P.chance = (float)in->rouletteChance;
     P.roulette = in->terminationMode == XMC_TERMINATE_ROULETTE;
 
-    // persistent launch shape
     sc->block = o.blockSize > 0 ? o.blockSize : 256;
-    if (sc->block % 64 != 0 || sc->block > 256) return fail(XMC_ERR_ARG, "blockSize must be 64, 128, 192 or 256");
-    size_t lds = transport_lds(sc->block);
-    int per_cu = 0;
-    HIP_TRY(xmc_transport_occupancy(sc->block, lds, &per_cu));
-    if (per_cu < 1) per_cu = 1;
-    if (o.blocksPerCU > 0 && o.blocksPerCU < per_cu) per_cu = o.blocksPerCU;
-    hipDeviceProp_t prop;
-    HIP_TRY(hipGetDeviceProperties(&prop, sc->device));
-    sc->blocks = prop.multiProcessorCount * per_cu;
-    // stack spill: the walk holds at most depth-1 entries
-    int need = sc->ref.depth - xmc::kStackLds;
-    P.spill_depth = need > 0 ? need : 0;
-    size_t lanes = (size_t)sc->blocks * sc->block;
-    HIP_TRY(sc->spill.alloc(P.spill_depth > 0 ? (size_t)P.spill_depth * lanes : 1));
-    P.spill = sc->spill.p;
-
-    HIP_TRY(hipStreamCreate(&sc->stream));
+    if (launchable) {
+        rc = configure_launch(sc->device, sc->ref.depth, sc->block, o.blocksPerCU, xmc_transport_occupancy,
+                              &sc->blocks, sc->spill, P);
+        if (rc) return rc;
+        HIP_TRY(hipStreamCreate(&sc->stream));
+    }
     *out = guard.release();
+    return XMC_OK;
+}
+
+int xmc_scene_set_launch(xmc_scene* sc, int block, int blocks_per_cu) {
+    if (!sc || !sc->stream) return fail(XMC_ERR_ARG, "scene is NULL or not launchable");
+    HIP_TRY(hipSetDevice(sc->device));
+    HIP_TRY(hipStreamSynchronize(sc->stream));
+    int blocks = 0;
+    int rc = configure_launch(sc->device, sc->ref.depth, block, blocks_per_cu, xmc_transport_occupancy, &blocks,
+                              sc->spill, sc->base);
+    if (rc) return rc;
+    sc->block = block;
+    sc->blocks = blocks;
+    return XMC_OK;
+}
+
+int xmc_scene_get_launch(xmc_scene* sc, int* block, int* blocks) {
+    if (!sc || !block || !blocks) return fail(XMC_ERR_ARG, "NULL argument");
+    *block = sc->block;
+    *blocks = sc->blocks;
     return XMC_OK;
 }
 
@@ -448,10 +489,16 @@ int xmc_scene_sync(xmc_scene* sc) {
     return XMC_OK;
 }
 
+static int collect_impl(xmc_scene* sc, xmc_output* out, hipStream_t stream);
+
 int xmc_scene_collect(xmc_scene* sc, xmc_output* out) {
     if (!sc || !out) return fail(XMC_ERR_ARG, "scene/output is NULL");
     HIP_TRY(hipSetDevice(sc->device));
     HIP_TRY(hipStreamSynchronize(sc->stream));
+    return collect_impl(sc, out, sc->stream);
+}
+
+static int collect_impl(xmc_scene* sc, xmc_output* out, hipStream_t stream) {
     HIP_TRY(hipDeviceSynchronize());
     if (out->numberOfDetectors != sc->n_det) return fail(XMC_ERR_ARG, "output detector count does not match input");
     std::vector<unsigned long long> h(sc->slab_words);
@@ -476,32 +523,19 @@ int xmc_scene_collect(xmc_scene* sc, xmc_output* out) {
     }
     out->specularReflectance = sc->rsp;
     // zero everything except the counters of xmc_scene_stats
-    HIP_TRY(xmc_launch_zero(sc->slab.p, sc->off_st, sc->stream));
-    HIP_TRY(hipStreamSynchronize(sc->stream));
+    HIP_TRY(xmc_launch_zero(sc->slab.p, sc->off_st, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
     return XMC_OK;
 }
 
-int xmc_scene_stats(xmc_scene* sc, xmc_stats* st) {
-    if (!sc || !st) return fail(XMC_ERR_ARG, "scene/stats is NULL");
-    HIP_TRY(hipSetDevice(sc->device));
-    HIP_TRY(hipStreamSynchronize(sc->stream));
-    HIP_TRY(hipDeviceSynchronize());
-    for (auto& e : sc->pending) {
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
-        sc->kernel_ms += ms;
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
-    }
-    sc->pending.clear();
+static int read_stats(unsigned long long* dev, hipStream_t stream, xmc_stats* st) {
     unsigned long long c[16];
-    HIP_TRY(hipMemcpy(c, sc->slab.p + sc->off_st, sizeof(c), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(c, dev, sizeof(c), hipMemcpyDeviceToHost));
     st->photons = c[0];
     st->scatterEvents = c[1];
     st->boundaryEvents = c[2];
     st->rouletteEvents = c[3];
     st->stackSpills = c[4];
-    st->kernelMs = sc->kernel_ms;
     double led[8];
     std::memcpy(led, c + 8, sizeof(led));
     st->wAbsorbed = led[0];
@@ -512,10 +546,34 @@ int xmc_scene_stats(xmc_scene* sc, xmc_stats* st) {
     st->wRouletteLost = led[5];
     st->wRouletteGain = led[6];
     st->wOther = led[7];
-    HIP_TRY(xmc_launch_zero(sc->slab.p + sc->off_st, 16, sc->stream));
-    HIP_TRY(hipStreamSynchronize(sc->stream));
-    sc->kernel_ms = 0.0;
+    HIP_TRY(xmc_launch_zero(dev, 16, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
     return XMC_OK;
+}
+
+static int drain_timers(std::vector<std::pair<hipEvent_t, hipEvent_t>>& pending, double* ms_total) {
+    for (auto& e : pending) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, e.first, e.second));
+        *ms_total += ms;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    pending.clear();
+    return XMC_OK;
+}
+
+int xmc_scene_stats(xmc_scene* sc, xmc_stats* st) {
+    if (!sc || !st) return fail(XMC_ERR_ARG, "scene/stats is NULL");
+    HIP_TRY(hipSetDevice(sc->device));
+    HIP_TRY(hipDeviceSynchronize());
+    double ms = sc->kernel_ms;
+    int rc = drain_timers(sc->pending, &ms);
+    if (rc) return rc;
+    rc = read_stats(sc->slab.p + sc->off_st, sc->stream, st);
+    st->kernelMs = ms;
+    sc->kernel_ms = 0.0;
+    return rc;
 }
 
 int xmc_scene_trace(xmc_scene* sc, uint64_t seed, uint64_t first, uint64_t n, xmc_trace_event* events,
@@ -692,6 +750,184 @@ void xmc_free_output(xmc_output* out) {
         free(out->detectorInfo);
     }
     std::memset(out, 0, sizeof(*out));
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: many configurations, one persistent launch, replayed as a graph.
+// ---------------------------------------------------------------------------
+struct xmc_batch {
+    int device = 0;
+    int n = 0;
+    std::vector<xmc_scene*> members;
+    std::vector<uint64_t> prefix;  // photons of configuration c: [prefix[c], prefix[c+1])
+    DevBuf<xmc::Params> cfgs;
+    DevBuf<uint64_t> dprefix;
+    DevBuf<unsigned long long> counters;  // 16 stats words + queue
+    DevBuf<int2> spill;
+    int depth = 0, block = 256, blocks = 0;
+    bool use_graph = true;
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t graph_seed = 0;
+    bool graph_valid = false;
+    xmc::Params run;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double kernel_ms = 0.0;
+};
+
+static void drop_graph(xmc_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_valid = false;
+}
+
+extern "C" {
+
+int xmc_batch_destroy(xmc_batch* b) {
+    if (!b) return XMC_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (auto& e : b->pending) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    for (xmc_scene* m : b->members) xmc_scene_destroy(m);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return XMC_OK;
+}
+
+int xmc_batch_set_launch(xmc_batch* b, int block, int blocks_per_cu, int use_graph) {
+    if (!b) return fail(XMC_ERR_ARG, "batch is NULL");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    int blocks = 0;
+    int rc = configure_launch(b->device, b->depth, block, blocks_per_cu, xmc_batch_occupancy, &blocks, b->spill,
+                              b->run);
+    if (rc) return rc;
+    b->block = block > 0 ? block : 256;
+    b->blocks = blocks;
+    b->use_graph = use_graph != 0;
+    drop_graph(b);
+    return XMC_OK;
+}
+
+int xmc_batch_create(const xmc_input* configs, int n, const xmc_options* opt, xmc_batch** out) {
+    if (!configs || n <= 0 || !out) return fail(XMC_ERR_ARG, "bad batch arguments");
+    *out = nullptr;
+    xmc_options o;
+    std::memset(&o, 0, sizeof(o));
+    if (opt) o = *opt;
+    auto* b = new xmc_batch();
+    std::unique_ptr<xmc_batch, int (*)(xmc_batch*)> guard(b, xmc_batch_destroy);
+    b->device = o.device;
+    b->n = n;
+    b->prefix.assign(n + 1, 0);
+    std::vector<xmc::Params> host(n);
+    for (int c = 0; c < n; c++) {
+        xmc_scene* m = nullptr;
+        int rc = scene_init(&configs[c], &o, false, &m);
+        if (rc) return fail(rc, "configuration " + std::to_string(c) + ": " + g_error);
+        b->members.push_back(m);
+        host[c] = m->base;
+        b->prefix[c + 1] = b->prefix[c] + configs[c].numberOfPhotons;
+        if (m->ref.depth > b->depth) b->depth = m->ref.depth;
+    }
+    if (b->prefix[n] > (1ull << xmc::kJumpBits)) return fail(XMC_ERR_LIMIT, "batch photon total beyond 2^40");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(b->cfgs.alloc(n));
+    HIP_TRY(hipMemcpy(b->cfgs.p, host.data(), n * sizeof(xmc::Params), hipMemcpyHostToDevice));
+    HIP_TRY(b->dprefix.alloc(n + 1));
+    HIP_TRY(hipMemcpy(b->dprefix.p, b->prefix.data(), (n + 1) * sizeof(uint64_t), hipMemcpyHostToDevice));
+    HIP_TRY(b->counters.alloc(17));
+    HIP_TRY(hipMemset(b->counters.p, 0, 17 * 8));
+    b->run = host[0];
+    b->run.tally.stats = b->counters.p;
+    b->run.queue = b->counters.p + 16;
+    b->run.first_photon = 0;
+    b->run.n_photons = b->prefix[n];
+    std::memset(&b->run.trace, 0, sizeof(b->run.trace));
+    HIP_TRY(hipStreamCreate(&b->stream));
+    int rc = configure_launch(b->device, b->depth, o.blockSize, o.blocksPerCU, xmc_batch_occupancy, &b->blocks,
+                              b->spill, b->run);
+    if (rc) return rc;
+    b->block = o.blockSize > 0 ? o.blockSize : 256;
+    *out = guard.release();
+    return XMC_OK;
+}
+
+// Runs every configuration's whole photon budget (seed `seed`, photon p of
+// configuration c drawing stream block prefix[c] + p) as one launch.
+int xmc_batch_run(xmc_batch* b, uint64_t seed, void* stream) {
+    if (!b) return fail(XMC_ERR_ARG, "batch is NULL");
+    HIP_TRY(hipSetDevice(b->device));
+    hipStream_t s = stream ? (hipStream_t)stream : b->stream;
+    b->run.seed_value = (2 * seed + 1) & xmc::kMcgMask;
+    size_t lds = transport_lds(b->block);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    b->pending.emplace_back(e0, e1);
+    if (b->use_graph) {
+        if (!b->graph_valid || b->graph_seed != seed) {
+            drop_graph(b);
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e1c = xmc_launch_zero(b->run.queue, 1, b->stream);
+            hipError_t e2c = xmc_launch_batch(&b->run, b->cfgs.p, b->dprefix.p, b->n, b->blocks, b->block, lds,
+                                              b->stream);
+            hipError_t ec = hipStreamEndCapture(b->stream, &b->graph);
+            if (e1c != hipSuccess || e2c != hipSuccess || ec != hipSuccess)
+                return fail(XMC_ERR_HIP, "graph capture failed");
+            HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->graph_valid = true;
+            b->graph_seed = seed;
+        }
+        HIP_TRY(hipEventRecord(e0, s));
+        HIP_TRY(hipGraphLaunch(b->exec, s));
+        HIP_TRY(hipEventRecord(e1, s));
+    } else {
+        HIP_TRY(xmc_launch_zero(b->run.queue, 1, s));
+        HIP_TRY(hipEventRecord(e0, s));
+        HIP_TRY(xmc_launch_batch(&b->run, b->cfgs.p, b->dprefix.p, b->n, b->blocks, b->block, lds, s));
+        HIP_TRY(hipEventRecord(e1, s));
+    }
+    return XMC_OK;
+}
+
+int xmc_batch_collect(xmc_batch* b, xmc_output* outputs) {
+    if (!b || !outputs) return fail(XMC_ERR_ARG, "batch/outputs is NULL");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (int c = 0; c < b->n; c++) {
+        int rc = collect_impl(b->members[c], &outputs[c], b->stream);
+        if (rc) return rc;
+    }
+    return XMC_OK;
+}
+
+int xmc_batch_stats(xmc_batch* b, xmc_stats* st) {
+    if (!b || !st) return fail(XMC_ERR_ARG, "batch/stats is NULL");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipDeviceSynchronize());
+    double ms = 0.0;
+    int rc = drain_timers(b->pending, &ms);
+    if (rc) return rc;
+    rc = read_stats(b->counters.p, b->stream, st);
+    st->kernelMs = ms;
+    return rc;
+}
+
+int xmc_batch_get_launch(xmc_batch* b, int* block, int* blocks) {
+    if (!b || !block || !blocks) return fail(XMC_ERR_ARG, "NULL argument");
+    *block = b->block;
+    *blocks = b->blocks;
+    return XMC_OK;
 }
 
 }  // extern "C"

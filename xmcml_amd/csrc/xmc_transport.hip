@@ -511,8 +511,13 @@ __device__ __forceinline__ uint64_t readlane64(uint64_t v, int lane) {
 // Weight ledger slots (per block in LDS, flushed to stats words 8..15).
 enum { kAbs = 0, kEsc, kDet, kArea, kKill, kRLost, kRGain, kOther, kLedger };
 
-template <bool TRACE>
-__global__ void __launch_bounds__(256) transport_kernel(Params P) {
+// The photon loop.  Run-wide fields (photon range, queue, streams, stack
+// spill, trace, counters) come from P; per-configuration fields (geometry,
+// optics, detectors, tallies, source, termination) from CFG(), which is P
+// itself for a single scene and the lane's entry of `cfgs` in a batch.
+template <bool TRACE, bool BATCHED>
+__device__ __forceinline__ void transport_body(const Params& P, const Params* __restrict__ cfgs,
+                                               const uint64_t* __restrict__ prefix, int ncfg) {
     extern __shared__ int2 lds_stack[];
     const int tid = threadIdx.x;
     const int lane = __lane_id();
@@ -523,8 +528,11 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
     if (tid < kLedger) ledger[tid] = 0.0;
     __syncthreads();
     double w_abs = 0.0;
-    Geo geo{P.nodes, P.tris, P.root_ref, {P.root_lo[0], P.root_lo[1], P.root_lo[2]},
-            {P.root_hi[0], P.root_hi[1], P.root_hi[2]}};
+    const Params* cfg = &P;
+    auto CFG = [&]() -> const Params& {
+        if constexpr (BATCHED) return *cfg;
+        else return P;
+    };
 
     V3 pos = {0, 0, 0}, dir = {0, 0, 1};
     float w = 0.0f, tim = 0.0f, trange = 0.0f, orange = 0.0f;
@@ -539,12 +547,13 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
     unsigned long long n_scatter = 0, n_boundary = 0, n_roulette = 0, n_photons = 0;
 
     auto load_layer = [&](int l) {
-        Ln = P.layers.n[l];
-        Lmua = P.layers.mua[l];
-        Lmut = P.layers.mua[l] + P.layers.mus[l];
-        Lg = P.layers.g[l];
-        Lglass = (P.layers.glass_mask >> l) & 1u;
-        Ltarget = (P.layers.target_mask >> l) & 1u;
+        const Layers& L = CFG().layers;
+        Ln = L.n[l];
+        Lmua = L.mua[l];
+        Lmut = L.mua[l] + L.mus[l];
+        Lg = L.g[l];
+        Lglass = (L.glass_mask >> l) & 1u;
+        Ltarget = (L.target_mask >> l) & 1u;
     };
 
     while (true) {
@@ -562,15 +571,26 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
                 uint64_t idx = base + rank;
                 if (idx < P.n_photons) {
                     uint64_t pid = P.first_photon + idx;
+                    if constexpr (BATCHED) {
+                        // configuration c owns photons [prefix[c], prefix[c+1])
+                        int lo = 0, hi = ncfg;
+                        while (hi - lo > 1) {
+                            int mid = (lo + hi) >> 1;
+                            if (prefix[mid] <= idx) lo = mid;
+                            else hi = mid;
+                        }
+                        cfg = cfgs + lo;
+                    }
                     x = P.seed_value;
                     for (int i = 0; i < kJumpBits; ++i)
                         if ((pid >> i) & 1ull) x = (x * P.jump[i]) & kMcgMask;
                     // LaunchPhoton (:128-149); float start offset along +z
-                    V3 s = {P.start_pos[0], P.start_pos[1], P.start_pos[2]};
+                    const Params& C = CFG();
+                    V3 s = {C.start_pos[0], C.start_pos[1], C.start_pos[2]};
                     pos = s;
                     pos.z = s.z + offset_scale(s);
-                    dir = {P.start_dir[0], P.start_dir[1], P.start_dir[2]};
-                    w = P.w0;
+                    dir = {C.start_dir[0], C.start_dir[1], C.start_dir[2]};
+                    w = C.w0;
                     tim = 0.0f;
                     trange = 0.0f;
                     orange = 0.0f;
@@ -611,6 +631,9 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
             } while (u == 0.0f);
             step = -xlogf(u) / Lmut;
         }
+        const Params& C = CFG();
+        Geo geo{C.nodes, C.tris, C.root_ref, {C.root_lo[0], C.root_lo[1], C.root_lo[2]},
+                {C.root_hi[0], C.root_hi[1], C.root_hi[2]}};
         Ray ray = make_ray(pos, dir);
         Hit h = walk(geo, ray, step, !Lglass, st);
         if (h.found) {
@@ -628,18 +651,18 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
             pos.x = hit.x - ofs * nrm.x;
             pos.y = hit.y - ofs * nrm.y;
             pos.z = hit.z - ofs * nrm.z;
-            if (area_index(P.area, pos) >= 0) {
+            if (area_index(C.area, pos) >= 0) {
                 // CrossBoundary (:229-284)
                 float ci = dot(dir, nrm);
                 if (ci < 0.0f) ci = -ci;
                 int surface = __float_as_int(reinterpret_cast<const float4*>(geo.tris)[3 * h.tri].w);
-                int nxt = P.next_layer[layer * P.n_surfaces + surface];
+                int nxt = C.next_layer[layer * C.n_surfaces + surface];
                 if (nxt < 0) {
                     alive = false;
                     atomicAdd(ledger + kOther, (double)w);
                 } else {
-                    float n1 = Ln, n2 = P.layers.n[nxt];
-                    float cc = P.layers.crit[layer][nxt];
+                    float n1 = Ln, n2 = C.layers.n[nxt];
+                    float cc = C.layers.crit[layer][nxt];
                     float refl;
                     if (ci <= cc) {
                         refl = 1.0f;
@@ -652,21 +675,21 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
                     draws++;
                     if (u > refl) {
                         if (nxt == 0) {
-                            int id = detector_id(P.det, pos, dir, visited);
+                            int id = detector_id(C.det, pos, dir, visited);
                             if (id >= 0) {
                                 double wd = (double)w;
                                 float total = trange + orange;
                                 float tw = trange * (w / total);
-                                unsafeAtomicAdd(P.tally.det_weight + id, wd);
-                                atomicAdd(P.tally.det_count + id, 1ull);
-                                int b = time_bin(P.det, (double)tim);
+                                unsafeAtomicAdd(C.tally.det_weight + id, wd);
+                                atomicAdd(C.tally.det_count + id, 1ull);
+                                int b = time_bin(C.det, (double)tim);
                                 if (b >= 0) {
-                                    size_t k = (size_t)id * P.det.T + b;
-                                    unsafeAtomicAdd(P.tally.bin_target + k, (double)tw);
-                                    atomicAdd(P.tally.bin_count + k, 1ull);
-                                    unsafeAtomicAdd(P.tally.bin_weight + k, wd);
+                                    size_t k = (size_t)id * C.det.T + b;
+                                    unsafeAtomicAdd(C.tally.bin_target + k, (double)tw);
+                                    atomicAdd(C.tally.bin_count + k, 1ull);
+                                    unsafeAtomicAdd(C.tally.bin_weight + k, wd);
                                 }
-                                unsafeAtomicAdd(P.tally.det_target + id, (double)tw);
+                                unsafeAtomicAdd(C.tally.det_target + id, (double)tw);
                                 atomicAdd(ledger + kDet, wd);
                             }
                             atomicAdd(ledger + kEsc, (double)w);
@@ -710,11 +733,11 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
             pos.x += step * dir.x;
             pos.y += step * dir.y;
             pos.z += step * dir.z;
-            int idx = area_index(P.area, pos);
+            int idx = area_index(C.area, pos);
             if (idx >= 0) {
                 float dw = w * Lmua / Lmut;  // ComputeDroppedWeightOfPhoton (:633-642)
                 w -= dw;
-                if (P.tally.absorption) unsafeAtomicAdd(P.tally.absorption + idx, (double)dw);
+                if (C.tally.absorption) unsafeAtomicAdd(C.tally.absorption + idx, (double)dw);
                 w_abs += (double)dw;
                 dir = spin(dir, Lg, x, draws);
                 n_scatter++;
@@ -726,8 +749,8 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
             }
         }
         // ---- termination: xmcml kill (:121-124) or MCML roulette
-        if (P.roulette) {
-            if (w < P.min_weight && alive) {
+        if (C.roulette) {
+            if (w < C.min_weight && alive) {
                 n_roulette++;
                 if (w == 0.0f) {
                     alive = false;
@@ -735,9 +758,9 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
                 } else {
                     float u = next_u(x);
                     draws++;
-                    if (u < P.chance) {
+                    if (u < C.chance) {
                         float before = w;
-                        w /= P.chance;
+                        w /= C.chance;
                         atomicAdd(ledger + kRGain, (double)w - (double)before);
                         rec.put(XMC_EV_ROULETTE_LIVE, layer, 0, draws, pos, dir, w, tim);
                     } else {
@@ -747,7 +770,7 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
                     }
                 }
             }
-        } else if (w < P.min_weight) {
+        } else if (w < C.min_weight) {
             if (alive) {
                 atomicAdd(ledger + kKill, (double)w);
                 rec.put(XMC_EV_DIE_WEIGHT, layer, 0, draws, pos, dir, w, tim);
@@ -767,6 +790,19 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
     atomicAdd(ledger + kAbs, w_abs);
     __syncthreads();
     if (tid < kLedger) unsafeAtomicAdd(reinterpret_cast<double*>(P.tally.stats + 8) + tid, ledger[tid]);
+}
+
+template <bool TRACE>
+__global__ void __launch_bounds__(256) transport_kernel(Params P) {
+    transport_body<TRACE, false>(P, nullptr, nullptr, 0);
+}
+
+// Many configurations in one launch: P carries the run-wide fields and the
+// photon range [0, prefix[ncfg]); cfgs[c] the geometry, optics, detectors and
+// tallies of configuration c.
+__global__ void __launch_bounds__(256) batch_kernel(Params P, const Params* __restrict__ cfgs,
+                                                    const uint64_t* __restrict__ prefix, int ncfg) {
+    transport_body<false, true>(P, cfgs, prefix, ncfg);
 }
 
 template __global__ void transport_kernel<false>(Params);
@@ -857,6 +893,16 @@ hipError_t xmc_launch_zero(unsigned long long* p, size_t n, hipStream_t s) {
 
 hipError_t xmc_transport_occupancy(int block, size_t lds, int* blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, xmc::transport_kernel<false>, block, lds);
+}
+
+hipError_t xmc_launch_batch(const xmc::Params* p, const xmc::Params* cfgs, const uint64_t* prefix, int ncfg, int grid,
+                            int block, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(xmc::batch_kernel, dim3(grid), dim3(block), lds, s, *p, cfgs, prefix, ncfg);
+    return hipGetLastError();
+}
+
+hipError_t xmc_batch_occupancy(int block, size_t lds, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, xmc::batch_kernel, block, lds);
 }
 
 }  // extern "C"

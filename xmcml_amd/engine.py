@@ -28,7 +28,9 @@ _lib = None
 
 EXPORTS = [
     "xmc_scene_create", "xmc_scene_destroy", "xmc_scene_run", "xmc_scene_sync", "xmc_scene_collect", "xmc_scene_stats",
-    "xmc_scene_trace", "xmc_scene_query", "xmc_scene_bvh", "xmc_launch", "xmc_init_output",
+    "xmc_scene_trace", "xmc_scene_query", "xmc_scene_bvh", "xmc_scene_set_launch", "xmc_scene_get_launch",
+    "xmc_batch_create", "xmc_batch_destroy", "xmc_batch_run", "xmc_batch_collect", "xmc_batch_stats",
+    "xmc_batch_set_launch", "xmc_batch_get_launch", "xmc_launch", "xmc_init_output",
     "xmc_free_output", "xmc_specular_reflectance", "xmc_last_error", "xmc_version",
 ]
 
@@ -60,6 +62,15 @@ def lib():
     sig("xmc_scene_trace", _I, [_P, _U64, _U64, _U64, _P, _I, _PI])
     sig("xmc_scene_query", _I, [_P, _PD, _PD, _PD, _I, _I, _PI, _PD, _PI, _PI, _PD])
     sig("xmc_scene_bvh", _I, [_P, _PI, _PI, _PI, _PD, _PI, _PI, _PI])
+    sig("xmc_scene_set_launch", _I, [_P, _I, _I])
+    sig("xmc_scene_get_launch", _I, [_P, _PI, _PI])
+    sig("xmc_batch_create", _I, [_P, _I, _P, C.POINTER(_P)])
+    sig("xmc_batch_destroy", _I, [_P])
+    sig("xmc_batch_run", _I, [_P, _U64, _P])
+    sig("xmc_batch_collect", _I, [_P, _P])
+    sig("xmc_batch_stats", _I, [_P, _P])
+    sig("xmc_batch_set_launch", _I, [_P, _I, _I, _I])
+    sig("xmc_batch_get_launch", _I, [_P, _PI, _PI])
     sig("xmc_launch", _I, [_P, _P, _P, _P])
     sig("xmc_init_output", _I, [_P, _P])
     sig("xmc_free_output", None, [_P])
@@ -153,6 +164,14 @@ class Scene:
                                      _ip(surf), _ip(tri), _dp(nrm)), "xmc_scene_query")
         return dict(found=found, distance=dist, surface=surf, triangle=tri, normal=nrm)
 
+    def set_launch(self, block: int, blocks_per_cu: int = 0):
+        _check(lib().xmc_scene_set_launch(self._h, block, blocks_per_cu), "xmc_scene_set_launch")
+
+    def get_launch(self):
+        b, n = C.c_int(), C.c_int()
+        _check(lib().xmc_scene_get_launch(self._h, C.byref(b), C.byref(n)), "xmc_scene_get_launch")
+        return b.value, n.value
+
     def bvh(self) -> dict:
         nn, root, nt = C.c_int(), C.c_int(), C.c_int()
         _check(lib().xmc_scene_bvh(self._h, C.byref(nn), C.byref(root), C.byref(nt), None, None, None, None),
@@ -174,3 +193,65 @@ def launch(spec: SceneSpec, out: OutputHandle, seed: int, first: int, n: int, ab
     opt = Options(0, 0, 0, 0 if absorption else 1)
     _check(lib().xmc_launch(inp.ptr, out.ptr, C.byref(rng), C.byref(opt)), "xmc_launch")
     return rng.nextPhoton
+
+
+class Batch:
+    """Many configurations run by one persistent launch (replayed as a HIP
+    graph); configuration c's budget is specs[c].photons."""
+
+    def __init__(self, specs, block_size: int = 0, blocks_per_cu: int = 0, absorption: bool = False,
+                 use_graph: bool = True, device: int = 0):
+        from .abi import Input
+        self.specs = list(specs)
+        self.inputs = [InputHandle(s) for s in self.specs]
+        arr = (Input * len(self.inputs))(*[h.c for h in self.inputs])
+        self._arr = arr
+        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1)
+        h = _P()
+        _check(lib().xmc_batch_create(arr, len(self.inputs), C.byref(opt), C.byref(h)), "xmc_batch_create")
+        self._h = h
+        self.absorption = absorption
+        if not use_graph or block_size or blocks_per_cu:
+            self.set_launch(block_size, blocks_per_cu, use_graph)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().xmc_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_launch(self, block: int, blocks_per_cu: int = 0, use_graph: bool = True):
+        _check(lib().xmc_batch_set_launch(self._h, block, blocks_per_cu, int(use_graph)), "xmc_batch_set_launch")
+
+    def get_launch(self):
+        b, n = C.c_int(), C.c_int()
+        _check(lib().xmc_batch_get_launch(self._h, C.byref(b), C.byref(n)), "xmc_batch_get_launch")
+        return b.value, n.value
+
+    def run(self, seed: int, stream: Optional[int] = None):
+        _check(lib().xmc_batch_run(self._h, seed, stream), "xmc_batch_run")
+
+    def collect(self):
+        from .abi import Output
+        outs = [OutputHandle(s, absorption=self.absorption) for s in self.specs]
+        arr = (Output * len(outs))(*[o.c for o in outs])
+        _check(lib().xmc_batch_collect(self._h, arr), "xmc_batch_collect")
+        for o, c in zip(outs, arr):
+            o.c = c
+        return outs
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().xmc_batch_stats(self._h, C.byref(s)), "xmc_batch_stats")
+        return {k: getattr(s, k) for k, _ in Stats._fields_}

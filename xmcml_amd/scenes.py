@@ -152,3 +152,56 @@ def small_scene(kind: str, photons: int = 2000) -> SceneSpec:
         s.name = "heightfield-small"
         return s
     raise ValueError(kind)
+
+
+def layered_scene(layers, photons: int, rings_n: int = 32, ring_width: float = 0.5, time_bins: int = 64,
+                  time_ns: float = 1.0, side: float = 200.0, name: str = "layered") -> SceneSpec:
+    """Planar stack of tissue layers [(n, mua, mus, g, thickness), ...] under an
+    ambient n = 1 half-space, pencil beam at the origin, `rings_n` rings of
+    `ring_width` mm, a `time_bins`-bin time-of-flight histogram over
+    0..time_ns ns per ring, roulette 1e-4 / 0.1."""
+    z = [0.0]
+    for l in layers:
+        z.append(z[-1] + l[4])
+    surfaces = [plane(zz, side=side) for zz in z]
+    nl = len(layers)
+    specs = [LayerSpec(1.0, 0.0, 0.0, 0.0, [0, nl])]
+    for i, (ri, mua, mus, g, _) in enumerate(layers):
+        specs.append(LayerSpec(ri, mua, mus, g, [i, i + 1]))
+    half = side / 2.0
+    return SceneSpec(layers=specs, surfaces=surfaces, area_corner=(-half, -half, 0.0),
+                     area_length=(side, side, z[-1] * (1.0 + 1e-9) + 1e-6), area_partitions=(1, 1, 1),
+                     rings=rings(rings_n, ring_width), min_weight=1e-4, termination=TERMINATE_ROULETTE,
+                     roulette_chance=0.1, time_start=0.0, time_finish=SPEED_OF_LIGHT_MM_PER_NS * time_ns,
+                     time_bins=time_bins, photons=photons, name=name)
+
+
+def draw_layers(rng: np.random.Generator, n_layers: int):
+    """Layer parameters of the batched workload: log-uniform mua in [0.001, 1]/mm,
+    reduced scattering mus' log-uniform in [0.5, 5]/mm (mus = mus'/(1-g)),
+    g uniform in [0.7, 0.95], n uniform in [1.33, 1.5], thickness log-uniform
+    in [0.05, 10] mm."""
+    out = []
+    for _ in range(n_layers):
+        mua = math.exp(rng.uniform(math.log(1e-3), math.log(1.0)))
+        musp = math.exp(rng.uniform(math.log(0.5), math.log(5.0)))
+        g = rng.uniform(0.7, 0.95)
+        n = rng.uniform(1.33, 1.5)
+        d = math.exp(rng.uniform(math.log(0.05), math.log(10.0)))
+        out.append((n, mua, musp / (1.0 - g), g, d))
+    return out
+
+
+def batched_configs(count: int = 1024, seed: int = 3, photon_scale: float = 1.0,
+                    min_photons: float = 1e5, max_photons: float = 1e7) -> List[SceneSpec]:
+    """The batched workload: `count` layered configurations of 2-5 layers with
+    parameters from `draw_layers` and photon budgets log-uniform in
+    [min_photons, max_photons] (times photon_scale), all drawn from `seed`.
+    Outputs per configuration: 32 rings of 0.5 mm and a 64-bin TOF over 0-1 ns."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(count):
+        layers = draw_layers(rng, int(rng.integers(2, 6)))
+        budget = math.exp(rng.uniform(math.log(min_photons), math.log(max_photons)))
+        out.append(layered_scene(layers, max(1, int(budget * photon_scale)), name=f"batch-{seed}-{c}"))
+    return out
