@@ -140,13 +140,20 @@ constexpr double kPi = 3.14159265358979323846;
 
 template <class R> struct Prec;
 
+// Precision 65: double arithmetic driven by the 24-bit uniforms the float
+// walk draws (the same recorded sequence), for float-vs-double comparisons
+// that isolate arithmetic from uniform quantisation.
+static bool g_uniform24 = false;
+
 template <> struct Prec<double> {
     static constexpr double kCosZero = 1.0 - 1.0e-12;  // mcml_kernel.cpp:11
     static constexpr double kCos90 = 1.0e-12;           // :12
     static constexpr double kMinDistance = 1.0e-8;      // :13
     static constexpr double kMaxDistance = 1.0e256;     // mcml_intersection.h:7
     static constexpr double kEps = 1.0e-6;              // mcml_intersection.h:6
-    static double uniform(uint64_t x) { return (double)x / (double)kMcgM; }  // mcml_mcg59.cpp:40
+    static double uniform(uint64_t x) {  // mcml_mcg59.cpp:40
+        return g_uniform24 ? (double)(x >> 35) * 0x1p-24 : (double)x / (double)kMcgM;
+    }
     static double log_(double u) { return log(u); }
     static void sincos_2pi(double u, double* s, double* c) {
         double psi = 2.0 * kPi * u;  // mcml_kernel.cpp:453-455
@@ -162,9 +169,13 @@ template <> struct Prec<double> {
 //  - after each spin the direction gets one first-order renormalisation
 //    step, d *= 1.5 - 0.5 |d|^2 (the rotation formula does not preserve
 //    |d| = 1 and float rounding would otherwise drift it by ~1e-4);
-//  - 1 - x^2 is evaluated as (1 - x)(1 + x), and the "cosine of zero"
-//    threshold is 1 - 1e-6 (1 - 1e-12 rounds to 1.0f, which would send
-//    normal-incidence directions through a 0/0 in the spin);
+//  - the scattering angle comes from hg_float (1 - cos and 1 + cos without
+//    cancellation) instead of cos first, which loses sin(theta) for the
+//    strongly forward scatters of g ~ 0.9;
+//  - 1 - x^2 is evaluated as (1 - x)(1 + x); in the spin, sin(theta_u) is
+//    sqrt(ux^2 + uy^2) and the rotation is used when that square is above
+//    2e-12 (the reference's |uz| <= 1 - 1e-12, which rounds to 1.0f in
+//    float); in the Fresnel formulas the "cosine of zero" is 1 - 1e-6;
 //  - the 1e-8 mm nudges of the reference (launch, the 1e-10 mm stop short of
 //    a surface, and the step after every reflection or refraction) are below
 //    the float spacing of coordinates, so they are replaced by an offset
@@ -226,7 +237,12 @@ inline V3<double> plane_segment_point(V3<double> a, V3<double> b, double z) {
     return r;
 }
 
-template <class R> V3<R> cvt(const xmc_double3& v) { return {(R)v.x, (R)v.y, (R)v.z}; }
+// Precision 65 also feeds the double walk the float-rounded inputs the float
+// walk sees (optics, geometry, detectors, source), so that a float-vs-double
+// comparison isolates the arithmetic.
+inline double in_val(double v) { return g_uniform24 ? (double)(float)v : v; }
+
+template <class R> V3<R> cvt(const xmc_double3& v) { return {(R)in_val(v.x), (R)in_val(v.y), (R)in_val(v.z)}; }
 
 // ----------------------------------------------------------------------------
 // geometry: triangle test, box test, BVH
@@ -473,8 +489,8 @@ template <class R> struct Scene {
         boxes.resize(bvh.nodes.size());
         for (size_t i = 0; i < bvh.nodes.size(); i++) {
             const Node& n = bvh.nodes[i];
-            boxes[i].lo = {(R)n.box.lo.x, (R)n.box.lo.y, (R)n.box.lo.z};
-            boxes[i].hi = {(R)n.box.hi.x, (R)n.box.hi.y, (R)n.box.hi.z};
+            boxes[i].lo = {(R)in_val(n.box.lo.x), (R)in_val(n.box.lo.y), (R)in_val(n.box.lo.z)};
+            boxes[i].hi = {(R)in_val(n.box.hi.x), (R)in_val(n.box.hi.y), (R)in_val(n.box.hi.z)};
         }
         Mesh m{input->surface, input->numberOfSurfaces};
         tv.resize(bvh.tris.size() * 3);
@@ -483,8 +499,8 @@ template <class R> struct Scene {
         layers.resize(input->numberOfLayers);
         for (int i = 0; i < input->numberOfLayers; i++) {
             const xmc_layer& l = input->layerInfo[i];
-            layers[i] = {(R)l.refractiveIndex, (R)l.absorptionCoefficient, (R)l.scatteringCoefficient,
-                         (R)l.anisotropy,
+            layers[i] = {(R)in_val(l.refractiveIndex), (R)in_val(l.absorptionCoefficient),
+                         (R)in_val(l.scatteringCoefficient), (R)in_val(l.anisotropy),
                          l.absorptionCoefficient == 0.0 && l.scatteringCoefficient == 0.0};
         }
         area_corner = cvt<R>(input->area->corner);
@@ -493,6 +509,7 @@ template <class R> struct Scene {
         part[1] = input->area->partitionNumber.y;
         part[2] = input->area->partitionNumber.z;
         for (int a = 0; a < 3; a++) area_part[a] = (R)part[a];
+        (void)0;
         for (int i = 0; i < input->numberOfCubeDetectors; i++) {
             const xmc_cube_detector& c = input->cubeDetector[i];
             Cube q;
@@ -506,16 +523,16 @@ template <class R> struct Scene {
             const xmc_ring_detector& c = input->ringDetector[i];
             Ring q;
             q.center = cvt<R>(c.center);
-            q.rs = (R)c.smallRadius;
-            q.rb = (R)c.bigRadius;
+            q.rs = (R)in_val(c.smallRadius);
+            q.rb = (R)in_val(c.bigRadius);
             std::memcpy(q.filter, c.filterLayers, XMC_MAX_LAYERS);
             q.cos_target = (R)cos(c.targetAngle * kPi / 180.0);
             rings.push_back(q);
         }
         rsp = xmco_specular(input->layerInfo);
-        w0 = (R)(1.0 - rsp);
-        min_weight = (R)input->minWeight;
-        chance = (R)input->rouletteChance;
+        w0 = (R)in_val(1.0 - rsp);
+        min_weight = (R)in_val(input->minWeight);
+        chance = (R)in_val(input->rouletteChance);
         start_pos = cvt<R>(input->startPosition);
         start_dir = cvt<R>(input->startDirection);
         int T = input->timeScaleSize;
@@ -772,15 +789,48 @@ template <class R> R cosine_theta(R g, Mcg59& rng) {  // :475-495
     return c;
 }
 
+// Float only: the Henyey-Greenstein inversion of ComputeCosineTheta
+// (:475-495) rewritten so that 1 - cos(theta) and 1 + cos(theta) come out
+// without cancellation.  With D = 1 - g + 2 g u and t = (1 - g^2)/D:
+//   1 - cos = (1 - g)(1 - u)(t + 1 - g)/D,   1 + cos = (1 + g) u (1 + g + t)/D,
+// exact rearrangements of cos = (1 + g^2 - t^2)/(2g) that also give the
+// isotropic 2(1 - u), 2u at g = 0.  sin(theta) = sqrt((1 - cos)(1 + cos)).
+inline void hg_float(float g, float u, float* c, float* s) {
+    float a = 1.0f - g, b = 1.0f + g;
+    float inv = 1.0f / (a + 2.0f * g * u);
+    float t = (1.0f - g * g) * inv;
+    float vers = a * (1.0f - u) * (t + a) * inv;
+    float cove = b * u * (b + t) * inv;
+    *c = vers < 1.0f ? 1.0f - vers : cove - 1.0f;
+    *s = std::sqrt(vers * cove);
+}
+
 template <class R> V3<R> new_direction(V3<R> d, R g, Mcg59& rng) {  // :442-473
     V3<R> r;
     R ux = d.x, uy = d.y, uz = d.z;
-    R ct = cosine_theta(g, rng);
-    R st = std::sqrt(Prec<R>::one_minus_sq(ct));
+    R ct, st;
+    if constexpr (sizeof(R) == 4) {
+        hg_float(g, Prec<R>::uniform(rng.next()), &ct, &st);
+    } else {
+        ct = cosine_theta(g, rng);
+        st = std::sqrt(Prec<R>::one_minus_sq(ct));
+    }
     R sp, cp;
     Prec<R>::sincos_2pi(Prec<R>::uniform(rng.next()), &sp, &cp);
-    if (std::fabs(uz) <= Prec<R>::kCosZero) {
-        R tmp = std::sqrt(Prec<R>::one_minus_sq(uz));
+    // float: sin(theta_u) from the components themselves, sqrt(ux^2 + uy^2),
+    // so the local frame stays orthonormal near the pole; the branch is the
+    // reference's |uz| <= 1 - 1e-12, i.e. ux^2 + uy^2 >= 2e-12
+    bool rotate;
+    R tmp;
+    if constexpr (sizeof(R) == 4) {
+        R s2 = ux * ux + uy * uy;
+        rotate = s2 > (R)2e-12;
+        tmp = std::sqrt(s2);
+    } else {
+        rotate = std::fabs(uz) <= Prec<R>::kCosZero;
+        tmp = rotate ? std::sqrt(Prec<R>::one_minus_sq(uz)) : (R)0;
+    }
+    if (rotate) {
         r.x = st * (ux * uz * cp - uy * sp) / tmp + ux * ct;
         r.y = st * (uy * uz * cp + ux * sp) / tmp + uy * ct;
         r.z = -st * cp * tmp + uz * ct;
@@ -1087,88 +1137,94 @@ template <class R> struct Walker {
         for (int i = 0; i < XMC_MAX_LAYERS; ++i) p.visited[i] = false;
         p.visited[p.layer] = true;
         record(XMC_EV_LAUNCH, p, 0, rng);
-        const R step_unlimited = Prec<R>::kMaxDistance;
-        while (!p.dead) {
-            if (rng.draws > kDrawLimit) {  // stream block exhausted: retire the photon
-                p.dead = true;
-                tl.w_other += (double)p.weight;
-                break;
-            }
-            const auto& L = sc.layers[p.layer];
-            Hit<R> h;
-            if (L.glass) {
-                h = sc.intersect(p.pos, p.dir, step_unlimited, false);
-            } else {
-                p.step = step_size(L.mua, L.mus, rng);
-                h = sc.intersect(p.pos, p.dir, p.step, true);
-            }
-            int v[3];
-            if (h.found) {
-                p.step = h.distance - (R)0.01 * Prec<R>::kMinDistance;
-                move(p);
-                V3<R> hit = p.pos;
-                if constexpr (sizeof(R) == 4) {
-                    R d = offset_scale(hit);
-                    if (dot(p.dir, h.normal) < (R)0) d = -d;
-                    p.pos.x = hit.x - d * h.normal.x;
-                    p.pos.y = hit.y - d * h.normal.y;
-                    p.pos.z = hit.z - d * h.normal.z;
-                }
-                if (sc.area_index(p.pos, v) >= 0) {
-                    cross(p, h, hit, rng);
-                } else {
-                    p.dead = true;
-                    tl.w_area += (double)p.weight;
-                    record(XMC_EV_DIE_AREA, p, 0, rng);
-                }
-            } else if (L.glass) {
-                p.dead = true;
-                tl.w_other += (double)p.weight;
-                record(XMC_EV_GLASS_MISS, p, 0, rng);
-            } else {
-                move(p);
-                int idx = sc.area_index(p.pos, v);
-                if (idx >= 0) {
-                    R dw = p.weight * L.mua / (L.mua + L.mus);  // :633-642
-                    p.weight -= dw;
-                    if (!tl.absorption.empty()) tl.absorption[idx] += (double)dw;
-                    tl.w_abs += (double)dw;
-                    p.dir = new_direction(p.dir, L.g, rng);
-                    tl.scatter++;
-                    record(XMC_EV_SCATTER, p, idx, rng);
-                } else {
-                    p.dead = true;
-                    tl.w_area += (double)p.weight;
-                    record(XMC_EV_DIE_AREA, p, 0, rng);
-                }
-            }
-            if (in->terminationMode == XMC_TERMINATE_ROULETTE) {
-                if (p.weight < sc.min_weight && !p.dead) {
-                    tl.roulette++;
-                    if (p.weight == (R)0.0) {
-                        p.dead = true;
-                        record(XMC_EV_ROULETTE_DIE, p, 0, rng);
-                    } else if (Prec<R>::uniform(rng.next()) < sc.chance) {
-                        R before = p.weight;
-                        p.weight /= sc.chance;
-                        tl.w_rgain += (double)p.weight - (double)before;
-                        record(XMC_EV_ROULETTE_LIVE, p, 0, rng);
-                    } else {
-                        p.dead = true;
-                        tl.w_rlost += (double)p.weight;
-                        record(XMC_EV_ROULETTE_DIE, p, 0, rng);
-                    }
-                }
-            } else if (p.weight < sc.min_weight) {
-                if (!p.dead) {
-                    tl.w_kill += (double)p.weight;
-                    record(XMC_EV_DIE_WEIGHT, p, 0, rng);
-                }
-                p.dead = true;
-            }
-        }
+        while (!p.dead) interact(p, rng);
         tl.draws += rng.draws;
         tl.photons++;
+    }
+
+    // One iteration of ComputePhoton's loop (:47-124) plus termination.
+    void interact(Photon<R>& p, Mcg59& rng) {
+        const xmc_input* in = sc.in;
+        const R step_unlimited = Prec<R>::kMaxDistance;
+
+        if (rng.draws > kDrawLimit) {  // stream block exhausted: retire the photon
+            p.dead = true;
+            tl.w_other += (double)p.weight;
+            return;
+        }
+        const auto& L = sc.layers[p.layer];
+        Hit<R> h;
+        if (L.glass) {
+            h = sc.intersect(p.pos, p.dir, step_unlimited, false);
+        } else {
+            p.step = step_size(L.mua, L.mus, rng);
+            h = sc.intersect(p.pos, p.dir, p.step, true);
+        }
+        int v[3];
+        if (h.found) {
+            p.step = h.distance - (R)0.01 * Prec<R>::kMinDistance;
+            move(p);
+            V3<R> hit = p.pos;
+            if constexpr (sizeof(R) == 4) {
+                R d = offset_scale(hit);
+                if (dot(p.dir, h.normal) < (R)0) d = -d;
+                p.pos.x = hit.x - d * h.normal.x;
+                p.pos.y = hit.y - d * h.normal.y;
+                p.pos.z = hit.z - d * h.normal.z;
+            }
+            if (sc.area_index(p.pos, v) >= 0) {
+                cross(p, h, hit, rng);
+            } else {
+                p.dead = true;
+                tl.w_area += (double)p.weight;
+                record(XMC_EV_DIE_AREA, p, 0, rng);
+            }
+        } else if (L.glass) {
+            p.dead = true;
+            tl.w_other += (double)p.weight;
+            record(XMC_EV_GLASS_MISS, p, 0, rng);
+        } else {
+            move(p);
+            int idx = sc.area_index(p.pos, v);
+            if (idx >= 0) {
+                R dw = p.weight * L.mua / (L.mua + L.mus);  // :633-642
+                p.weight -= dw;
+                if (!tl.absorption.empty()) tl.absorption[idx] += (double)dw;
+                tl.w_abs += (double)dw;
+                p.dir = new_direction(p.dir, L.g, rng);
+                tl.scatter++;
+                record(XMC_EV_SCATTER, p, idx, rng);
+            } else {
+                p.dead = true;
+                tl.w_area += (double)p.weight;
+                record(XMC_EV_DIE_AREA, p, 0, rng);
+            }
+        }
+        if (in->terminationMode == XMC_TERMINATE_ROULETTE) {
+            if (p.weight < sc.min_weight && !p.dead) {
+                tl.roulette++;
+                if (p.weight == (R)0.0) {
+                    p.dead = true;
+                    record(XMC_EV_ROULETTE_DIE, p, 0, rng);
+                } else if (Prec<R>::uniform(rng.next()) < sc.chance) {
+                    R before = p.weight;
+                    p.weight /= sc.chance;
+                    tl.w_rgain += (double)p.weight - (double)before;
+                    record(XMC_EV_ROULETTE_LIVE, p, 0, rng);
+                } else {
+                    p.dead = true;
+                    tl.w_rlost += (double)p.weight;
+                    record(XMC_EV_ROULETTE_DIE, p, 0, rng);
+                }
+            }
+        } else if (p.weight < sc.min_weight) {
+            if (!p.dead) {
+                tl.w_kill += (double)p.weight;
+                record(XMC_EV_DIE_WEIGHT, p, 0, rng);
+            }
+            p.dead = true;
+        }
+        
     }
 };
 
@@ -1266,6 +1322,57 @@ int trace_impl(const xmc_input* in, uint64_t seed, uint64_t photon, xmc_trace_ev
     return w.n_ev <= max_ev ? w.n_ev : -w.n_ev;
 }
 
+// One interaction from each recorded state of a trace (a float walk of
+// `photon`): position, direction, weight, time and layer from the event,
+// visited layers accumulated along the trace, random stream advanced by the
+// recorded number of draws.  out[2i..2i+1] receive the events of the
+// iteration started from trace[i], counts[i] their number.
+template <class R>
+int shadow_impl(const xmc_input* in, uint64_t seed, uint64_t photon, const xmc_trace_event* trace, int n,
+                xmc_trace_event* out, int* counts) {
+    Scene<R> sc;
+    sc.init(in);
+    int ndet = in->numberOfCubeDetectors + in->numberOfRingDetectors;
+    Tally t;
+    t.init(1, ndet, in->timeScaleSize, false, false);
+    Walker<R> w(sc, t, false);
+    uint32_t visited = 1u << 1;
+    Mcg59 g = stream_for(seed, photon);
+    uint32_t drawn = 0;
+    for (int i = 0; i < n; ++i) {
+        const xmc_trace_event* st = trace + i;
+        visited |= 1u << st->layer;
+        while (drawn < st->draws) {
+            g.next();
+            drawn++;
+        }
+        Mcg59 gi = g;
+        gi.draws = st->draws;
+        Photon<R> p;
+        p.pos = {(R)st->pos[0], (R)st->pos[1], (R)st->pos[2]};
+        p.dir = {(R)st->dir[0], (R)st->dir[1], (R)st->dir[2]};
+        // the recorded float direction is unit only to float rounding; the
+        // double formulas (sqrt(1 - uz^2) in the spin) need it unit to double
+        // rounding, so replay from the normalised direction
+        if constexpr (sizeof(R) == 8) p.dir = normalize(p.dir);
+        p.weight = (R)st->weight;
+        p.time = (R)st->time;
+        p.step = 0;
+        p.target_range = 0;
+        p.other_range = 0;
+        p.layer = st->layer;
+        p.dead = st->type == XMC_EV_ESCAPE || st->type == XMC_EV_DIE_AREA || st->type == XMC_EV_DIE_WEIGHT ||
+                 st->type == XMC_EV_ROULETTE_DIE || st->type == XMC_EV_GLASS_MISS;
+        for (int k = 0; k < XMC_MAX_LAYERS; ++k) p.visited[k] = (visited >> k) & 1u;
+        w.ev = out + 2 * i;
+        w.max_ev = 2;
+        w.n_ev = 0;
+        if (!p.dead) w.interact(p, gi);
+        counts[i] = w.n_ev;
+    }
+    return XMC_OK;
+}
+
 template <class R>
 int query_impl(const xmc_input* in, const double* o, const double* d, const double* steps, int nr,
                int mode, int* found, double* dist, int* surf, int* tri, double* normal) {
@@ -1300,6 +1407,7 @@ uint64_t xmco_photon_stream(uint64_t seed, uint64_t photon) { return photon_stre
 int xmco_run(const xmc_input* input, xmc_output* output, uint64_t seed, uint64_t first, uint64_t n,
              int precision, int nthreads, int scattering, xmco_counters* counters) {
     if (!input || !output) return XMC_ERR_ARG;
+    g_uniform24 = precision == 65;
     if (precision == 32)
         return run_impl<float>(input, output, seed, first, n, nthreads, scattering != 0, counters);
     return run_impl<double>(input, output, seed, first, n, nthreads, scattering != 0, counters);
@@ -1307,8 +1415,16 @@ int xmco_run(const xmc_input* input, xmc_output* output, uint64_t seed, uint64_t
 
 int xmco_trace(const xmc_input* input, uint64_t seed, uint64_t photon, int precision,
                xmc_trace_event* events, int maxEvents) {
+    g_uniform24 = precision == 65;
     if (precision == 32) return trace_impl<float>(input, seed, photon, events, maxEvents);
     return trace_impl<double>(input, seed, photon, events, maxEvents);
+}
+
+int xmco_shadow_trace(const xmc_input* input, uint64_t seed, uint64_t photon, int precision,
+                      const xmc_trace_event* trace, int n, xmc_trace_event* out, int* counts) {
+    g_uniform24 = precision == 65;
+    if (precision == 32) return shadow_impl<float>(input, seed, photon, trace, n, out, counts);
+    return shadow_impl<double>(input, seed, photon, trace, n, out, counts);
 }
 
 int xmco_bvh(const xmc_input* input, int* nn, int* root, int* nt, double* boxes, int* links,
@@ -1351,6 +1467,7 @@ uint64_t xmco_mcg59_init(uint64_t seed, uint32_t id, uint32_t step, uint64_t* of
 }
 
 double xmco_mcg59_next(uint64_t* value, uint64_t offset) {
+    g_uniform24 = false;
     Mcg59 g{*value, offset, 0};
     double u = Prec<double>::uniform(g.next());
     *value = g.value;
@@ -1464,6 +1581,9 @@ double xmco_triangle_distance(const double* o, const double* d, const double* a,
 
 float xmco_logf(float x) { return f_logf(x); }
 void xmco_sincos2pi(float u, float* s, float* c) { f_sincos2pi(u, s, c); }
+void xmco_hg_float_many(float g, const float* u, float* c, float* s, int64_t n) {
+    for (int64_t i = 0; i < n; i++) hg_float(g, u[i], c + i, s + i);
+}
 void xmco_logf_many(const float* x, float* y, int64_t n) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) y[i] = f_logf(x[i]);

@@ -13,6 +13,8 @@
  *                 the GPU kernel uses (own log / sincos(2*pi*u), 24-bit
  *                 uniforms, coordinate-scaled surface offsets, accurate
  *                 1-x^2).  This mode is bit-comparable with the GPU kernel.
+ * precision = 65: the 64-bit arithmetic driven by the 24-bit uniforms the
+ *                 32-bit mode draws ("the same recorded sequence").
  */
 #ifndef XMC_ORACLE_H
 #define XMC_ORACLE_H
@@ -57,6 +59,15 @@ int xmco_run(const xmc_input* input, xmc_output* output, uint64_t seed, uint64_t
 int xmco_trace(const xmc_input* input, uint64_t seed, uint64_t photon, int precision,
                xmc_trace_event* events, int maxEvents);
 
+/* One loop iteration of ComputePhoton from every recorded state of a trace
+ * (e.g. the GPU trace of photon `photon`): position, direction, weight, time
+ * and layer from trace[i], visited layers accumulated along the trace, random
+ * stream advanced by trace[i].draws.  out[2i], out[2i+1] receive the events of
+ * that iteration (interaction, then termination if any), counts[i] their
+ * number (0 after a terminal event). */
+int xmco_shadow_trace(const xmc_input* input, uint64_t seed, uint64_t photon, int precision,
+                      const xmc_trace_event* trace, int n, xmc_trace_event* out, int* counts);
+
 /* BVH (GenerateBVHTree restated): sizes first (NULL arrays), then contents. */
 int xmco_bvh(const xmc_input* input, int* numberOfNodes, int* root, int* numberOfTriangles,
              double* boxes, int* links, int* triangleSurface, int* triangleIndex);
@@ -92,6 +103,9 @@ double xmco_triangle_distance(const double* o, const double* d, const double* a,
 float xmco_logf(float x);
 void xmco_sincos2pi(float u, float* s, float* c);
 void xmco_logf_many(const float* x, float* y, int64_t n);
+/* Henyey-Greenstein cos/sin of the scattering angle as the 32-bit mode
+ * computes them (1 - cos and 1 + cos without cancellation). */
+void xmco_hg_float_many(float g, const float* u, float* c, float* s, int64_t n);
 void xmco_sincos2pi_many(const float* u, float* s, float* c, int64_t n);
 
 #ifdef __cplusplus

@@ -55,6 +55,7 @@ def oracle():
         _sig(lib, "xmco_run", I, [P, P, U64, U64, U64, I, I, I, C.POINTER(Counters)])
         _sig(lib, "xmco_trace", I, [P, U64, U64, I, P, I])
         _sig(lib, "xmco_bvh", I, [P, PI, PI, PI, PD, PI, PI, PI])
+        _sig(lib, "xmco_shadow_trace", I, [P, U64, U64, I, P, I, P, PI])
         _sig(lib, "xmco_query", I, [P, PD, PD, PD, I, I, I, PI, PD, PI, PI, PD])
         _sig(lib, "xmco_mcg59_init", U64, [U64, C.c_uint32, C.c_uint32, PU64])
         _sig(lib, "xmco_mcg59_next", D, [PU64, U64])
@@ -74,6 +75,7 @@ def oracle():
         _sig(lib, "xmco_logf", C.c_float, [C.c_float])
         _sig(lib, "xmco_sincos2pi", None, [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)])
         _sig(lib, "xmco_logf_many", None, [P, P, C.c_int64])
+        _sig(lib, "xmco_hg_float_many", None, [C.c_float, P, P, P, C.c_int64])
         _sig(lib, "xmco_sincos2pi_many", None, [P, P, P, C.c_int64])
         _oracle = lib
     return _oracle
@@ -145,3 +147,35 @@ def run_oracle(spec, inp, out, seed, first, n, precision=64, nthreads=1, scatter
     rc = oracle().xmco_run(inp.ptr, out.ptr, seed, first, n, precision, nthreads, int(scattering), C.byref(c))
     assert rc == 0, rc
     return c
+
+
+def shadow_compare(inp, seed, photon, trace, precision=65, detail=False):
+    """Replays every event of `trace` (a float walk of `photon`) one
+    interaction at a time in `precision` from the recorded state.  Returns
+    (decisions_equal, worst relative error of position / direction / weight
+    over all steps[, index of the worst step])."""
+    from xmcml_amd.abi import TRACE_DTYPE
+    n = len(trace)
+    tr = np.ascontiguousarray(trace)
+    out = np.zeros(2 * n, dtype=TRACE_DTYPE)
+    cnt = np.zeros(n, np.int32)
+    oracle().xmco_shadow_trace(inp.ptr, seed, photon, precision, tr.ctypes.data, n, out.ctypes.data, iptr(cnt))
+    ok, worst, where = True, 0.0, -1
+    i = 0
+    while i < n - 1:
+        k = int(cnt[i])
+        got = trace[i + 1:i + 1 + k]
+        if k == 0 or len(got) != k:
+            return (False, worst, i) if detail else (False, worst)
+        for a, b in zip(got, out[2 * i:2 * i + k]):
+            if a["type"] != b["type"] or a["layer"] != b["layer"] or a["draws"] != b["draws"]:
+                ok = False
+            scale = max(np.abs(b["pos"]).max(), 1.0)
+            e = max(np.abs(a["pos"] - b["pos"]).max() / scale, np.abs(a["dir"] - b["dir"]).max(),
+                    abs(a["weight"] - b["weight"]) / max(abs(b["weight"]), 1e-30))
+            if e > worst:
+                worst, where = e, i
+        if not ok:
+            break
+        i += k
+    return (ok, worst, where) if detail else (ok, worst)

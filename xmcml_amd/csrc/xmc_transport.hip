@@ -381,26 +381,31 @@ __device__ __forceinline__ V3 refract(float n1, float n2, V3 i, V3 n) {  // :595
     return normalize(t);
 }
 
-// GetNewPhotonDirection + ComputeCosineTheta (:442-495)
+// GetNewPhotonDirection + ComputeCosineTheta (:442-495).  The
+// Henyey-Greenstein inversion is rearranged so that 1 - cos(theta) and
+// 1 + cos(theta) come out without cancellation (D = 1 - g + 2gu,
+// t = (1 - g^2)/D: 1 - cos = (1-g)(1-u)(t+1-g)/D, 1 + cos = (1+g)u(1+g+t)/D;
+// 2(1-u) and 2u at g = 0): with cos computed first, float keeps only a few
+// digits of sin(theta) for the strongly forward scatters of g ~ 0.9.
 __device__ __forceinline__ V3 spin(V3 d, float g, uint64_t& x, uint32_t& draws) {
     float u = next_u(x);
-    float c;
-    if (g != 0.0f) {
-        float tmp = (1.0f - g * g) / (1.0f - g + 2.0f * g * u);
-        c = (1.0f + g * g - tmp * tmp) / (2.0f * g);
-    } else {
-        c = 2.0f * u - 1.0f;
-    }
-    if (c > 1.0f) c = 1.0f;
-    else if (c < -1.0f) c = -1.0f;
-    float st = __builtin_sqrtf(one_minus_sq(c));
+    float a = 1.0f - g, b = 1.0f + g;
+    float inv = 1.0f / (a + 2.0f * g * u);
+    float t = (1.0f - g * g) * inv;
+    float vers = a * (1.0f - u) * (t + a) * inv;
+    float cove = b * u * (b + t) * inv;
+    float c = vers < 1.0f ? 1.0f - vers : cove - 1.0f;
+    float st = __builtin_sqrtf(vers * cove);
     float sp, cp;
     xsincos2pi(next_u(x), sp, cp);
     draws += 2;
     float ux = d.x, uy = d.y, uz = d.z;
     V3 r;
-    if (fabsf(uz) <= kCosZeroF) {
-        float tmp = __builtin_sqrtf(one_minus_sq(uz));
+    // sin(theta_u) from the components (frame stays orthonormal near the
+    // pole); rotate when ux^2 + uy^2 > 2e-12, the reference's |uz| <= 1 - 1e-12
+    float s2 = ux * ux + uy * uy;
+    if (s2 > 2e-12f) {
+        float tmp = __builtin_sqrtf(s2);
         r.x = st * (ux * uz * cp - uy * sp) / tmp + ux * c;
         r.y = st * (uy * uz * cp + ux * sp) / tmp + uy * c;
         r.z = -st * cp * tmp + uz * c;
