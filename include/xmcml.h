@@ -174,6 +174,7 @@ typedef struct xmc_stats {
     uint64_t boundaryEvents; /* Fresnel reflect / refract / escape */
     uint64_t rouletteEvents;
     uint64_t stackSpills;    /* BVH stack pushes that spilled beyond the LDS entries */
+    uint64_t culledWalks;    /* step-limited walks skipped by the distance grid */
     double kernelMs;         /* device time of the transport kernels (HIP events) */
     /* where the launched weight went; with the specular part they sum to the
      * number of photons, up to float rounding */
@@ -216,6 +217,7 @@ typedef struct xmc_options {
     int blockSize;       /* threads per workgroup (64..256, multiple of 64); 0 = 256 */
     int blocksPerCU;     /* cap on persistent workgroups per CU; 0 = occupancy limit */
     int skipAbsorption;  /* 1 = do not tally the absorption grid */
+    int noDistanceGrid;  /* 1 = always walk the BVH (no conservative distance culling) */
 } xmc_options;
 
 typedef struct xmc_scene xmc_scene;
@@ -285,6 +287,12 @@ int xmc_batch_collect(xmc_batch* batch, xmc_output* outputs);
 int xmc_batch_stats(xmc_batch* batch, xmc_stats* stats);
 int xmc_batch_set_launch(xmc_batch* batch, int blockSize, int blocksPerCU, int useGraph);
 int xmc_batch_get_launch(xmc_batch* batch, int* blockSize, int* blocks);
+
+/* The conservative distance grid the scene would build (host only; for
+ * tests): every point of cell (x, y, z) is at least q[(z*dims[1]+y)*dims[0]+x]
+ * * stepQ away from every triangle.  q may be NULL to query the sizes. */
+int xmc_distance_grid(const xmc_input* input, uint64_t targetCells, float* lo, float* h, float* stepQ,
+                      int* dims, uint8_t* q);
 
 /* --- The LaunchOMP replacement. --- */
 /* Builds a scene, runs input->numberOfPhotons photons starting at

@@ -141,6 +141,7 @@ class Stats(C.Structure):
         ("boundaryEvents", C.c_uint64),
         ("rouletteEvents", C.c_uint64),
         ("stackSpills", C.c_uint64),
+        ("culledWalks", C.c_uint64),
         ("kernelMs", C.c_double),
         ("wAbsorbed", C.c_double),
         ("wEscaped", C.c_double),
@@ -187,6 +188,7 @@ class Options(C.Structure):
         ("blockSize", C.c_int),
         ("blocksPerCU", C.c_int),
         ("skipAbsorption", C.c_int),
+        ("noDistanceGrid", C.c_int),
     ]
 
 

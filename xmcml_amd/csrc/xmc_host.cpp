@@ -15,6 +15,7 @@
 
 #include "../../include/xmcml.h"
 #include "xmc_bvh.h"
+#include "xmc_distance.h"
 #include "xmc_params.h"
 
 extern "C" {
@@ -101,6 +102,7 @@ struct xmc_scene {
            off_q = 0, slab_words = 0;
     DevBuf<int2> spill;
     DevBuf<int> perm_dev;
+    DevBuf<uint8_t> dist;
     xmc::Params base;
     std::vector<double> h_t_start, h_t_finish;
 };
@@ -253,6 +255,27 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     for (int a = 0; a < 3; a++) {
         P.root_lo[a] = gb.root_lo[a];
         P.root_hi[a] = gb.root_hi[a];
+    }
+
+    // distance grid that lets step-limited walks that cannot hit be skipped
+    if (nt > 256 && o.noDistanceGrid == 0) {
+        xmc::DistanceGrid dg;
+        double lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            double pad = 1e-3 * (gb.root_hi[a] - gb.root_lo[a]) + 1e-3;
+            lo[a] = gb.root_lo[a] - pad;
+            hi[a] = gb.root_hi[a] + pad;
+        }
+        xmc::build_distance_grid(in->surface, in->numberOfSurfaces, lo, hi, (size_t)16 << 20, dg);
+        HIP_TRY(sc->dist.alloc(dg.q.size()));
+        HIP_TRY(hipMemcpy(sc->dist.p, dg.q.data(), dg.q.size(), hipMemcpyHostToDevice));
+        P.dist.q = sc->dist.p;
+        for (int a = 0; a < 3; a++) {
+            P.dist.lo[a] = dg.lo[a];
+            P.dist.dims[a] = dg.dims[a];
+        }
+        P.dist.inv_h = 1.0f / dg.h;
+        P.dist.step_q = dg.step_q;
     }
 
     // layer boundary table: FindIntersectionLayer (mcml_kernel.cpp:566-580)
@@ -536,6 +559,7 @@ static int read_stats(unsigned long long* dev, hipStream_t stream, xmc_stats* st
     st->boundaryEvents = c[2];
     st->rouletteEvents = c[3];
     st->stackSpills = c[4];
+    st->culledWalks = c[5];
     double led[8];
     std::memcpy(led, c + 8, sizeof(led));
     st->wAbsorbed = led[0];
@@ -703,6 +727,33 @@ int xmc_launch(const xmc_input* in, xmc_output* out, xmc_rng* rng, const xmc_opt
     if (rc == XMC_OK) rng->nextPhoton += in->numberOfPhotons;
     xmc_scene_destroy(sc);
     return rc;
+}
+
+int xmc_distance_grid(const xmc_input* in, uint64_t target_cells, float* lo, float* h, float* step_q, int* dims,
+                      uint8_t* q) {
+    if (!in || !lo || !h || !step_q || !dims) return fail(XMC_ERR_ARG, "NULL argument");
+    xmc::RefBvh ref;
+    const char* err = nullptr;
+    if (!xmc::build_reference_bvh(in->surface, in->numberOfSurfaces, ref, &err))
+        return fail(XMC_ERR_ARG, std::string("BVH: ") + (err ? err : "failed"));
+    xmc::GpuBvh gb;
+    xmc::flatten_bvh(ref, in->surface, gb);
+    double glo[3], ghi[3];
+    for (int a = 0; a < 3; a++) {
+        double pad = 1e-3 * (gb.root_hi[a] - gb.root_lo[a]) + 1e-3;
+        glo[a] = gb.root_lo[a] - pad;
+        ghi[a] = gb.root_hi[a] + pad;
+    }
+    xmc::DistanceGrid dg;
+    xmc::build_distance_grid(in->surface, in->numberOfSurfaces, glo, ghi, target_cells ? target_cells : (16u << 20), dg);
+    for (int a = 0; a < 3; a++) {
+        lo[a] = dg.lo[a];
+        dims[a] = dg.dims[a];
+    }
+    *h = dg.h;
+    *step_q = dg.step_q;
+    if (q) std::memcpy(q, dg.q.data(), dg.q.size());
+    return XMC_OK;
 }
 
 int xmc_init_output(const xmc_input* in, xmc_output* out) {  // InitOutput, launcher_omp.cpp:10-58

@@ -49,7 +49,7 @@ struct Tallies {
     double* bin_weight;      // ndet * T
     double* bin_target;
     unsigned long long* bin_count;
-    unsigned long long* stats;  // 5 event counters, then 8 ledger doubles from word 8
+    unsigned long long* stats;  // 6 event counters, then 8 ledger doubles from word 8
 };
 
 struct Area {
@@ -67,9 +67,19 @@ struct Trace {
     uint64_t n;
 };
 
+// Conservative distance grid (see xmc_distance.h); q == nullptr disables it.
+struct DistGrid {
+    const uint8_t* q;
+    float lo[3];
+    float inv_h;
+    float step_q;
+    int dims[3];
+};
+
 struct Params {
     const GpuNode* nodes;
     const GpuTri* tris;
+    DistGrid dist;
     int root_ref;
     float root_lo[3], root_hi[3];
     const int8_t* next_layer;  // [layer * n_surfaces + surface] -> layer or -1

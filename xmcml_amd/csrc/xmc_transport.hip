@@ -483,6 +483,22 @@ __device__ __forceinline__ int time_bin(const Detectors& D, double t) {
     return -1;
 }
 
+// True when no triangle can lie within the segment of length step * |d|
+// from p: the distance grid's lower bound for p's cell exceeds it by a
+// margin that covers float rounding of the cell lookup and the triangle test.
+__device__ __forceinline__ bool no_hit_possible(const DistGrid& G, V3 p, V3 d, float step) {
+    float fx = (p.x - G.lo[0]) * G.inv_h;
+    float fy = (p.y - G.lo[1]) * G.inv_h;
+    float fz = (p.z - G.lo[2]) * G.inv_h;
+    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f)) return false;
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    if (ix >= G.dims[0] || iy >= G.dims[1] || iz >= G.dims[2]) return false;
+    float L = (float)G.q[((size_t)iz * G.dims[1] + iy) * G.dims[0] + ix] * G.step_q;
+    float margin = 1.0e-4f + 16.0f * offset_scale(p);
+    float room = L - margin;
+    return room > 0.0f && step * step * dot(d, d) < room * room;
+}
+
 template <bool TRACE>
 struct Recorder {
     xmc_trace_event* ev;
@@ -549,7 +565,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     bool Lglass = false, Ltarget = false;
     uint32_t draws = 0;
     Recorder<TRACE> rec{nullptr, 0, 0};
-    unsigned long long n_scatter = 0, n_boundary = 0, n_roulette = 0, n_photons = 0;
+    unsigned long long n_scatter = 0, n_boundary = 0, n_roulette = 0, n_photons = 0, n_culled = 0;
 
     auto load_layer = [&](int l) {
         const Layers& L = CFG().layers;
@@ -639,8 +655,16 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
         const Params& C = CFG();
         Geo geo{C.nodes, C.tris, C.root_ref, {C.root_lo[0], C.root_lo[1], C.root_lo[2]},
                 {C.root_hi[0], C.root_hi[1], C.root_hi[2]}};
-        Ray ray = make_ray(pos, dir);
-        Hit h = walk(geo, ray, step, !Lglass, st);
+        Hit h;
+        if (!Lglass && C.dist.q != nullptr && no_hit_possible(C.dist, pos, dir, step)) {
+            h.found = false;  // the walk could not find anything within the step
+            h.dist = kInf;
+            h.tri = -1;
+            n_culled++;
+        } else {
+            Ray ray = make_ray(pos, dir);
+            h = walk(geo, ray, step, !Lglass, st);
+        }
         if (h.found) {
             step = h.dist - 0.01f * kMinDistF;
             tim += step * Ln;
@@ -792,6 +816,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     if (n_boundary) atomicAdd(P.tally.stats + 2, n_boundary);
     if (n_roulette) atomicAdd(P.tally.stats + 3, n_roulette);
     if (st.spills) atomicAdd(P.tally.stats + 4, (unsigned long long)st.spills);
+    if (n_culled) atomicAdd(P.tally.stats + 5, n_culled);
     atomicAdd(ledger + kAbs, w_abs);
     __syncthreads();
     if (tid < kLedger) unsafeAtomicAdd(reinterpret_cast<double*>(P.tally.stats + 8) + tid, ledger[tid]);

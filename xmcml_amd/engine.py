@@ -30,7 +30,7 @@ EXPORTS = [
     "xmc_scene_create", "xmc_scene_destroy", "xmc_scene_run", "xmc_scene_sync", "xmc_scene_collect", "xmc_scene_stats",
     "xmc_scene_trace", "xmc_scene_query", "xmc_scene_bvh", "xmc_scene_set_launch", "xmc_scene_get_launch",
     "xmc_batch_create", "xmc_batch_destroy", "xmc_batch_run", "xmc_batch_collect", "xmc_batch_stats",
-    "xmc_batch_set_launch", "xmc_batch_get_launch", "xmc_launch", "xmc_init_output",
+    "xmc_batch_set_launch", "xmc_batch_get_launch", "xmc_distance_grid", "xmc_launch", "xmc_init_output",
     "xmc_free_output", "xmc_specular_reflectance", "xmc_last_error", "xmc_version",
 ]
 
@@ -71,6 +71,8 @@ def lib():
     sig("xmc_batch_stats", _I, [_P, _P])
     sig("xmc_batch_set_launch", _I, [_P, _I, _I, _I])
     sig("xmc_batch_get_launch", _I, [_P, _PI, _PI])
+    sig("xmc_distance_grid", _I, [_P, _U64, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                  _PI, _P])
     sig("xmc_launch", _I, [_P, _P, _P, _P])
     sig("xmc_init_output", _I, [_P, _P])
     sig("xmc_free_output", None, [_P])
@@ -99,10 +101,10 @@ class Scene:
     """Geometry, optics and device tallies of one input, resident on the GPU."""
 
     def __init__(self, spec: SceneSpec, block_size: int = 0, blocks_per_cu: int = 0,
-                 absorption: bool = True, device: int = 0):
+                 absorption: bool = True, device: int = 0, distance_grid: bool = True):
         self.spec = spec
         self.input = InputHandle(spec)
-        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1)
+        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1, 0 if distance_grid else 1)
         h = _P()
         _check(lib().xmc_scene_create(self.input.ptr, C.byref(opt), C.byref(h)), "xmc_scene_create")
         self._h = h
@@ -185,12 +187,27 @@ class Scene:
         return dict(root=root.value, boxes=boxes, links=links, tsurf=ts, tidx=ti)
 
 
+def distance_grid(spec: SceneSpec, target_cells: int = 0) -> dict:
+    """The conservative distance grid a scene builds (host computation)."""
+    inp = InputHandle(spec)
+    lo = (C.c_float * 3)()
+    h, q = C.c_float(), C.c_float()
+    dims = (C.c_int * 3)()
+    _check(lib().xmc_distance_grid(inp.ptr, target_cells, lo, C.byref(h), C.byref(q), dims, None),
+           "xmc_distance_grid")
+    grid = np.zeros(dims[0] * dims[1] * dims[2], np.uint8)
+    _check(lib().xmc_distance_grid(inp.ptr, target_cells, lo, C.byref(h), C.byref(q), dims, grid.ctypes.data),
+           "xmc_distance_grid")
+    return dict(lo=np.array(lo[:], np.float32), h=h.value, step_q=q.value, dims=tuple(dims[:]),
+                q=grid.reshape(dims[2], dims[1], dims[0]))
+
+
 def launch(spec: SceneSpec, out: OutputHandle, seed: int, first: int, n: int, absorption: bool = True) -> int:
     """The LaunchOMP replacement: one call, tallies added into ``out``.
     Returns the next photon index of the stream."""
     inp = InputHandle(spec, photons=n)
     rng = Rng(seed, first)
-    opt = Options(0, 0, 0, 0 if absorption else 1)
+    opt = Options(0, 0, 0, 0 if absorption else 1, 0)
     _check(lib().xmc_launch(inp.ptr, out.ptr, C.byref(rng), C.byref(opt)), "xmc_launch")
     return rng.nextPhoton
 
@@ -206,7 +223,7 @@ class Batch:
         self.inputs = [InputHandle(s) for s in self.specs]
         arr = (Input * len(self.inputs))(*[h.c for h in self.inputs])
         self._arr = arr
-        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1)
+        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1, 0)
         h = _P()
         _check(lib().xmc_batch_create(arr, len(self.inputs), C.byref(opt), C.byref(h)), "xmc_batch_create")
         self._h = h
