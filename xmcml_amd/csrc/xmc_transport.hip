@@ -549,7 +549,9 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     if (tid < kLedger) ledger[tid] = 0.0;
     __syncthreads();
     double w_abs = 0.0;
-    const Params* cfg = &P;
+    // never take P's address: that would move the kernel-argument struct
+    // into private memory
+    const Params* cfg = cfgs;
     auto CFG = [&]() -> const Params& {
         if constexpr (BATCHED) return *cfg;
         else return P;
