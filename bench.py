@@ -55,21 +55,49 @@ def cpu_baseline(spec, target_seconds: float, seed: int) -> dict:
     inp = InputHandle(spec)
     n = max(200, 20 * threads)
     out = OutputHandle(spec, absorption=False)
-    c = ol.run_oracle(spec, inp, out, seed, 0, n, 64, threads)
+    c = ol.run_oracle(spec, inp, out, seed, 0, n, 64, threads)  # warm-up, not counted
     rate = c.photons / max(c.seconds, 1e-9)
-    n2 = int(max(n, rate * target_seconds))
+    first, photons, seconds, scatter = n, 0, 0.0, 0
     out = OutputHandle(spec, absorption=False)
-    c = ol.run_oracle(spec, inp, out, seed, n, n2, 64, threads)
+    while seconds < target_seconds:
+        chunk = int(max(n, rate * (target_seconds - seconds)))
+        c = ol.run_oracle(spec, inp, out, seed, first + photons, chunk, 64, threads)
+        photons += c.photons
+        seconds += c.seconds
+        scatter += c.scatterEvents
+        rate = photons / seconds
     return {
-        "value": c.photons / c.seconds,
+        "value": photons / seconds,
         "unit": "photons/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{n2} photons of the same workload (photons {n}..{n + n2 - 1} of seed {seed}), "
-                  f"double precision, OpenMP dynamic schedule, {c.seconds:.1f} s",
-        "scatter_events_per_photon": c.scatterEvents / c.photons,
-        "reflectance": float(out.det_weight().sum() / c.photons),
+        "sample": f"{photons} photons of the same workload (photons {first}..{first + photons - 1} of seed "
+                  f"{seed}), double precision, OpenMP dynamic schedule, {seconds:.1f} s",
+        "scatter_events_per_photon": scatter / photons,
+        "reflectance": float(out.det_weight().sum() / photons),
     }
+
+
+def copy_calibration(nbytes: int) -> None:
+    """One device-to-device hipMemcpy of `nbytes` (the runtime's blit copy
+    kernel): it reads and writes exactly nbytes, which scales the HBM counters
+    of a profiler pass over the same process."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for name, ptr in (("src", src), ("dst", dst)):
+        if hip.hipMalloc(C.byref(ptr), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError(f"hipMalloc of the calibration {name} buffer failed")
+    try:
+        if hip.hipMemset(src, 1, C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
 
 
 def traffic_from_profiles():
@@ -93,6 +121,9 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--seed", type=int, default=777)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="before the steps, run one device-to-device copy of this many MiB (a known-size "
+                         "kernel that scales the HBM counters of a profiler pass)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -105,6 +136,9 @@ def main(argv=None) -> int:
     spec = scenes.heightfield_scene(args.photons)
     print(f"bench: building scene ({spec.num_triangles} triangles)", file=sys.stderr, flush=True)
     sc = engine.Scene(spec, absorption=False, block_size=args.block)
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib << 20)
+        print(f"bench: calibration copy of {args.copy_calibration_mib} MiB done", file=sys.stderr, flush=True)
     first = 0
     for _ in range(args.warmup):
         sc.run(args.seed, first, args.photons)

@@ -1,0 +1,181 @@
+"""The surrogate's matrix-core kernels against the CPU reference network
+(tests/mlp_ref.py, itself pinned by finite differences), and the trained
+surrogate against plain Monte Carlo.
+
+Tolerances: the kernels accumulate bf16 products in fp32 where the reference
+accumulates in float64, so a value rounded to bf16 afterwards can land one
+bf16 step (2^-8 relative) away on rare elements.  Element-wise checks are
+therefore made on fp32 results before any rounding (GEMM, head output), and
+gradients are compared by relative L2 norm per tensor (<= 5e-3) plus their
+cosine (>= 0.9999)."""
+import math
+import time
+
+import numpy as np
+import pytest
+
+import mlp_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def test_gemm_matches_reference(torch_cuda):
+    from xmcml_amd.surrogate import gemm_check
+    rng = np.random.default_rng(0)
+    for (M, N, K) in [(64, 64, 32), (128, 192, 96), (256, 64, 512)]:
+        a = rng.standard_normal((M, K)).astype(np.float32)
+        b = rng.standard_normal((N, K)).astype(np.float32)
+        c = gemm_check(a, b)
+        ref = R.bf16(a).astype(np.float64) @ R.bf16(b).astype(np.float64).T
+        bound = (np.abs(R.bf16(a)).astype(np.float64) @ np.abs(R.bf16(b)).astype(np.float64).T) * K * 2.0 ** -24
+        assert np.all(np.abs(c - ref) <= bound + 1e-30), (M, N, K, np.max(np.abs(c - ref) / bound))
+        # exact on integer-valued operands (every partial sum is representable)
+        ai = rng.integers(-8, 9, (M, K)).astype(np.float32)
+        bi = rng.integers(-8, 9, (N, K)).astype(np.float32)
+        assert np.array_equal(gemm_check(ai, bi), (ai.astype(np.float64) @ bi.T.astype(np.float64)).astype(np.float32))
+
+
+def _model(torch, hidden=128, depth=2, outputs=32, max_batch=256, seed=3):
+    from xmcml_amd.surrogate import MLP
+    m = MLP(30, hidden, depth, outputs, max_batch, seed=seed)
+    rng = np.random.default_rng(seed)
+    p = m.get_params()
+    # nonzero biases so their gradients and the bias epilogue are exercised
+    shapes = R.layer_shapes(30, hidden, depth, outputs)
+    layers = R.unpack(p.copy(), shapes)
+    layers = [(W, 0.1 * rng.standard_normal(W.shape[0]).astype(np.float32)) for W, _ in layers]
+    p = R.pack(layers).astype(np.float32)
+    m.set_params(p)
+    return m, p, shapes
+
+
+def _batch(torch, n, outputs, seed=4):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (n, 30)).astype(np.float32)
+    y = rng.standard_normal((n, outputs)).astype(np.float32)
+    mask = (rng.random((n, outputs)) > 0.2).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    return x, y, mask, [torch.from_numpy(a).to(dev) for a in (x, y, mask)]
+
+
+def test_forward_matches_reference(torch_cuda):
+    torch = torch_cuda
+    m, p, shapes = _model(torch)
+    x, y, mask, (xt, _, _) = _batch(torch, 200, 32)  # 200: not a multiple of the 64-row tile
+    out = m.forward(xt).cpu().numpy()
+    ref, _ = R.RefMLP(p, 30, 128, 2, 32).forward(x)
+    err = np.abs(out - ref)
+    assert np.linalg.norm(out - ref) <= 5e-3 * np.linalg.norm(ref), np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    assert err.max() <= 0.05 * np.abs(ref).max()
+    # more rows than maxBatch: processed in chunks, same answer per row
+    big = np.concatenate([x, x, x[:100]])
+    out2 = m.forward(torch.from_numpy(big).to(xt.device)).cpu().numpy()
+    assert np.array_equal(out2[:200], out) and np.array_equal(out2[200:400], out)
+    assert np.array_equal(out2[400:], out[:100])
+
+
+def _cmp(g, r):
+    g = np.asarray(g, np.float64).ravel()
+    r = np.asarray(r, np.float64).ravel()
+    rel = np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-30)
+    cos = float(g @ r) / max(np.linalg.norm(g) * np.linalg.norm(r), 1e-30)
+    return rel, cos
+
+
+def test_backward_matches_reference(torch_cuda):
+    torch = torch_cuda
+    m, p, shapes = _model(torch)
+    n = 256
+    x, y, mask, (xt, yt, mt) = _batch(torch, n, 32)
+    scale = 1.0 / mask.sum()
+    loss = m.backward(xt, yt, mt, scale)
+    grads = m.get_grads()
+    rloss, _, rgrads = R.RefMLP(p, 30, 128, 2, 32).loss_and_grads(x, y, mask, scale)
+    assert loss == pytest.approx(rloss, rel=2e-3)
+    g_layers = R.unpack(grads, shapes)
+    for l, ((gW, gb), (rW, rb)) in enumerate(zip(g_layers, rgrads)):
+        for name, a, b in (("W", gW, rW), ("b", gb, rb)):
+            rel, cos = _cmp(a, b)
+            assert rel <= 5e-3 and cos >= 0.9999, (l, name, rel, cos)
+    # the padded input columns carry no gradient into the caller's view
+    assert np.isfinite(grads).all()
+
+
+def test_adam_matches_reference(torch_cuda):
+    torch = torch_cuda
+    m, p, shapes = _model(torch)
+    x, y, mask, (xt, yt, mt) = _batch(torch, 128, 32)
+    flags = R.weight_flags(shapes)
+    m1 = np.zeros_like(p)
+    m2 = np.zeros_like(p)
+    cur = p.copy()
+    for step in (1, 2, 3):
+        m.backward(xt, yt, mt, 1.0 / mask.sum(), want_loss=False)
+        g = m.get_grads()
+        m.adam(1e-3, 0.9, 0.999, 1e-8, 0.01)
+        cur, m1, m2 = R.adamw(cur, g, m1, m2, step, 1e-3, 0.9, 0.999, 1e-8, 0.01, flags)
+        got = m.get_params()
+        assert np.allclose(got, cur, rtol=1e-5, atol=1e-7), (step, np.max(np.abs(got - cur)))
+
+
+def test_training_fits_a_smooth_function(torch_cuda):
+    torch = torch_cuda
+    from xmcml_amd.surrogate import Surrogate
+    rng = np.random.default_rng(7)
+    n = 8192
+    X = rng.uniform(-1, 1, (n, 30)).astype(np.float32)
+    A = rng.standard_normal((30, 96)).astype(np.float32) / math.sqrt(30)
+    Y = (np.sin(X @ A) + 0.01 * rng.standard_normal((n, 96))).astype(np.float32)
+    M = np.ones_like(Y)
+    sur, log = Surrogate.fit(X[:7168], Y[:7168], M[:7168], steps=600, batch=1024, lr=2e-3, hidden=256, depth=3)
+    mu, sig = sur.predict(X[7168:])
+    rmse = float(np.sqrt(np.mean((mu - Y[7168:]) ** 2)))
+    base = float(np.sqrt(np.mean((Y[7168:] - Y[:7168].mean(0)) ** 2)))
+    print(f"fit {log.seconds:.2f} s, loss {log.first_loss:.3f} -> {log.last_loss:.3f}, rmse {rmse:.4f} vs {base:.4f}")
+    assert log.last_loss < log.first_loss - 1.0
+    assert rmse < 0.15 * base
+
+
+def test_surrogate_beats_plain_mc_at_equal_cost(torch_cuda):
+    """Trained on 2048 batched-MC configurations, the surrogate answers
+    held-out configurations with a smaller ring-reflectance error than plain
+    Monte Carlo given 1000x the surrogate's time per configuration."""
+    torch = torch_cuda
+    from xmcml_amd import surrogate as S
+    train_cfg = S.draw_configs(2048, 11)
+    test_cfg = S.draw_configs(64, 12)
+    t0 = time.perf_counter()
+    Y, M, _, _ = S.simulate(train_cfg, 20000, seed=100)
+    t_train_mc = time.perf_counter() - t0
+    _, Mref, Kref, Wref = S.simulate(test_cfg, 400000, seed=200)
+    sur, log = S.Surrogate.fit(S.encode_many(train_cfg), Y, M, steps=1500, batch=1024, lr=2e-3)
+    Xte = S.encode_many(test_cfg)
+    xt = torch.from_numpy(Xte).cuda()
+    sur.predict_device(xt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        sur.predict_device(xt)
+    torch.cuda.synchronize()
+    t_per_cfg = (time.perf_counter() - t0) / (20 * len(test_cfg))
+    mu, _ = sur.predict(Xte)
+    valid = (Kref[:, :S.RINGS] >= 400)
+    err_s = np.median(S.ring_relative_error(np.exp(mu[:, :S.RINGS]), Wref[:, :S.RINGS], valid))
+    # plain MC at 1000x the surrogate's time per configuration
+    rate = len(train_cfg) * 20000 / t_train_mc  # photons/s of the batched MC, launch overheads included
+    photons = max(1, int(1000 * t_per_cfg * rate))
+    _, _, _, Wmc = S.simulate(test_cfg, photons, seed=300)
+    err_mc = np.median(S.ring_relative_error(Wmc[:, :S.RINGS], Wref[:, :S.RINGS], valid))
+    print(f"surrogate median ring error {err_s:.4f} ({t_per_cfg * 1e6:.3f} us/config); plain MC with "
+          f"{photons} photons/config (1000x the time): {err_mc:.4f}; fit {log.seconds:.1f} s, MC data "
+          f"{t_train_mc:.1f} s")
+    assert err_s < 0.25
+    assert err_s < err_mc

@@ -106,6 +106,11 @@ class TimeInfo(C.Structure):
     ]
 
 
+TIME_INFO_DTYPE = np.dtype([("timeStart", "<f8"), ("timeFinish", "<f8"), ("numberOfPhotons", "<u8"),
+                            ("weight", "<f8"), ("targetWeight", "<f8")])
+assert TIME_INFO_DTYPE.itemsize == C.sizeof(TimeInfo)
+
+
 class DetectorInfo(C.Structure):
     _fields_ = [
         ("numberOfPhotons", C.c_uint64),
@@ -386,8 +391,8 @@ class OutputHandle:
         return np.array([self.dets[d].numberOfPhotons for d in range(self.nd)], dtype=np.uint64)
 
     def _bins(self, name, dtype=np.float64) -> np.ndarray:
-        return np.array([getattr(self.bins[i], name) for i in range(self.nd * self.T)], dtype=dtype).reshape(
-            self.nd, self.T)
+        rec = np.frombuffer(self.bins, dtype=TIME_INFO_DTYPE, count=self.nd * self.T)
+        return rec[name].astype(dtype).reshape(self.nd, self.T)
 
     def bin_weight(self) -> np.ndarray:
         return self._bins("weight")

@@ -37,6 +37,16 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+namespace xmc {
+// Shared with the other translation units of the library so that every
+// entry point reports through xmc_last_error().
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace xmc
+
+namespace {
+
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
         hipError_t _e = (expr);                                                                \

@@ -1,0 +1,700 @@
+// Surrogate MLP on the CDNA4 matrix cores: forward, masked Gaussian NLL,
+// backward and AdamW, bf16 operands with fp32 accumulation.
+//
+// Matrix products use one LDS-tiled kernel: a 256-thread block computes a
+// 64 x 64 tile of C = A * B^T, each 64-wide wave a 32 x 32 quarter with
+// v_mfma_f32_32x32x16_bf16.  Operands are staged through LDS as [row][k]
+// tiles of 32 k; an operand whose reduction index is not the contiguous one
+// (the transposed products of the backward pass) is loaded with 16-byte
+// coalesced reads along its contiguous index and transposed on the LDS store,
+// so no transposed copies of weights or activations are ever written.
+// Epilogues are fused: bias + GELU (keeping the pre-activation), bias only
+// (the head, fp32 out), GELU' times the incoming gradient, or a raw fp32
+// split-K partial of a weight gradient.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "xmcml_surrogate.h"
+
+namespace xmc {
+int set_error(int code, const std::string& msg);
+}
+
+namespace xmlp {
+
+typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
+typedef float f32x16 __attribute__((__vector_size__(16 * sizeof(float))));
+typedef uint32_t u32x4 __attribute__((__vector_size__(16)));
+
+constexpr int BM = 64, BN = 64, BK = 32;
+constexpr int LDS_STRIDE = BK + 8;  // halfwords per LDS tile row (80 bytes: 16-byte aligned rows)
+constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2/pi)
+constexpr float kGeluA = 0.044715f;
+constexpr float kLogVarClamp = 10.0f;
+
+__host__ __device__ inline uint16_t f2bf(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest, ties to even (finite inputs)
+    return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float bf2f(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ inline float tanh_f(float y) { return 1.0f - 2.0f / (1.0f + expf(2.0f * y)); }
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + tanh_f(kGeluC * (z + kGeluA * z * z * z))); }
+
+__device__ inline float gelu_grad(float z) {
+    float t = tanh_f(kGeluC * (z + kGeluA * z * z * z));
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * z * z);
+}
+
+enum Epi { EPI_GELU = 0, EPI_LINEAR = 1, EPI_DGELU = 2, EPI_PARTIAL = 3 };
+
+// C[m][n] = sum_k A(m,k) B(k,n) over k in this block's split-K chunk.
+// A(m,k) = A[m*lda + k] if AK else A[k*lda + m];
+// B(k,n) = B[n*ldb + k] if BKC else B[k*ldb + n].
+struct GemmArgs {
+    const uint16_t* A;
+    long lda;
+    const uint16_t* B;
+    long ldb;
+    int M, N, kchunk;
+    const float* bias;   // EPI_GELU / EPI_LINEAR: [N]
+    const uint16_t* zin; // EPI_DGELU: pre-activation Z[m*ldo + n] of the layer the gradient enters
+    uint16_t* out;       // EPI_GELU: activation; EPI_DGELU: gradient; [m*ldo + n]
+    uint16_t* zout;      // EPI_GELU: pre-activation [m*ldo + n]
+    long ldo;
+    float* outf;         // EPI_LINEAR: [m*ldo + n]; EPI_PARTIAL: outf + z*M*ldo + m*ldo + n
+};
+
+template <bool AK>
+__device__ inline void load_tile(uint16_t* lds, const uint16_t* src, long ld, int r0, int k0, int t) {
+    if constexpr (AK) {
+        // 64 rows x 32 k, contiguous in k: one 16-byte read per thread
+        int r = t >> 2, c = (t & 3) * 8;
+        u32x4 v = *(const u32x4*)(src + (long)(r0 + r) * ld + k0 + c);
+        *(u32x4*)(lds + r * LDS_STRIDE + c) = v;
+    } else {
+        // 32 k x 64 rows, contiguous in the row index: read along rows, transpose into LDS
+        int kr = t >> 3, c = (t & 7) * 8;
+        u32x4 v = *(const u32x4*)(src + (long)(k0 + kr) * ld + r0 + c);
+        uint16_t h[8];
+        std::memcpy(h, &v, 16);
+#pragma unroll
+        for (int i = 0; i < 8; i++) lds[(c + i) * LDS_STRIDE + kr] = h[i];
+    }
+}
+
+template <bool AK, bool BKC, int EPI>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDS_STRIDE];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDS_STRIDE];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int kb = blockIdx.z * g.kchunk, ke = kb + g.kchunk;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    const int lr = lane & 31, lk = (lane >> 5) * 8;
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc[v] = 0.0f;
+
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+        load_tile<AK>(As, g.A, g.lda, m0, k0, t);
+        load_tile<BKC>(Bs, g.B, g.ldb, n0, k0, t);
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            // lane holds row (lane % 32), k = kk + 8*(lane / 32) .. +7 of each operand
+            u32x4 av = *(const u32x4*)(As + (wm + lr) * LDS_STRIDE + kk + lk);
+            u32x4 bv = *(const u32x4*)(Bs + (wn + lr) * LDS_STRIDE + kk + lk);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av),
+                                                          __builtin_bit_cast(bf16x8, bv), acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    // accumulator element v of a lane: row 8*(v/4) + 4*(lane/32) + v%4, column lane%32
+    const int n = n0 + wn + lr;
+    float bias = 0.0f;
+    if constexpr (EPI == EPI_GELU || EPI == EPI_LINEAR) bias = g.bias[n];
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int m = m0 + wm + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const long o = (long)m * g.ldo + n;
+        float x = acc[v];
+        if constexpr (EPI == EPI_GELU) {
+            float z = x + bias;
+            g.zout[o] = f2bf(z);
+            g.out[o] = f2bf(gelu(z));
+        } else if constexpr (EPI == EPI_LINEAR) {
+            g.outf[o] = x + bias;
+        } else if constexpr (EPI == EPI_DGELU) {
+            g.out[o] = f2bf(x * gelu_grad(bf2f(g.zin[o])));
+        } else {
+            g.outf[(long)blockIdx.z * g.M * g.ldo + o] = x;
+        }
+    }
+}
+
+// x[b][0..nin) fp32 -> bf16 rows of width ldx, zero padding beyond nin and batch
+__global__ void pack_input_kernel(const float* x, int batch, int nin, uint16_t* a0, int ldx, int rows) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)rows * ldx) return;
+    int r = (int)(i / ldx), c = (int)(i % ldx);
+    a0[i] = (r < batch && c < nin) ? f2bf(x[(long)r * nin + c]) : (uint16_t)0;
+}
+
+// masked Gaussian NLL of the head output; writes the bf16 gradient of the
+// head (zero for padded rows) and one partial loss sum per block
+__global__ void loss_kernel(const float* out, const float* y, const float* mask, int batch, int nout, int rows,
+                            float scale, uint16_t* dout, float* partial) {
+    __shared__ float red[256];
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (i < (long)rows * nout) {
+        int r = (int)(i / nout), j = (int)(i % nout);
+        long oi = (long)r * 2 * nout;
+        float dmu = 0.0f, ds = 0.0f;
+        if (r < batch) {
+            float mk = mask[i];
+            float mu = out[oi + j], s = out[oi + nout + j];
+            float sc = fminf(fmaxf(s, -kLogVarClamp), kLogVarClamp);
+            float e = expf(-sc), d = y[i] - mu;
+            float ed2 = e * d * d;
+            l = mk * 0.5f * (ed2 + sc);
+            dmu = -scale * mk * e * d;
+            ds = (s >= -kLogVarClamp && s <= kLogVarClamp) ? scale * mk * 0.5f * (1.0f - ed2) : 0.0f;
+        }
+        dout[oi + j] = f2bf(dmu);
+        dout[oi + nout + j] = f2bf(ds);
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// dst[n] = sum over rows of src[r][n] (bf16 in, fp32 out), fixed order
+__global__ void colsum_kernel(const uint16_t* src, int rows, int cols, float* dst) {
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= cols) return;
+    float s = 0.0f;
+    for (int r = 0; r < rows; r++) s += bf2f(src[(long)r * cols + n]);
+    dst[n] = s;
+}
+
+// g[i] = sum_z p[z*count + i], fixed order
+__global__ void reduce_partials_kernel(const float* p, int splits, long count, float* g) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float s = 0.0f;
+    for (int z = 0; z < splits; z++) s += p[z * count + i];
+    g[i] = s;
+}
+
+__global__ void adam_kernel(float* w, const float* g, float* m1, float* m2, uint16_t* mirror, long count,
+                            float lr, float b1, float b2, float eps, float c1, float c2, float decay) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float gi = g[i];
+    float a = b1 * m1[i] + (1.0f - b1) * gi;
+    float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    float x = w[i];
+    x -= lr * (a / c1) / (sqrtf(b / c2) + eps) + lr * decay * x;
+    w[i] = x;
+    if (mirror) mirror[i] = f2bf(x);
+}
+
+__global__ void to_bf16_kernel(const float* src, uint16_t* dst, long count) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) dst[i] = f2bf(src[i]);
+}
+
+}  // namespace xmlp
+
+using namespace xmlp;
+
+namespace {
+
+int fail(int code, const std::string& msg) { return xmc::set_error(code, msg); }
+
+#define MLP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail(-2, std::string(#expr) + ": " + hipGetErrorString(_e));                \
+    } while (0)
+
+inline int round_up(int x, int q) { return (x + q - 1) / q * q; }
+
+template <bool AK, bool BKC, int EPI>
+hipError_t launch_gemm(const GemmArgs& g, int splits, hipStream_t s) {
+    dim3 grid(g.N / BN, g.M / BM, splits);
+    hipLaunchKernelGGL((gemm_kernel<AK, BKC, EPI>), grid, dim3(256), 0, s, g);
+    return hipGetLastError();
+}
+
+int split_count(int k) {
+    for (int s : {8, 4, 2})
+        if (k % (s * BK) == 0 && k / s >= 256) return s;
+    return 1;
+}
+
+}  // namespace
+
+struct xmc_mlp {
+    xmc_mlp_desc d{};
+    int in_pad = 0, head = 0, rows = 0;  // padded input width, head width, padded batch rows
+    int layers = 0;
+    std::vector<int> fin, fout;          // padded fan-in / fan-out per layer
+    std::vector<long> woff, boff;        // offsets in the flat padded parameter array
+    long nparam = 0;
+    float* w = nullptr;   // fp32 master parameters, padded layout
+    float* g = nullptr;   // gradients
+    float* m1 = nullptr;
+    float* m2 = nullptr;
+    uint16_t* wb = nullptr;  // bf16 mirror (same offsets as w; only weight blocks used)
+    uint16_t* a0 = nullptr;  // packed input [rows][in_pad]
+    std::vector<uint16_t*> z, a;  // per hidden layer [rows][hidden]
+    float* out = nullptr;         // [rows][head]
+    uint16_t* dout = nullptr;     // [rows][head]
+    uint16_t* dz[2] = {nullptr, nullptr};
+    float* part = nullptr;        // split-K partials
+    long part_count = 0;
+    float* lossp = nullptr;       // loss partial sums
+    int loss_blocks = 0;
+    int step = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+hipError_t dalloc(xmc_mlp* m, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipSuccess) m->allocs.push_back(*p);
+    return e;
+}
+
+// calls f(index in caller order, index in the padded device layout) for every parameter
+void for_each_param(const xmc_mlp* m, const std::function<void(long, long)>& f) {
+    long u = 0;
+    for (int l = 0; l < m->layers; l++) {
+        int in_user = l == 0 ? m->d.inputs : m->d.hidden;
+        int out_user = l == m->layers - 1 ? m->head : m->d.hidden;
+        for (int r = 0; r < out_user; r++)
+            for (int c = 0; c < in_user; c++) f(u++, m->woff[l] + (long)r * m->fin[l] + c);
+        for (int r = 0; r < out_user; r++) f(u++, m->boff[l] + r);
+    }
+}
+
+long user_count(const xmc_mlp* m) {
+    long n = 0;
+    for (int l = 0; l < m->layers; l++) {
+        int in_user = l == 0 ? m->d.inputs : m->d.hidden;
+        int out_user = l == m->layers - 1 ? m->head : m->d.hidden;
+        n += (long)out_user * in_user + out_user;
+    }
+    return n;
+}
+
+int refresh_mirror(xmc_mlp* m, hipStream_t s) {
+    long n = m->nparam;
+    hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->w, m->wb, n);
+    MLP_TRY(hipGetLastError());
+    return 0;
+}
+
+// forward of `batch` rows (<= rows) already packed into a0
+int forward_packed(xmc_mlp* m, int rows, hipStream_t s) {
+    const uint16_t* prev = m->a0;
+    long ldp = m->in_pad;
+    for (int l = 0; l < m->layers; l++) {
+        GemmArgs g{};
+        g.A = prev;
+        g.lda = ldp;
+        g.B = m->wb + m->woff[l];
+        g.ldb = m->fin[l];
+        g.M = rows;
+        g.N = m->fout[l];
+        g.kchunk = m->fin[l];
+        g.bias = m->w + m->boff[l];
+        if (l < m->layers - 1) {
+            g.out = m->a[l];
+            g.zout = m->z[l];
+            g.ldo = m->d.hidden;
+            MLP_TRY((launch_gemm<true, true, EPI_GELU>(g, 1, s)));
+            prev = m->a[l];
+            ldp = m->d.hidden;
+        } else {
+            g.outf = m->out;
+            g.ldo = m->head;
+            MLP_TRY((launch_gemm<true, true, EPI_LINEAR>(g, 1, s)));
+        }
+    }
+    return 0;
+}
+
+int pack(xmc_mlp* m, const float* x, int batch, int rows, hipStream_t s) {
+    long n = (long)rows * m->in_pad;
+    hipLaunchKernelGGL(pack_input_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, batch,
+                       m->d.inputs, m->a0, m->in_pad, rows);
+    MLP_TRY(hipGetLastError());
+    return 0;
+}
+
+// dW_l (+ db_l) of layer l from the gradient `dz` [rows][fout[l]] and the
+// layer's input `in` [rows][fin[l]]
+int weight_grad(xmc_mlp* m, int l, const uint16_t* dz, const uint16_t* in, int rows, hipStream_t s) {
+    int splits = split_count(rows);
+    GemmArgs g{};
+    g.A = dz;
+    g.lda = m->fout[l];
+    g.B = in;
+    g.ldb = m->fin[l];
+    g.M = m->fout[l];
+    g.N = m->fin[l];
+    g.kchunk = rows / splits;
+    g.outf = m->part;
+    g.ldo = m->fin[l];
+    MLP_TRY((launch_gemm<false, false, EPI_PARTIAL>(g, splits, s)));
+    long cnt = (long)m->fout[l] * m->fin[l];
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, m->part,
+                       splits, cnt, m->g + m->woff[l]);
+    MLP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(colsum_kernel, dim3((m->fout[l] + 255) / 256), dim3(256), 0, s, dz, rows, m->fout[l],
+                       m->g + m->boff[l]);
+    MLP_TRY(hipGetLastError());
+    return 0;
+}
+
+int backward_impl(xmc_mlp* m, const float* x, const float* y, const float* mask, int batch, float scale,
+                  float* loss, hipStream_t s) {
+    if (batch <= 0 || batch > m->d.maxBatch) return fail(-1, "batch outside 1..maxBatch");
+    if (!x || !y || !mask) return fail(-1, "NULL device array");
+    int rows = round_up(batch, BM);
+    int rc = pack(m, x, batch, rows, s);
+    if (rc) return rc;
+    if ((rc = forward_packed(m, rows, s))) return rc;
+    long n = (long)rows * m->d.outputs;
+    int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(loss_kernel, dim3(blocks), dim3(256), 0, s, m->out, y, mask, batch, m->d.outputs, rows,
+                       scale, m->dout, m->lossp);
+    MLP_TRY(hipGetLastError());
+
+    const int L = m->layers;
+    // head
+    const uint16_t* head_in = m->a[L - 2];
+    if ((rc = weight_grad(m, L - 1, m->dout, head_in, rows, s))) return rc;
+    int cur = 0;
+    {
+        GemmArgs g{};
+        g.A = m->dout;
+        g.lda = m->head;
+        g.B = m->wb + m->woff[L - 1];
+        g.ldb = m->fin[L - 1];
+        g.M = rows;
+        g.N = m->fin[L - 1];
+        g.kchunk = m->head;
+        g.zin = m->z[L - 2];
+        g.out = m->dz[cur];
+        g.ldo = m->d.hidden;
+        MLP_TRY((launch_gemm<true, false, EPI_DGELU>(g, 1, s)));
+    }
+    for (int l = L - 2; l >= 0; l--) {
+        const uint16_t* in = l == 0 ? m->a0 : m->a[l - 1];
+        if ((rc = weight_grad(m, l, m->dz[cur], in, rows, s))) return rc;
+        if (l > 0) {
+            GemmArgs g{};
+            g.A = m->dz[cur];
+            g.lda = m->d.hidden;
+            g.B = m->wb + m->woff[l];
+            g.ldb = m->fin[l];
+            g.M = rows;
+            g.N = m->fin[l];
+            g.kchunk = m->d.hidden;
+            g.zin = m->z[l - 1];
+            g.out = m->dz[cur ^ 1];
+            g.ldo = m->d.hidden;
+            MLP_TRY((launch_gemm<true, false, EPI_DGELU>(g, 1, s)));
+            cur ^= 1;
+        }
+    }
+    if (loss) {
+        std::vector<float> h(blocks);
+        MLP_TRY(hipMemcpyAsync(h.data(), m->lossp, blocks * sizeof(float), hipMemcpyDeviceToHost, s));
+        MLP_TRY(hipStreamSynchronize(s));
+        double acc = 0.0;
+        for (float v : h) acc += v;
+        *loss = (float)(acc * scale);
+    }
+    return 0;
+}
+
+int adam_impl(xmc_mlp* m, float lr, float b1, float b2, float eps, float decay, hipStream_t s) {
+    m->step++;
+    float c1 = 1.0f - std::pow(b1, (float)m->step);
+    float c2 = 1.0f - std::pow(b2, (float)m->step);
+    for (int l = 0; l < m->layers; l++) {
+        long nw = (long)m->fout[l] * m->fin[l];
+        hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, m->w + m->woff[l],
+                           m->g + m->woff[l], m->m1 + m->woff[l], m->m2 + m->woff[l], m->wb + m->woff[l], nw, lr,
+                           b1, b2, eps, c1, c2, decay);
+        MLP_TRY(hipGetLastError());
+        long nb = m->fout[l];
+        hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, m->w + m->boff[l],
+                           m->g + m->boff[l], m->m1 + m->boff[l], m->m2 + m->boff[l], (uint16_t*)nullptr, nb, lr,
+                           b1, b2, eps, c1, c2, 0.0f);
+        MLP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+// SplitMix64, for the initial weights
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+extern "C" {
+
+int xmc_mlp_destroy(xmc_mlp* m) {
+    if (!m) return 0;
+    (void)hipSetDevice(m->d.device);
+    (void)hipDeviceSynchronize();
+    for (void* p : m->allocs) (void)hipFree(p);
+    delete m;
+    return 0;
+}
+
+int xmc_mlp_create(const xmc_mlp_desc* desc, uint64_t seed, xmc_mlp** out) {
+    if (!desc || !out) return fail(-1, "NULL argument");
+    *out = nullptr;
+    const xmc_mlp_desc& d = *desc;
+    if (d.inputs < 1 || d.inputs > 64) return fail(-1, "inputs must be in 1..64");
+    if (d.hidden < 64 || d.hidden % 64) return fail(-1, "hidden must be a positive multiple of 64");
+    if (d.depth < 1 || d.depth > 8) return fail(-1, "depth must be in 1..8");
+    if (d.outputs < 32 || d.outputs % 32) return fail(-1, "outputs must be a positive multiple of 32");
+    if (d.maxBatch < 1 || d.maxBatch > (1 << 20)) return fail(-1, "maxBatch must be in 1..2^20");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(-4, "no GPU");
+    if (d.device < 0 || d.device >= ndev) return fail(-1, "device index out of range");
+    MLP_TRY(hipSetDevice(d.device));
+
+    auto* m = new xmc_mlp();
+    std::unique_ptr<xmc_mlp, int (*)(xmc_mlp*)> guard(m, xmc_mlp_destroy);
+    m->d = d;
+    m->in_pad = 64;
+    m->head = 2 * d.outputs;
+    m->rows = round_up(d.maxBatch, BM);
+    m->layers = d.depth + 1;
+    long off = 0;
+    for (int l = 0; l < m->layers; l++) {
+        int fi = l == 0 ? m->in_pad : d.hidden;
+        int fo = l == m->layers - 1 ? m->head : d.hidden;
+        m->fin.push_back(fi);
+        m->fout.push_back(fo);
+        m->woff.push_back(off);
+        off += (long)fi * fo;
+        m->boff.push_back(off);
+        off += fo;
+        off = (off + 63) / 64 * 64;  // keep every block 256-byte aligned
+    }
+    m->nparam = off;
+    const size_t pb = off * sizeof(float);
+    MLP_TRY(dalloc(m, (void**)&m->w, pb));
+    MLP_TRY(dalloc(m, (void**)&m->g, pb));
+    MLP_TRY(dalloc(m, (void**)&m->m1, pb));
+    MLP_TRY(dalloc(m, (void**)&m->m2, pb));
+    MLP_TRY(dalloc(m, (void**)&m->wb, off * sizeof(uint16_t)));
+    MLP_TRY(dalloc(m, (void**)&m->a0, (size_t)m->rows * m->in_pad * 2));
+    for (int l = 0; l < d.depth; l++) {
+        uint16_t *zp, *ap;
+        MLP_TRY(dalloc(m, (void**)&zp, (size_t)m->rows * d.hidden * 2));
+        MLP_TRY(dalloc(m, (void**)&ap, (size_t)m->rows * d.hidden * 2));
+        m->z.push_back(zp);
+        m->a.push_back(ap);
+    }
+    MLP_TRY(dalloc(m, (void**)&m->out, (size_t)m->rows * m->head * 4));
+    MLP_TRY(dalloc(m, (void**)&m->dout, (size_t)m->rows * m->head * 2));
+    MLP_TRY(dalloc(m, (void**)&m->dz[0], (size_t)m->rows * d.hidden * 2));
+    MLP_TRY(dalloc(m, (void**)&m->dz[1], (size_t)m->rows * d.hidden * 2));
+    long maxw = 0;
+    for (int l = 0; l < m->layers; l++) maxw = std::max(maxw, (long)m->fin[l] * m->fout[l]);
+    m->part_count = 8 * maxw;
+    MLP_TRY(dalloc(m, (void**)&m->part, m->part_count * 4));
+    m->loss_blocks = (int)(((long)m->rows * d.outputs + 255) / 256);
+    MLP_TRY(dalloc(m, (void**)&m->lossp, (size_t)m->loss_blocks * 4));
+
+    // Glorot-uniform weights, zero biases (padded columns stay zero)
+    std::vector<float> host(off, 0.0f);
+    uint64_t st = seed;
+    for (int l = 0; l < m->layers; l++) {
+        int in_user = l == 0 ? d.inputs : d.hidden;
+        int out_user = m->fout[l];
+        float lim = std::sqrt(6.0f / (float)(in_user + out_user));
+        for (int r = 0; r < out_user; r++)
+            for (int c = 0; c < in_user; c++) {
+                double u = (double)(splitmix(st) >> 11) * (1.0 / 9007199254740992.0);
+                host[m->woff[l] + (long)r * m->fin[l] + c] = (float)((2.0 * u - 1.0) * lim);
+            }
+    }
+    MLP_TRY(hipMemcpy(m->w, host.data(), pb, hipMemcpyHostToDevice));
+    MLP_TRY(hipMemset(m->g, 0, pb));
+    MLP_TRY(hipMemset(m->m1, 0, pb));
+    MLP_TRY(hipMemset(m->m2, 0, pb));
+    int rc = refresh_mirror(m, nullptr);
+    if (rc) return rc;
+    MLP_TRY(hipDeviceSynchronize());
+    *out = guard.release();
+    return 0;
+}
+
+int xmc_mlp_param_count(const xmc_mlp* m, int64_t* count) {
+    if (!m || !count) return fail(-1, "NULL argument");
+    *count = user_count(m);
+    return 0;
+}
+
+static int copy_out(xmc_mlp* m, const float* src, float* dst, int64_t count) {
+    if (!m || !dst) return fail(-1, "NULL argument");
+    if (count != user_count(m)) return fail(-1, "parameter count mismatch");
+    MLP_TRY(hipSetDevice(m->d.device));
+    MLP_TRY(hipDeviceSynchronize());
+    std::vector<float> h(m->nparam);
+    MLP_TRY(hipMemcpy(h.data(), src, m->nparam * 4, hipMemcpyDeviceToHost));
+    for_each_param(m, [&](long u, long p) { dst[u] = h[p]; });
+    return 0;
+}
+
+int xmc_mlp_get_params(xmc_mlp* m, float* params, int64_t count) {
+    return m ? copy_out(m, m->w, params, count) : fail(-1, "NULL model");
+}
+
+int xmc_mlp_get_grads(xmc_mlp* m, float* grads, int64_t count) {
+    return m ? copy_out(m, m->g, grads, count) : fail(-1, "NULL model");
+}
+
+int xmc_mlp_set_params(xmc_mlp* m, const float* params, int64_t count) {
+    if (!m || !params) return fail(-1, "NULL argument");
+    if (count != user_count(m)) return fail(-1, "parameter count mismatch");
+    MLP_TRY(hipSetDevice(m->d.device));
+    MLP_TRY(hipDeviceSynchronize());
+    std::vector<float> h(m->nparam, 0.0f);
+    for_each_param(m, [&](long u, long p) { h[p] = params[u]; });
+    MLP_TRY(hipMemcpy(m->w, h.data(), m->nparam * 4, hipMemcpyHostToDevice));
+    MLP_TRY(hipMemset(m->m1, 0, m->nparam * 4));
+    MLP_TRY(hipMemset(m->m2, 0, m->nparam * 4));
+    m->step = 0;
+    int rc = refresh_mirror(m, nullptr);
+    if (rc) return rc;
+    MLP_TRY(hipDeviceSynchronize());
+    return 0;
+}
+
+int xmc_mlp_forward(xmc_mlp* m, const float* x, int batch, float* out, void* stream) {
+    if (!m || !x || !out) return fail(-1, "NULL argument");
+    if (batch < 0) return fail(-1, "negative batch");
+    MLP_TRY(hipSetDevice(m->d.device));
+    hipStream_t s = (hipStream_t)stream;
+    for (int b0 = 0; b0 < batch; b0 += m->d.maxBatch) {
+        int nb = std::min(m->d.maxBatch, batch - b0);
+        int rows = round_up(nb, BM);
+        int rc = pack(m, x + (long)b0 * m->d.inputs, nb, rows, s);
+        if (rc) return rc;
+        if ((rc = forward_packed(m, rows, s))) return rc;
+        MLP_TRY(hipMemcpyAsync(out + (long)b0 * m->head, m->out, (size_t)nb * m->head * 4,
+                               hipMemcpyDeviceToDevice, s));
+    }
+    return 0;
+}
+
+int xmc_mlp_backward(xmc_mlp* m, const float* x, const float* y, const float* mask, int batch, float scale,
+                     float* loss, void* stream) {
+    if (!m) return fail(-1, "NULL model");
+    MLP_TRY(hipSetDevice(m->d.device));
+    return backward_impl(m, x, y, mask, batch, scale, loss, (hipStream_t)stream);
+}
+
+int xmc_mlp_adam(xmc_mlp* m, float lr, float b1, float b2, float eps, float decay, void* stream) {
+    if (!m) return fail(-1, "NULL model");
+    MLP_TRY(hipSetDevice(m->d.device));
+    return adam_impl(m, lr, b1, b2, eps, decay, (hipStream_t)stream);
+}
+
+int xmc_mlp_train_step(xmc_mlp* m, const float* x, const float* y, const float* mask, int batch, float scale,
+                       float lr, float b1, float b2, float eps, float decay, void* stream) {
+    if (!m) return fail(-1, "NULL model");
+    MLP_TRY(hipSetDevice(m->d.device));
+    int rc = backward_impl(m, x, y, mask, batch, scale, nullptr, (hipStream_t)stream);
+    if (rc) return rc;
+    return adam_impl(m, lr, b1, b2, eps, decay, (hipStream_t)stream);
+}
+
+int xmc_mlp_gemm_check(const float* a, const float* b, int M, int N, int K, float* c, int device) {
+    if (!a || !b || !c) return fail(-1, "NULL argument");
+    if (M <= 0 || N <= 0 || K <= 0 || M % BM || N % BN || K % BK)
+        return fail(-1, "M, N must be multiples of 64 and K of 32");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(-4, "no GPU");
+    MLP_TRY(hipSetDevice(device));
+    std::vector<uint16_t> ha((size_t)M * K), hb((size_t)N * K);
+    for (size_t i = 0; i < ha.size(); i++) ha[i] = f2bf(a[i]);
+    for (size_t i = 0; i < hb.size(); i++) hb[i] = f2bf(b[i]);
+    uint16_t *da = nullptr, *db = nullptr;
+    float *dc = nullptr, *bias = nullptr;
+    int rc = 0;
+    hipError_t e = hipMalloc((void**)&da, ha.size() * 2);
+    if (e == hipSuccess) e = hipMalloc((void**)&db, hb.size() * 2);
+    if (e == hipSuccess) e = hipMalloc((void**)&dc, (size_t)M * N * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&bias, (size_t)N * 4);
+    if (e == hipSuccess) e = hipMemcpy(da, ha.data(), ha.size() * 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, hb.data(), hb.size() * 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(bias, 0, (size_t)N * 4);
+    if (e == hipSuccess) {
+        GemmArgs g{};
+        g.A = da;
+        g.lda = K;
+        g.B = db;
+        g.ldb = K;
+        g.M = M;
+        g.N = N;
+        g.kchunk = K;
+        g.bias = bias;
+        g.outf = dc;
+        g.ldo = N;
+        e = launch_gemm<true, true, EPI_LINEAR>(g, 1, nullptr);
+    }
+    if (e == hipSuccess) e = hipMemcpy(c, dc, (size_t)M * N * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = fail(-2, std::string("gemm check: ") + hipGetErrorString(e));
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dc);
+    (void)hipFree(bias);
+    return rc;
+}
+
+}  // extern "C"

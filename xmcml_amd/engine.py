@@ -32,6 +32,10 @@ EXPORTS = [
     "xmc_batch_create", "xmc_batch_destroy", "xmc_batch_run", "xmc_batch_collect", "xmc_batch_stats",
     "xmc_batch_set_launch", "xmc_batch_get_launch", "xmc_distance_grid", "xmc_launch", "xmc_init_output",
     "xmc_free_output", "xmc_specular_reflectance", "xmc_last_error", "xmc_version",
+    # surrogate model (include/xmcml_surrogate.h)
+    "xmc_mlp_create", "xmc_mlp_destroy", "xmc_mlp_param_count", "xmc_mlp_get_params", "xmc_mlp_set_params",
+    "xmc_mlp_forward", "xmc_mlp_backward", "xmc_mlp_get_grads", "xmc_mlp_adam", "xmc_mlp_train_step",
+    "xmc_mlp_gemm_check",
 ]
 
 
@@ -79,6 +83,19 @@ def lib():
     sig("xmc_specular_reflectance", C.c_double, [_P])
     sig("xmc_last_error", C.c_char_p, [])
     sig("xmc_version", C.c_char_p, [])
+    _F = C.c_float
+    _PF = C.POINTER(C.c_float)
+    sig("xmc_mlp_create", _I, [_P, _U64, C.POINTER(_P)])
+    sig("xmc_mlp_destroy", _I, [_P])
+    sig("xmc_mlp_param_count", _I, [_P, C.POINTER(C.c_int64)])
+    sig("xmc_mlp_get_params", _I, [_P, _PF, C.c_int64])
+    sig("xmc_mlp_set_params", _I, [_P, _PF, C.c_int64])
+    sig("xmc_mlp_forward", _I, [_P, _P, _I, _P, _P])
+    sig("xmc_mlp_backward", _I, [_P, _P, _P, _P, _I, _F, _PF, _P])
+    sig("xmc_mlp_get_grads", _I, [_P, _PF, C.c_int64])
+    sig("xmc_mlp_adam", _I, [_P, _F, _F, _F, _F, _F, _P])
+    sig("xmc_mlp_train_step", _I, [_P, _P, _P, _P, _I, _F, _F, _F, _F, _F, _F, _P])
+    sig("xmc_mlp_gemm_check", _I, [_PF, _PF, _I, _I, _I, _PF, _I])
     _lib = L
     return L
 

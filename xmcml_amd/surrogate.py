@@ -1,0 +1,350 @@
+"""Learned surrogate of layered-tissue photon transport, trained on what the
+batched GPU Monte Carlo produces, with plain Monte Carlo as its fallback.
+
+Question: a planar stack of 2-5 tissue layers under an n = 1 half-space, a
+pencil beam at the origin.  Answer: the diffuse reflectance of 32 rings of
+0.5 mm (weight fraction per ring) and the 64-bin time-of-flight histogram
+over 0-1 ns of all detected weight.  Both are learned in log space; the
+network is the one of include/xmcml_surrogate.h (30 inputs, 4 x 512 GELU,
+96 means + 96 log-variances) running on the hand-written matrix-core
+kernels of xmcml_amd/csrc/xmc_mlp.hip.
+
+Inputs (30): for each of 5 layer slots, ln(mua), ln(mus'), g, n, ln(d)
+mapped linearly from the sampling box to [-1, 1], zeros for empty slots; then
+5 slot-occupied flags.  mus' = mus (1 - g).  The sampling box (LAYER_BOX) is
+the batched workload's distribution (scenes.draw_layers).
+
+Fallback: a configuration whose layers leave the box, or whose predicted
+standard deviation of ln R over the inner rings exceeds the model's
+threshold, is answered by plain batched Monte Carlo instead.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import time
+from dataclasses import dataclass
+from pathlib import Path
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import engine, scenes
+
+MAX_LAYERS = 5
+N_INPUTS = 6 * MAX_LAYERS
+RINGS = 32
+RING_WIDTH = 0.5
+TOF_BINS = 64
+TOF_NS = 1.0
+N_OUTPUTS = RINGS + TOF_BINS
+MIN_COUNT = 16  # detected packets below which a target is masked out
+
+# (low, high, log-scaled) of mua [1/mm], mus' [1/mm], g, n, thickness [mm]
+LAYER_BOX = [(1e-3, 1.0, True), (0.5, 5.0, True), (0.7, 0.95, False), (1.33, 1.5, False), (0.05, 10.0, True)]
+
+Layer = Tuple[float, float, float, float, float]  # (n, mua, mus, g, thickness), as scenes.layered_scene
+
+
+def _unit(v, lo, hi, log):
+    if log:
+        return 2.0 * (math.log(v) - math.log(lo)) / (math.log(hi) - math.log(lo)) - 1.0
+    return 2.0 * (v - lo) / (hi - lo) - 1.0
+
+
+def encode(layers: Sequence[Layer]) -> np.ndarray:
+    """Feature vector (30,) of one configuration."""
+    if not 1 <= len(layers) <= MAX_LAYERS:
+        raise ValueError(f"{len(layers)} layers: the surrogate takes 1..{MAX_LAYERS}")
+    x = np.zeros(N_INPUTS, np.float32)
+    for i, (n, mua, mus, g, d) in enumerate(layers):
+        vals = (mua, mus * (1.0 - g), g, n, d)
+        for k, (v, (lo, hi, lg)) in enumerate(zip(vals, LAYER_BOX)):
+            x[5 * i + k] = _unit(v, lo, hi, lg)
+        x[5 * MAX_LAYERS + i] = 1.0
+    return x
+
+
+def encode_many(configs: Sequence[Sequence[Layer]]) -> np.ndarray:
+    return np.stack([encode(c) for c in configs]) if configs else np.zeros((0, N_INPUTS), np.float32)
+
+
+def in_range(layers: Sequence[Layer], tol: float = 1e-9) -> bool:
+    """True when every layer lies in the sampling box (and 2..5 layers)."""
+    if not 2 <= len(layers) <= MAX_LAYERS:
+        return False
+    for (n, mua, mus, g, d) in layers:
+        vals = (mua, mus * (1.0 - g), g, n, d)
+        for v, (lo, hi, _) in zip(vals, LAYER_BOX):
+            if not (lo * (1 - tol) <= v <= hi * (1 + tol)):
+                return False
+    return True
+
+
+def draw_configs(count: int, seed: int) -> List[List[Layer]]:
+    """`count` configurations of the batched workload's distribution."""
+    rng = np.random.default_rng(seed)
+    return [scenes.draw_layers(rng, int(rng.integers(2, MAX_LAYERS + 1))) for _ in range(count)]
+
+
+def config_scene(layers: Sequence[Layer], photons: int) -> scenes.SceneSpec:
+    return scenes.layered_scene(list(layers), photons, rings_n=RINGS, ring_width=RING_WIDTH, time_bins=TOF_BINS,
+                                time_ns=TOF_NS)
+
+
+def targets(out, photons: int) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """(ln values, mask, packet counts), each (96,): 32 ring reflectances then
+    64 TOF bins of the ring-summed histogram, as weight fractions."""
+    w = np.concatenate([out.det_weight(), out.bin_weight().sum(axis=0)]) / float(photons)
+    cnt = np.concatenate([out.det_count().astype(np.float64), out.bin_count().sum(axis=0).astype(np.float64)])
+    mask = (cnt >= MIN_COUNT) & (w > 0)
+    y = np.where(mask, np.log(np.where(w > 0, w, 1.0)), 0.0)
+    return y.astype(np.float32), mask.astype(np.float32), cnt
+
+
+def simulate(configs: Sequence[Sequence[Layer]], photons, seed: int, chunk: int = 1024,
+             progress=None) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """Plain batched Monte Carlo of every configuration (photons: int or one
+    per configuration), chunk configurations per persistent launch.
+    Returns (ln targets, mask, counts, raw weight fractions), each (n, 96)."""
+    n = len(configs)
+    budgets = np.broadcast_to(np.asarray(photons, dtype=np.int64), (n,))
+    Y = np.zeros((n, N_OUTPUTS), np.float32)
+    M = np.zeros((n, N_OUTPUTS), np.float32)
+    K = np.zeros((n, N_OUTPUTS), np.float64)
+    W = np.zeros((n, N_OUTPUTS), np.float64)
+    for c0 in range(0, n, chunk):
+        specs = [config_scene(configs[i], int(budgets[i])) for i in range(c0, min(n, c0 + chunk))]
+        with engine.Batch(specs) as b:
+            b.run(seed + c0)
+            outs = b.collect()
+        for k, (o, s) in enumerate(zip(outs, specs)):
+            y, m, cnt = targets(o, s.photons)
+            Y[c0 + k], M[c0 + k], K[c0 + k] = y, m, cnt
+            W[c0 + k] = np.where(m > 0, np.exp(y), 0.0)
+        if progress:
+            progress(min(n, c0 + chunk), n)
+    return Y, M, K, W
+
+
+class MLP:
+    """The matrix-core MLP of include/xmcml_surrogate.h.  Device arrays are
+    torch tensors (float32, contiguous, on the model's device)."""
+
+    def __init__(self, inputs: int = N_INPUTS, hidden: int = 512, depth: int = 4, outputs: int = N_OUTPUTS,
+                 max_batch: int = 4096, seed: int = 0, device: int = 0):
+        class Desc(C.Structure):
+            _fields_ = [("inputs", C.c_int), ("hidden", C.c_int), ("depth", C.c_int), ("outputs", C.c_int),
+                        ("maxBatch", C.c_int), ("device", C.c_int)]
+
+        self.inputs, self.hidden, self.depth, self.outputs = inputs, hidden, depth, outputs
+        self.max_batch, self.device = max_batch, device
+        d = Desc(inputs, hidden, depth, outputs, max_batch, device)
+        h = C.c_void_p()
+        engine._check(engine.lib().xmc_mlp_create(C.byref(d), seed, C.byref(h)), "xmc_mlp_create")
+        self._h = h
+        n = C.c_int64()
+        engine._check(engine.lib().xmc_mlp_param_count(h, C.byref(n)), "xmc_mlp_param_count")
+        self.nparam = n.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            engine.lib().xmc_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _vec(self, fn, what) -> np.ndarray:
+        a = np.zeros(self.nparam, np.float32)
+        engine._check(fn(self._h, a.ctypes.data_as(C.POINTER(C.c_float)), self.nparam), what)
+        return a
+
+    def get_params(self) -> np.ndarray:
+        return self._vec(engine.lib().xmc_mlp_get_params, "xmc_mlp_get_params")
+
+    def get_grads(self) -> np.ndarray:
+        return self._vec(engine.lib().xmc_mlp_get_grads, "xmc_mlp_get_grads")
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.nparam:
+            raise ValueError(f"{p.size} parameters, the model has {self.nparam}")
+        engine._check(engine.lib().xmc_mlp_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float)),
+                                                      self.nparam), "xmc_mlp_set_params")
+
+    @staticmethod
+    def _dev(t, cols, name):
+        import torch
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+            raise TypeError(f"{name} must be a contiguous float32 CUDA tensor")
+        if t.dim() != 2 or t.shape[1] != cols:
+            raise ValueError(f"{name} must have shape (batch, {cols})")
+        return C.c_void_p(t.data_ptr())
+
+    def forward(self, x, out=None):
+        import torch
+        if out is None:
+            out = torch.empty((x.shape[0], 2 * self.outputs), dtype=torch.float32, device=x.device)
+        engine._check(engine.lib().xmc_mlp_forward(self._h, self._dev(x, self.inputs, "x"), x.shape[0],
+                                                   self._dev(out, 2 * self.outputs, "out"), None),
+                      "xmc_mlp_forward")
+        return out
+
+    def backward(self, x, y, mask, scale: float, want_loss: bool = True) -> Optional[float]:
+        loss = C.c_float()
+        engine._check(engine.lib().xmc_mlp_backward(
+            self._h, self._dev(x, self.inputs, "x"), self._dev(y, self.outputs, "y"),
+            self._dev(mask, self.outputs, "mask"), x.shape[0], scale, C.byref(loss) if want_loss else None, None),
+            "xmc_mlp_backward")
+        return loss.value if want_loss else None
+
+    def adam(self, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):
+        engine._check(engine.lib().xmc_mlp_adam(self._h, lr, beta1, beta2, eps, weight_decay, None), "xmc_mlp_adam")
+
+    def train_step(self, x, y, mask, scale, lr, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):
+        engine._check(engine.lib().xmc_mlp_train_step(
+            self._h, self._dev(x, self.inputs, "x"), self._dev(y, self.outputs, "y"),
+            self._dev(mask, self.outputs, "mask"), x.shape[0], scale, lr, beta1, beta2, eps, weight_decay, None),
+            "xmc_mlp_train_step")
+
+
+def gemm_check(a: np.ndarray, b: np.ndarray, device: int = 0) -> np.ndarray:
+    """C = A B^T on the matrix-core kernel (bf16 operands, fp32 accumulate)."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    M, K = a.shape
+    N = b.shape[0]
+    c = np.zeros((M, N), np.float32)
+    pf = C.POINTER(C.c_float)
+    engine._check(engine.lib().xmc_mlp_gemm_check(a.ctypes.data_as(pf), b.ctypes.data_as(pf), M, N, K,
+                                                  c.ctypes.data_as(pf), device), "xmc_mlp_gemm_check")
+    return c
+
+
+@dataclass
+class FitLog:
+    steps: int
+    seconds: float
+    first_loss: float
+    last_loss: float
+
+
+class Surrogate:
+    """The trained network plus the target standardisation and the fallback
+    threshold."""
+
+    def __init__(self, mlp: MLP, y_mean: np.ndarray, y_std: np.ndarray, sigma_max: float = math.inf):
+        self.mlp = mlp
+        self.y_mean = np.asarray(y_mean, np.float32)
+        self.y_std = np.asarray(y_std, np.float32)
+        self.sigma_max = float(sigma_max)
+
+    # ---- training -------------------------------------------------------
+    @staticmethod
+    def standardisation(Y: np.ndarray, M: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+        cnt = np.maximum(M.sum(0), 1.0)
+        mean = (Y * M).sum(0) / cnt
+        var = (((Y - mean) ** 2) * M).sum(0) / cnt
+        return mean.astype(np.float32), np.sqrt(np.maximum(var, 1e-6)).astype(np.float32)
+
+    @classmethod
+    def fit(cls, X: np.ndarray, Y: np.ndarray, M: np.ndarray, steps: int = 2000, batch: int = 4096,
+            lr: float = 2e-3, lr_final: float = 2e-5, weight_decay: float = 0.0, seed: int = 0,
+            hidden: int = 512, depth: int = 4, log_every: int = 0, device: int = 0) -> Tuple["Surrogate", FitLog]:
+        import torch
+        dev = torch.device("cuda", device)
+        mean, std = cls.standardisation(Y, M)
+        Ys = np.where(M > 0, (Y - mean) / std, 0.0).astype(np.float32)
+        Xt = torch.from_numpy(np.ascontiguousarray(X, np.float32)).to(dev)
+        Yt = torch.from_numpy(Ys).to(dev)
+        Mt = torch.from_numpy(np.ascontiguousarray(M, np.float32)).to(dev)
+        n = X.shape[0]
+        batch = min(batch, n)
+        mlp = MLP(X.shape[1], hidden, depth, Y.shape[1], batch, seed=seed, device=device)
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        scale = 1.0 / (batch * Y.shape[1])
+        t0 = time.perf_counter()
+        first = last = float("nan")
+        for step in range(steps):
+            frac = step / max(1, steps - 1)
+            lr_t = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * frac))
+            idx = torch.randint(0, n, (batch,), device=dev, generator=g)
+            xb, yb, mb = Xt[idx].contiguous(), Yt[idx].contiguous(), Mt[idx].contiguous()
+            if step == 0 or step == steps - 1 or (log_every and step % log_every == 0):
+                loss = mlp.backward(xb, yb, mb, scale, want_loss=True)
+                mlp.adam(lr_t, weight_decay=weight_decay)
+                if step == 0:
+                    first = loss
+                last = loss
+                if log_every:
+                    print(f"surrogate fit: step {step} loss {loss:.5f}", flush=True)
+            else:
+                mlp.train_step(xb, yb, mb, scale, lr_t, weight_decay=weight_decay)
+        torch.cuda.synchronize(dev)
+        return cls(mlp, mean, std), FitLog(steps, time.perf_counter() - t0, first, last)
+
+    # ---- inference ------------------------------------------------------
+    def predict_device(self, x):
+        """Raw head output for feature rows already on the device."""
+        return self.mlp.forward(x)
+
+    def predict(self, X: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+        """(ln-value means, ln-value standard deviations), each (n, 96)."""
+        import torch
+        x = torch.from_numpy(np.ascontiguousarray(X, np.float32)).to(torch.device("cuda", self.mlp.device))
+        out = self.mlp.forward(x).cpu().numpy()
+        O = self.y_mean.size
+        mu = out[:, :O] * self.y_std + self.y_mean
+        sig = np.exp(0.5 * np.clip(out[:, O:], -10, 10)) * self.y_std
+        return mu, sig
+
+    def uncertainty(self, sig: np.ndarray) -> np.ndarray:
+        """Per-configuration score: mean predicted sd of ln R over the inner 16 rings."""
+        return sig[:, :16].mean(axis=1)
+
+    def answer(self, configs: Sequence[Sequence[Layer]], mc_photons: int = 1_000_000, seed: int = 1):
+        """Ring reflectance (n, 32) and TOF (n, 64) weight fractions for every
+        configuration, with a boolean array of those answered by Monte Carlo."""
+        n = len(configs)
+        ok = np.array([in_range(c) for c in configs], bool)
+        R = np.zeros((n, N_OUTPUTS))
+        if ok.any():
+            X = encode_many([configs[i] for i in np.nonzero(ok)[0]])
+            mu, sig = self.predict(X)
+            good = self.uncertainty(sig) <= self.sigma_max
+            idx = np.nonzero(ok)[0]
+            R[idx] = np.exp(mu)
+            ok[idx[~good]] = False
+        fb = np.nonzero(~ok)[0]
+        if fb.size:
+            _, _, _, W = simulate([configs[i] for i in fb], mc_photons, seed)
+            R[fb] = W
+        return R[:, :RINGS], R[:, RINGS:], ~ok
+
+    # ---- persistence ----------------------------------------------------
+    def save(self, path: Path, meta: Optional[dict] = None):
+        np.savez_compressed(path, params=self.mlp.get_params(), y_mean=self.y_mean, y_std=self.y_std,
+                            sigma_max=np.float64(self.sigma_max),
+                            shape=np.array([self.mlp.inputs, self.mlp.hidden, self.mlp.depth, self.mlp.outputs]),
+                            meta=np.frombuffer(json.dumps(meta or {}).encode(), np.uint8))
+
+    @classmethod
+    def load(cls, path: Path, max_batch: int = 4096, device: int = 0) -> "Surrogate":
+        z = np.load(path, allow_pickle=False)
+        inputs, hidden, depth, outputs = (int(v) for v in z["shape"])
+        mlp = MLP(inputs, hidden, depth, outputs, max_batch, device=device)
+        mlp.set_params(z["params"])
+        s = cls(mlp, z["y_mean"], z["y_std"], float(z["sigma_max"]))
+        s.meta = json.loads(bytes(z["meta"]).decode()) if "meta" in z else {}
+        return s
+
+
+def ring_relative_error(R_pred: np.ndarray, R_ref: np.ndarray, valid: np.ndarray) -> np.ndarray:
+    """|R_pred - R_ref| / R_ref over the valid (configuration, ring) entries."""
+    return np.abs(R_pred[valid] - R_ref[valid]) / R_ref[valid]
