@@ -33,8 +33,9 @@
  *     mcml_kernel.cpp:121-124) or MCML Russian roulette
  *     (stable/plane_parallel/1.0/McmlGpu/kernel/kernel.cu:214-222).
  *   - The scattering grid and the per-detector voxel trajectory maps
- *     (mcml_kernel.cpp:750-810) are not produced by the GPU path; their output
- *     arrays are left untouched.
+ *     (mcml_kernel.cpp:750-810) cost memory and time, so a scene produces them
+ *     only when xmc_options asks (xmc_launch: when the output has the arrays);
+ *     otherwise their output arrays are left untouched.
  */
 #ifndef XMCML_AMD_XMCML_H
 #define XMCML_AMD_XMCML_H
@@ -186,6 +187,7 @@ typedef struct xmc_stats {
     double wRouletteLost;
     double wRouletteGain;
     double wOther;        /* glass misses, surfaces without a next layer */
+    uint64_t trajectoryOverflows; /* detected photons whose trajectory exceeded trajectoryCapacity */
 } xmc_stats;
 
 /* One recorded step of a traced photon (debug / parity).  Positions and
@@ -218,6 +220,11 @@ typedef struct xmc_options {
     int blocksPerCU;     /* cap on persistent workgroups per CU; 0 = occupancy limit */
     int skipAbsorption;  /* 1 = do not tally the absorption grid */
     int noDistanceGrid;  /* 1 = always walk the BVH (no conservative distance culling) */
+    int scattering;      /* 1 = tally the scattering grid (UpdatePhotonTrajectory, mcml_kernel.cpp:750-792) */
+    int trajectories;    /* 1 = tally the per-detector voxel counts (UpdateDetectorTrajectory, :794-810) */
+    int trajectoryCapacity; /* voxels kept per photon, 0 = 20480 (MAX_TRAJECTORY_SIZE, mcml_kernel_types.h:8);
+                             * a detected photon with more crossings counts only the first ones and is
+                             * reported in xmc_stats.trajectoryOverflows */
 } xmc_options;
 
 typedef struct xmc_scene xmc_scene;

@@ -73,6 +73,23 @@ int xmc_mlp_get_grads(xmc_mlp* m, float* grads, int64_t count);
  *   w -= lr * (m/(1-b1^t)) / (sqrt(v/(1-b2^t)) + eps) + lr*weightDecay*w   (weights only). */
 int xmc_mlp_adam(xmc_mlp* m, float lr, float beta1, float beta2, float eps, float weightDecay, void* stream);
 
+/* One training step on `batch` rows drawn uniformly (with replacement) from
+ * a resident data set X_dev [rows][inputs], Y_dev and M_dev [rows][outputs]:
+ * row i of the batch is hash(seed, step, i) mod rows (SplitMix64), then
+ * backward and AdamW as above.  If loss != NULL the call synchronises and
+ * stores the batch loss before the update. */
+int xmc_mlp_train_sampled(xmc_mlp* m, const float* X_dev, const float* Y_dev, const float* M_dev, int64_t rows,
+                          int batch, uint64_t seed, int64_t step, float scale, float lr, float beta1, float beta2,
+                          float eps, float weightDecay, float* loss, void* stream);
+
+/* Device memory for the arrays above, on the model's kind of device (plain
+ * hipMalloc/hipMemcpy; kind 1 = host to device, 2 = device to host,
+ * 3 = device to device).  xmc_dev_sync waits for all work on `device`. */
+int xmc_dev_alloc(int device, uint64_t bytes, void** ptr);
+int xmc_dev_free(void* ptr);
+int xmc_dev_copy(void* dst, const void* src, uint64_t bytes, int kind);
+int xmc_dev_sync(int device);
+
 /* Convenience: xmc_mlp_backward (no loss readback) then xmc_mlp_adam. */
 int xmc_mlp_train_step(xmc_mlp* m, const float* x_dev, const float* y_dev, const float* mask_dev, int batch,
                        float scale, float lr, float beta1, float beta2, float eps, float weightDecay, void* stream);

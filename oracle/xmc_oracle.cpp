@@ -237,6 +237,24 @@ inline V3<double> plane_segment_point(V3<double> a, V3<double> b, double z) {
     return r;
 }
 
+// The same steps in float, written as the GPU kernel evaluates them: with
+// n = e_z the two dot products are exactly z - a.z and b.z - a.z.
+inline V3<float> plane_segment_point_f(V3<float> a, V3<float> b, float z) {
+    const float kFar = 1.0e30f;
+    float d = z - a.z;
+    V3<float> w = sub(b, a);
+    float e = w.z;
+    V3<float> r = {kFar, kFar, kFar};
+    if (e != 0.0f) {
+        float t = d / e;
+        r.x = a.x + w.x * t;
+        r.y = a.y + w.y * t;
+        r.z = a.z + w.z * t;
+        if (dot(sub(a, r), sub(b, r)) > 0.0f) r = {kFar, kFar, kFar};
+    }
+    return r;
+}
+
 // Precision 65 also feeds the double walk the float-rounded inputs the float
 // walk sees (optics, geometry, detectors, source), so that a float-vs-double
 // comparison isolates the arithmetic.
@@ -928,9 +946,33 @@ template <class R> struct Walker {
         }
     }
 
-    // UpdatePhotonTrajectory (:750-792), double precision only
+    // UpdatePhotonTrajectory (:750-792).  Double: the reference's arithmetic
+    // on the raw Area.  Float: the same steps in float on the float-rounded
+    // Area, as the GPU kernel performs them.
     void update_trajectory(const Photon<R>& p, V3<R> prev) {
         if (!scat) return;
+        if constexpr (sizeof(R) == 4) {
+            V3<float> a, b;
+            if (p.pos.z > prev.z) { a = prev; b = p.pos; }
+            else if (p.pos.z < prev.z) { a = p.pos; b = prev; }
+            else return;
+            const float step = sc.area_length.z / sc.area_part[2];
+            int plane_index = (int)((a.z - sc.area_corner.z + step / 2.0f) / step);
+            float plane = sc.area_corner.z + plane_index * step + step / 2.0f;
+            const float top = sc.area_corner.z + sc.area_length.z;
+            while ((plane <= b.z) && (plane < top)) {
+                V3<float> q = plane_segment_point_f(a, b, plane);
+                int v[3];
+                int idx = sc.area_index(q, v);
+                if (idx < 0) break;
+                traj.x.push_back((uint8_t)v[0]);
+                traj.y.push_back((uint8_t)v[1]);
+                traj.z.push_back((uint8_t)v[2]);
+                tl.scattering[idx] += (double)p.weight;
+                plane += step;
+            }
+            return;
+        }
         V3<double> cur = {(double)p.pos.x, (double)p.pos.y, (double)p.pos.z};
         V3<double> pv = {(double)prev.x, (double)prev.y, (double)prev.z};
         V3<double> a, b;
@@ -968,7 +1010,9 @@ template <class R> struct Walker {
         return bx * a->partitionNumber.y * a->partitionNumber.z + by * a->partitionNumber.z + bz;
     }
 
-    void move(Photon<R>& p) {  // MovePhotonAndUpdateTrajectory (:158-174)
+    // MovePhotonAndUpdateTrajectory (:158-174); `track` = false leaves the
+    // trajectory update to the caller
+    void move(Photon<R>& p, bool track = true) {
         V3<R> prev = p.pos;
         p.time += p.step * sc.layers[p.layer].n;
         if (sc.in->targetRangeLayers[p.layer]) p.target_range += p.step;
@@ -976,7 +1020,7 @@ template <class R> struct Walker {
         p.pos.x += p.step * p.dir.x;
         p.pos.y += p.step * p.dir.y;
         p.pos.z += p.step * p.dir.z;
-        update_trajectory(p, prev);
+        if (track) update_trajectory(p, prev);
     }
 
     // MovePhotonOnMinDistanceAndUpdateTrajectory (:176-188) in double; in
@@ -993,9 +1037,11 @@ template <class R> struct Walker {
         } else {
             R d = offset_scale(hit);
             if (dot(p.dir, n) < (R)0) d = -d;
+            V3<R> prev = p.pos;  // the incident-side point the hit move ended at
             p.pos.x = hit.x + d * n.x;
             p.pos.y = hit.y + d * n.y;
             p.pos.z = hit.z + d * n.z;
+            update_trajectory(p, prev);
         }
     }
 
@@ -1163,14 +1209,19 @@ template <class R> struct Walker {
         int v[3];
         if (h.found) {
             p.step = h.distance - (R)0.01 * Prec<R>::kMinDistance;
-            move(p);
+            V3<R> start = p.pos;
+            move(p, sizeof(R) == 8);
             V3<R> hit = p.pos;
             if constexpr (sizeof(R) == 4) {
+                // the float walk stops delta short of the surface on the incident
+                // side (the double walk stops 1e-10 short along the direction);
+                // its path, and so its plane crossings, ends there
                 R d = offset_scale(hit);
                 if (dot(p.dir, h.normal) < (R)0) d = -d;
                 p.pos.x = hit.x - d * h.normal.x;
                 p.pos.y = hit.y - d * h.normal.y;
                 p.pos.z = hit.z - d * h.normal.z;
+                update_trajectory(p, start);
             }
             if (sc.area_index(p.pos, v) >= 0) {
                 cross(p, h, hit, rng);
@@ -1260,7 +1311,6 @@ void add_into(xmc_output* out, const Tally& t, int T, bool scat) {
 template <class R>
 int run_impl(const xmc_input* in, xmc_output* out, uint64_t seed, uint64_t first, uint64_t n,
              int nthreads, bool scat, xmco_counters* c) {
-    if (sizeof(R) == 4) scat = false;  // float mode mirrors the GPU, which has no scattering grid
     Scene<R> sc;
     sc.init(in);
     int T = in->timeScaleSize;

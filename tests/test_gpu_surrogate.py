@@ -20,14 +20,12 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def torch_cuda():
-    torch = pytest.importorskip("torch")
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
-    return torch
+def dev(engine):
+    """engine.DeviceArray factory (the engine fixture skips without a GPU)."""
+    return engine.DeviceArray
 
 
-def test_gemm_matches_reference(torch_cuda):
+def test_gemm_matches_reference(dev):
     from xmcml_amd.surrogate import gemm_check
     rng = np.random.default_rng(0)
     for (M, N, K) in [(64, 64, 32), (128, 192, 96), (256, 64, 512)]:
@@ -43,7 +41,7 @@ def test_gemm_matches_reference(torch_cuda):
         assert np.array_equal(gemm_check(ai, bi), (ai.astype(np.float64) @ bi.T.astype(np.float64)).astype(np.float32))
 
 
-def _model(torch, hidden=128, depth=2, outputs=32, max_batch=256, seed=3):
+def _model(hidden=128, depth=2, outputs=32, max_batch=256, seed=3):
     from xmcml_amd.surrogate import MLP
     m = MLP(30, hidden, depth, outputs, max_batch, seed=seed)
     rng = np.random.default_rng(seed)
@@ -57,27 +55,25 @@ def _model(torch, hidden=128, depth=2, outputs=32, max_batch=256, seed=3):
     return m, p, shapes
 
 
-def _batch(torch, n, outputs, seed=4):
+def _batch(dev, n, outputs, seed=4):
     rng = np.random.default_rng(seed)
     x = rng.uniform(-1, 1, (n, 30)).astype(np.float32)
     y = rng.standard_normal((n, outputs)).astype(np.float32)
     mask = (rng.random((n, outputs)) > 0.2).astype(np.float32)
-    dev = torch.device("cuda", 0)
-    return x, y, mask, [torch.from_numpy(a).to(dev) for a in (x, y, mask)]
+    return x, y, mask, [dev(a.shape, a) for a in (x, y, mask)]
 
 
-def test_forward_matches_reference(torch_cuda):
-    torch = torch_cuda
-    m, p, shapes = _model(torch)
-    x, y, mask, (xt, _, _) = _batch(torch, 200, 32)  # 200: not a multiple of the 64-row tile
-    out = m.forward(xt).cpu().numpy()
+def test_forward_matches_reference(dev):
+    m, p, shapes = _model()
+    x, y, mask, (xt, _, _) = _batch(dev, 200, 32)  # 200: not a multiple of the 64-row tile
+    out = m.forward(xt).numpy()
     ref, _ = R.RefMLP(p, 30, 128, 2, 32).forward(x)
     err = np.abs(out - ref)
     assert np.linalg.norm(out - ref) <= 5e-3 * np.linalg.norm(ref), np.linalg.norm(out - ref) / np.linalg.norm(ref)
     assert err.max() <= 0.05 * np.abs(ref).max()
     # more rows than maxBatch: processed in chunks, same answer per row
     big = np.concatenate([x, x, x[:100]])
-    out2 = m.forward(torch.from_numpy(big).to(xt.device)).cpu().numpy()
+    out2 = m.forward(dev(big.shape, big)).numpy()
     assert np.array_equal(out2[:200], out) and np.array_equal(out2[200:400], out)
     assert np.array_equal(out2[400:], out[:100])
 
@@ -90,11 +86,10 @@ def _cmp(g, r):
     return rel, cos
 
 
-def test_backward_matches_reference(torch_cuda):
-    torch = torch_cuda
-    m, p, shapes = _model(torch)
+def test_backward_matches_reference(dev):
+    m, p, shapes = _model()
     n = 256
-    x, y, mask, (xt, yt, mt) = _batch(torch, n, 32)
+    x, y, mask, (xt, yt, mt) = _batch(dev, n, 32)
     scale = 1.0 / mask.sum()
     loss = m.backward(xt, yt, mt, scale)
     grads = m.get_grads()
@@ -109,10 +104,9 @@ def test_backward_matches_reference(torch_cuda):
     assert np.isfinite(grads).all()
 
 
-def test_adam_matches_reference(torch_cuda):
-    torch = torch_cuda
-    m, p, shapes = _model(torch)
-    x, y, mask, (xt, yt, mt) = _batch(torch, 128, 32)
+def test_adam_matches_reference(dev):
+    m, p, shapes = _model()
+    x, y, mask, (xt, yt, mt) = _batch(dev, 128, 32)
     flags = R.weight_flags(shapes)
     m1 = np.zeros_like(p)
     m2 = np.zeros_like(p)
@@ -126,8 +120,7 @@ def test_adam_matches_reference(torch_cuda):
         assert np.allclose(got, cur, rtol=1e-5, atol=1e-7), (step, np.max(np.abs(got - cur)))
 
 
-def test_training_fits_a_smooth_function(torch_cuda):
-    torch = torch_cuda
+def test_training_fits_a_smooth_function(dev):
     from xmcml_amd.surrogate import Surrogate
     rng = np.random.default_rng(7)
     n = 8192
@@ -144,13 +137,13 @@ def test_training_fits_a_smooth_function(torch_cuda):
     assert rmse < 0.15 * base
 
 
-def test_surrogate_beats_plain_mc_at_equal_cost(torch_cuda):
-    """Trained on 2048 batched-MC configurations, the surrogate answers
+def test_surrogate_beats_plain_mc_at_equal_cost(dev):
+    """Trained on 4096 batched-MC configurations, the surrogate answers
     held-out configurations with a smaller ring-reflectance error than plain
     Monte Carlo given 1000x the surrogate's time per configuration."""
-    torch = torch_cuda
+    from xmcml_amd import engine
     from xmcml_amd import surrogate as S
-    train_cfg = S.draw_configs(2048, 11)
+    train_cfg = S.draw_configs(4096, 11)
     test_cfg = S.draw_configs(64, 12)
     t0 = time.perf_counter()
     Y, M, _, _ = S.simulate(train_cfg, 20000, seed=100)
@@ -158,14 +151,16 @@ def test_surrogate_beats_plain_mc_at_equal_cost(torch_cuda):
     _, Mref, Kref, Wref = S.simulate(test_cfg, 400000, seed=200)
     sur, log = S.Surrogate.fit(S.encode_many(train_cfg), Y, M, steps=1500, batch=1024, lr=2e-3)
     Xte = S.encode_many(test_cfg)
-    xt = torch.from_numpy(Xte).cuda()
-    sur.predict_device(xt)
-    torch.cuda.synchronize()
+    # throughput on a query batch of realistic size (the test configurations, repeated)
+    Xq = np.tile(Xte, (512, 1))
+    xt = dev(Xq.shape, Xq)
+    out = sur.predict_device(xt)
+    engine.device_sync()
     t0 = time.perf_counter()
-    for _ in range(20):
-        sur.predict_device(xt)
-    torch.cuda.synchronize()
-    t_per_cfg = (time.perf_counter() - t0) / (20 * len(test_cfg))
+    for _ in range(5):
+        sur.predict_device(xt, out)
+    engine.device_sync()
+    t_per_cfg = (time.perf_counter() - t0) / (5 * len(Xq))
     mu, _ = sur.predict(Xte)
     valid = (Kref[:, :S.RINGS] >= 400)
     err_s = np.median(S.ring_relative_error(np.exp(mu[:, :S.RINGS]), Wref[:, :S.RINGS], valid))

@@ -156,6 +156,7 @@ class Stats(C.Structure):
         ("wRouletteLost", C.c_double),
         ("wRouletteGain", C.c_double),
         ("wOther", C.c_double),
+        ("trajectoryOverflows", C.c_uint64),
     ]
 
 
@@ -194,6 +195,9 @@ class Options(C.Structure):
         ("blocksPerCU", C.c_int),
         ("skipAbsorption", C.c_int),
         ("noDistanceGrid", C.c_int),
+        ("scattering", C.c_int),
+        ("trajectories", C.c_int),
+        ("trajectoryCapacity", C.c_int),
     ]
 
 

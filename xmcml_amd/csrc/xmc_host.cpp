@@ -113,6 +113,12 @@ struct xmc_scene {
     DevBuf<int2> spill;
     DevBuf<int> perm_dev;
     DevBuf<uint8_t> dist;
+    // voxel-plane crossings: scattering grid, detector trajectory counts and
+    // the per-lane trajectory lists (allocated only when asked for)
+    DevBuf<double> scat;
+    DevBuf<unsigned long long> dtraj;
+    DevBuf<uint32_t> lane_vox;
+    int traj_cap = 0;
     xmc::Params base;
     std::vector<double> h_t_start, h_t_finish;
 };
@@ -208,6 +214,16 @@ static int configure_launch(int device, int depth, int block, int blocks_per_cu,
     size_t lanes = (size_t)(*blocks_out) * block;
     HIP_TRY(spill.alloc(P.spill_depth > 0 ? (size_t)P.spill_depth * lanes : 1));
     P.spill = spill.p;
+    return XMC_OK;
+}
+
+// One trajectory list of traj_cap voxel indices per persistent lane.
+static int alloc_lane_voxels(xmc_scene* sc) {
+    xmc::Params& P = sc->base;
+    if (!P.paths.det_traj) return XMC_OK;
+    size_t lanes = (size_t)sc->blocks * (size_t)(sc->block > 0 ? sc->block : 256);
+    HIP_TRY(sc->lane_vox.alloc(lanes * (size_t)sc->traj_cap));
+    P.paths.lane_voxels = sc->lane_vox.p;
     return XMC_OK;
 }
 
@@ -425,6 +441,23 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     P.tally.stats = s + sc->off_st;
     P.queue = s + sc->off_q;
 
+    // scattering grid and detector trajectory maps (mcml_kernel.cpp:750-810)
+    if (o.trajectoryCapacity < 0) return fail(XMC_ERR_ARG, "trajectoryCapacity must be >= 0");
+    std::memset(&P.paths, 0, sizeof(P.paths));
+    sc->traj_cap = o.trajectoryCapacity > 0 ? o.trajectoryCapacity : 20480;
+    P.paths.grid_size = sc->grid;
+    P.paths.capacity = sc->traj_cap;
+    if (o.scattering) {
+        HIP_TRY(sc->scat.alloc(sc->grid));
+        HIP_TRY(hipMemset(sc->scat.p, 0, (size_t)sc->grid * 8));
+        P.paths.scattering = sc->scat.p;
+    }
+    if (o.trajectories && sc->n_det > 0) {
+        HIP_TRY(sc->dtraj.alloc((size_t)sc->n_det * sc->grid));
+        HIP_TRY(hipMemset(sc->dtraj.p, 0, (size_t)sc->n_det * sc->grid * 8));
+        P.paths.det_traj = sc->dtraj.p;
+    }
+
     // photons and streams
     const uint64_t jump0 = mcg_pow(xmc::kMcgA, 1ull << xmc::kStreamLog2) & xmc::kMcgMask;
     uint64_t j = jump0;
@@ -448,6 +481,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
         rc = configure_launch(sc->device, sc->ref.depth, sc->block, o.blocksPerCU, xmc_transport_occupancy,
                               &sc->blocks, sc->spill, P);
         if (rc) return rc;
+        if ((rc = alloc_lane_voxels(sc))) return rc;
         HIP_TRY(hipStreamCreate(&sc->stream));
     }
     *out = guard.release();
@@ -464,7 +498,7 @@ int xmc_scene_set_launch(xmc_scene* sc, int block, int blocks_per_cu) {
     if (rc) return rc;
     sc->block = block;
     sc->blocks = blocks;
-    return XMC_OK;
+    return alloc_lane_voxels(sc);
 }
 
 int xmc_scene_get_launch(xmc_scene* sc, int* block, int* blocks) {
@@ -554,6 +588,28 @@ static int collect_impl(xmc_scene* sc, xmc_output* out, hipStream_t stream) {
             di.timeScale[b].numberOfPhotons += h[sc->off_bc + k];
         }
     }
+    const size_t G = (size_t)sc->grid;
+    if (sc->scat.p) {
+        if (out->scattering) {
+            if (out->gridSize != sc->grid) return fail(XMC_ERR_ARG, "output gridSize does not match area");
+            std::vector<double> g(G);
+            HIP_TRY(hipMemcpy(g.data(), sc->scat.p, G * 8, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < G; i++) out->scattering[i] += g[i];
+        }
+        HIP_TRY(xmc_launch_zero(reinterpret_cast<unsigned long long*>(sc->scat.p), G, stream));
+    }
+    if (sc->dtraj.p) {
+        std::vector<unsigned long long> g(G);
+        for (int d = 0; d < sc->n_det; d++) {
+            uint64_t* t = out->detectorInfo[d].trajectory;
+            if (!t) continue;
+            if (out->detectorInfo[d].trajectorySize != sc->grid)
+                return fail(XMC_ERR_ARG, "detector trajectorySize does not match area");
+            HIP_TRY(hipMemcpy(g.data(), sc->dtraj.p + d * G, G * 8, hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < G; i++) t[i] += g[i];
+        }
+        HIP_TRY(xmc_launch_zero(sc->dtraj.p, (size_t)sc->n_det * G, stream));
+    }
     out->specularReflectance = sc->rsp;
     // zero everything except the counters of xmc_scene_stats
     HIP_TRY(xmc_launch_zero(sc->slab.p, sc->off_st, stream));
@@ -580,6 +636,7 @@ static int read_stats(unsigned long long* dev, hipStream_t stream, xmc_stats* st
     st->wRouletteLost = led[5];
     st->wRouletteGain = led[6];
     st->wOther = led[7];
+    st->trajectoryOverflows = c[6];
     HIP_TRY(xmc_launch_zero(dev, 16, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     return XMC_OK;
@@ -729,6 +786,9 @@ int xmc_launch(const xmc_input* in, xmc_output* out, xmc_rng* rng, const xmc_opt
     std::memset(&o, 0, sizeof(o));
     if (opt) o = *opt;
     if (!out->absorption) o.skipAbsorption = 1;
+    if (out->scattering) o.scattering = 1;
+    for (int d = 0; d < out->numberOfDetectors; d++)
+        if (out->detectorInfo && out->detectorInfo[d].trajectory) o.trajectories = 1;
     xmc_scene* sc = nullptr;
     int rc = xmc_scene_create(in, &o, &sc);
     if (rc) return rc;
@@ -885,6 +945,8 @@ int xmc_batch_create(const xmc_input* configs, int n, const xmc_options* opt, xm
     xmc_options o;
     std::memset(&o, 0, sizeof(o));
     if (opt) o = *opt;
+    if (o.scattering || o.trajectories)
+        return fail(XMC_ERR_ARG, "a batch tallies no scattering grid or trajectories (use one scene per input)");
     auto* b = new xmc_batch();
     std::unique_ptr<xmc_batch, int (*)(xmc_batch*)> guard(b, xmc_batch_destroy);
     b->device = o.device;

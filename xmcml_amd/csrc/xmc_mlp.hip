@@ -225,6 +225,26 @@ __global__ void adam_kernel(float* w, const float* g, float* m1, float* m2, uint
     if (mirror) mirror[i] = f2bf(x);
 }
 
+__device__ inline uint64_t splitmix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// batch row r <- data row splitmix64(key + r) mod rows
+__global__ void gather_kernel(const float* X, const float* Y, const float* M, long rows, int nin, int nout,
+                              int batch, uint64_t key, float* xb, float* yb, float* mb) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= batch) return;
+    long i = (long)(splitmix64(key + (uint64_t)r) % (uint64_t)rows);
+    for (int c = 0; c < nin; c++) xb[(long)r * nin + c] = X[i * nin + c];
+    for (int c = 0; c < nout; c++) {
+        yb[(long)r * nout + c] = Y[i * nout + c];
+        mb[(long)r * nout + c] = M[i * nout + c];
+    }
+}
+
 __global__ void to_bf16_kernel(const float* src, uint16_t* dst, long count) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) dst[i] = f2bf(src[i]);
@@ -282,6 +302,7 @@ struct xmc_mlp {
     float* part = nullptr;        // split-K partials
     long part_count = 0;
     float* lossp = nullptr;       // loss partial sums
+    float *xb = nullptr, *yb = nullptr, *mb = nullptr;  // sampled batch (xmc_mlp_train_sampled)
     int loss_blocks = 0;
     int step = 0;
     std::vector<void*> allocs;
@@ -546,6 +567,9 @@ int xmc_mlp_create(const xmc_mlp_desc* desc, uint64_t seed, xmc_mlp** out) {
     for (int l = 0; l < m->layers; l++) maxw = std::max(maxw, (long)m->fin[l] * m->fout[l]);
     m->part_count = 8 * maxw;
     MLP_TRY(dalloc(m, (void**)&m->part, m->part_count * 4));
+    MLP_TRY(dalloc(m, (void**)&m->xb, (size_t)d.maxBatch * d.inputs * 4));
+    MLP_TRY(dalloc(m, (void**)&m->yb, (size_t)d.maxBatch * d.outputs * 4));
+    MLP_TRY(dalloc(m, (void**)&m->mb, (size_t)d.maxBatch * d.outputs * 4));
     m->loss_blocks = (int)(((long)m->rows * d.outputs + 255) / 256);
     MLP_TRY(dalloc(m, (void**)&m->lossp, (size_t)m->loss_blocks * 4));
 
@@ -652,6 +676,53 @@ int xmc_mlp_train_step(xmc_mlp* m, const float* x, const float* y, const float* 
     int rc = backward_impl(m, x, y, mask, batch, scale, nullptr, (hipStream_t)stream);
     if (rc) return rc;
     return adam_impl(m, lr, b1, b2, eps, decay, (hipStream_t)stream);
+}
+
+int xmc_mlp_train_sampled(xmc_mlp* m, const float* X, const float* Y, const float* M, int64_t rows, int batch,
+                          uint64_t seed, int64_t step, float scale, float lr, float b1, float b2, float eps,
+                          float decay, float* loss, void* stream) {
+    if (!m || !X || !Y || !M) return fail(-1, "NULL argument");
+    if (rows <= 0) return fail(-1, "empty data set");
+    if (batch <= 0 || batch > m->d.maxBatch) return fail(-1, "batch outside 1..maxBatch");
+    MLP_TRY(hipSetDevice(m->d.device));
+    hipStream_t s = (hipStream_t)stream;
+    uint64_t key = seed * 0x9e3779b97f4a7c15ull + (uint64_t)step * 0xd1b54a32d192ed03ull;
+    hipLaunchKernelGGL(gather_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, X, Y, M, (long)rows,
+                       m->d.inputs, m->d.outputs, batch, key, m->xb, m->yb, m->mb);
+    MLP_TRY(hipGetLastError());
+    int rc = backward_impl(m, m->xb, m->yb, m->mb, batch, scale, loss, s);
+    if (rc) return rc;
+    return adam_impl(m, lr, b1, b2, eps, decay, s);
+}
+
+int xmc_dev_alloc(int device, uint64_t bytes, void** ptr) {
+    if (!ptr) return fail(-1, "NULL argument");
+    *ptr = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(-4, "no GPU");
+    MLP_TRY(hipSetDevice(device));
+    MLP_TRY(hipMalloc(ptr, bytes > 0 ? bytes : 1));
+    return 0;
+}
+
+int xmc_dev_free(void* ptr) {
+    if (ptr) MLP_TRY(hipFree(ptr));
+    return 0;
+}
+
+int xmc_dev_copy(void* dst, const void* src, uint64_t bytes, int kind) {
+    if ((!dst || !src) && bytes) return fail(-1, "NULL argument");
+    hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost
+                    : kind == 3 ? hipMemcpyDeviceToDevice : hipMemcpyDefault;
+    if (kind < 1 || kind > 3) return fail(-1, "copy kind must be 1, 2 or 3");
+    if (bytes) MLP_TRY(hipMemcpy(dst, src, bytes, k));
+    return 0;
+}
+
+int xmc_dev_sync(int device) {
+    MLP_TRY(hipSetDevice(device));
+    MLP_TRY(hipDeviceSynchronize());
+    return 0;
 }
 
 int xmc_mlp_gemm_check(const float* a, const float* b, int M, int N, int K, float* c, int device) {

@@ -49,7 +49,18 @@ struct Tallies {
     double* bin_weight;      // ndet * T
     double* bin_target;
     unsigned long long* bin_count;
-    unsigned long long* stats;  // 6 event counters, then 8 ledger doubles from word 8
+    unsigned long long* stats;  // 6 event counters, trajectory overflows, -, then 8 ledger doubles from word 8
+};
+
+// Voxel-centre plane crossings of the photon path (UpdatePhotonTrajectory,
+// mcml_kernel.cpp:750-792): the scattering grid and, for detected photons,
+// the per-detector voxel counts (UpdateDetectorTrajectory, :794-810).
+struct Paths {
+    double* scattering;              // gridSize, or null
+    unsigned long long* det_traj;    // ndet * gridSize, or null
+    uint32_t* lane_voxels;           // lanes * capacity voxel indices (the photon's trajectory)
+    int capacity;                    // voxels kept per photon
+    int grid_size;
 };
 
 struct Area {
@@ -102,6 +113,7 @@ struct Params {
     int2* spill;
     int spill_depth;  // entries per lane in the spill
     Trace trace;
+    Paths paths;
 };
 
 struct QueryParams {

@@ -436,6 +436,55 @@ __device__ __forceinline__ int area_index(const Area& A, V3 p) {
     return bx * A.part[1] * A.part[2] + by * A.part[2] + bz;
 }
 
+// GetPlaneSegmentIntersectionPoint (mcml_math.cpp:59-94) for the plane
+// z = const: the two dot products with e_z are exactly z - a.z and b.z - a.z.
+__device__ __forceinline__ V3 plane_point(V3 a, V3 b, float z) {
+    const float kFar = 1.0e30f;
+    float d = z - a.z;
+    V3 w = sub(b, a);
+    float e = w.z;
+    V3 r = {kFar, kFar, kFar};
+    if (e != 0.0f) {
+        float t = d / e;
+        r = {a.x + w.x * t, a.y + w.y * t, a.z + w.z * t};
+        if (dot(sub(a, r), sub(b, r)) > 0.0f) r = {kFar, kFar, kFar};
+    }
+    return r;
+}
+
+// UpdatePhotonTrajectory (:750-792): every voxel-centre plane z_k = corner +
+// (k + 1/2) step crossed by the segment p0-p1 adds the weight to the
+// scattering grid and the voxel to the photon's trajectory (kept up to
+// `capacity` voxels; n counts them all).  Stops at the first crossing outside
+// the Area, as the reference does.
+__device__ void path_crossings(const Params& C, V3 p0, V3 p1, float w, uint32_t* vox, int& n) {
+    V3 a, b;
+    if (p1.z > p0.z) {
+        a = p0;
+        b = p1;
+    } else if (p1.z < p0.z) {
+        a = p1;
+        b = p0;
+    } else {
+        return;
+    }
+    const Area& A = C.area;
+    const float step = A.length[2] / A.partf[2];
+    int plane_index = (int)((a.z - A.corner[2] + step / 2.0f) / step);
+    float plane = A.corner[2] + plane_index * step + step / 2.0f;
+    const float top = A.corner[2] + A.length[2];
+    while ((plane <= b.z) && (plane < top)) {
+        int idx = area_index(A, plane_point(a, b, plane));
+        if (idx < 0) break;
+        if (C.paths.scattering) unsafeAtomicAdd(C.paths.scattering + idx, (double)w);
+        if (vox) {
+            if (n < C.paths.capacity) vox[n] = (uint32_t)idx;
+            n++;
+        }
+        plane += step;
+    }
+}
+
 // GetDetectorId (:663-742): first matching cube, then ring.
 __device__ int detector_id(const Detectors& D, V3 p, V3 d, uint32_t visited) {
     for (int i = 0; i < D.n_cube; ++i) {
@@ -536,7 +585,7 @@ enum { kAbs = 0, kEsc, kDet, kArea, kKill, kRLost, kRGain, kOther, kLedger };
 // spill, trace, counters) come from P; per-configuration fields (geometry,
 // optics, detectors, tallies, source, termination) from CFG(), which is P
 // itself for a single scene and the lane's entry of `cfgs` in a batch.
-template <bool TRACE, bool BATCHED>
+template <bool TRACE, bool BATCHED, bool PATHS>
 __device__ __forceinline__ void transport_body(const Params& P, const Params* __restrict__ cfgs,
                                                const uint64_t* __restrict__ prefix, int ncfg) {
     extern __shared__ int2 lds_stack[];
@@ -568,6 +617,13 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     uint32_t draws = 0;
     Recorder<TRACE> rec{nullptr, 0, 0};
     unsigned long long n_scatter = 0, n_boundary = 0, n_roulette = 0, n_photons = 0, n_culled = 0;
+    unsigned long long n_overflow = 0;
+    uint32_t* vox = nullptr;  // this lane's trajectory (voxel indices)
+    int nvox = 0;
+    if constexpr (PATHS) {
+        if (P.paths.det_traj)
+            vox = P.paths.lane_voxels + (size_t)(blockIdx.x * blockDim.x + tid) * (size_t)P.paths.capacity;
+    }
 
     auto load_layer = [&](int l) {
         const Layers& L = CFG().layers;
@@ -623,6 +679,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                     alive = true;
                     n_photons++;
                     draws = 0;
+                    nvox = 0;
                     if (TRACE) {
                         uint64_t k = pid - P.trace.first;
                         rec.ev = (k < P.trace.n) ? P.trace.events + k * (uint64_t)P.trace.max_events : nullptr;
@@ -672,6 +729,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             tim += step * Ln;
             if (Ltarget) trange += step;
             else orange += step;
+            V3 start = pos;
             pos.x += step * dir.x;
             pos.y += step * dir.y;
             pos.z += step * dir.z;
@@ -682,6 +740,8 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             pos.x = hit.x - ofs * nrm.x;
             pos.y = hit.y - ofs * nrm.y;
             pos.z = hit.z - ofs * nrm.z;
+            // the path ends at the incident-side point short of the surface
+            if constexpr (PATHS) path_crossings(C, start, pos, w, vox, nvox);
             if (area_index(C.area, pos) >= 0) {
                 // CrossBoundary (:229-284)
                 float ci = dot(dir, nrm);
@@ -722,6 +782,15 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                                 }
                                 unsafeAtomicAdd(C.tally.det_target + id, (double)tw);
                                 atomicAdd(ledger + kDet, wd);
+                                if constexpr (PATHS) {
+                                    if (vox) {  // UpdateDetectorTrajectory (:794-810)
+                                        unsigned long long* dt =
+                                            C.paths.det_traj + (size_t)id * (size_t)C.paths.grid_size;
+                                        int m = nvox < C.paths.capacity ? nvox : C.paths.capacity;
+                                        for (int i = 0; i < m; ++i) atomicAdd(dt + vox[i], 1ull);
+                                        if (nvox > C.paths.capacity) n_overflow++;
+                                    }
+                                }
                             }
                             atomicAdd(ledger + kEsc, (double)w);
                             alive = false;
@@ -733,18 +802,22 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                             load_layer(nxt);
                             float o2 = offset_scale(hit);
                             if (dot(dir, nrm) < 0.0f) o2 = -o2;
+                            V3 before = pos;
                             pos.x = hit.x + o2 * nrm.x;
                             pos.y = hit.y + o2 * nrm.y;
                             pos.z = hit.z + o2 * nrm.z;
+                            if constexpr (PATHS) path_crossings(C, before, pos, w, vox, nvox);
                             rec.put(XMC_EV_TRANSMIT, layer, surface, draws, pos, dir, w, tim);
                         }
                     } else {
                         dir = reflect(dir, nrm);
                         float o2 = offset_scale(hit);
                         if (dot(dir, nrm) < 0.0f) o2 = -o2;
+                        V3 before = pos;
                         pos.x = hit.x + o2 * nrm.x;
                         pos.y = hit.y + o2 * nrm.y;
                         pos.z = hit.z + o2 * nrm.z;
+                        if constexpr (PATHS) path_crossings(C, before, pos, w, vox, nvox);
                         rec.put(XMC_EV_REFLECT, layer, surface, draws, pos, dir, w, tim);
                     }
                 }
@@ -761,9 +834,11 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             tim += step * Ln;
             if (Ltarget) trange += step;
             else orange += step;
+            V3 prev = pos;
             pos.x += step * dir.x;
             pos.y += step * dir.y;
             pos.z += step * dir.z;
+            if constexpr (PATHS) path_crossings(C, prev, pos, w, vox, nvox);
             int idx = area_index(C.area, pos);
             if (idx >= 0) {
                 float dw = w * Lmua / Lmut;  // ComputeDroppedWeightOfPhoton (:633-642)
@@ -819,14 +894,15 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     if (n_roulette) atomicAdd(P.tally.stats + 3, n_roulette);
     if (st.spills) atomicAdd(P.tally.stats + 4, (unsigned long long)st.spills);
     if (n_culled) atomicAdd(P.tally.stats + 5, n_culled);
+    if (n_overflow) atomicAdd(P.tally.stats + 6, n_overflow);
     atomicAdd(ledger + kAbs, w_abs);
     __syncthreads();
     if (tid < kLedger) unsafeAtomicAdd(reinterpret_cast<double*>(P.tally.stats + 8) + tid, ledger[tid]);
 }
 
-template <bool TRACE>
+template <bool TRACE, bool PATHS>
 __global__ void __launch_bounds__(256) transport_kernel(Params P) {
-    transport_body<TRACE, false>(P, nullptr, nullptr, 0);
+    transport_body<TRACE, false, PATHS>(P, nullptr, nullptr, 0);
 }
 
 // Many configurations in one launch: P carries the run-wide fields and the
@@ -834,11 +910,13 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
 // tallies of configuration c.
 __global__ void __launch_bounds__(256) batch_kernel(Params P, const Params* __restrict__ cfgs,
                                                     const uint64_t* __restrict__ prefix, int ncfg) {
-    transport_body<false, true>(P, cfgs, prefix, ncfg);
+    transport_body<false, true, false>(P, cfgs, prefix, ncfg);
 }
 
-template __global__ void transport_kernel<false>(Params);
-template __global__ void transport_kernel<true>(Params);
+template __global__ void transport_kernel<false, false>(Params);
+template __global__ void transport_kernel<true, false>(Params);
+template __global__ void transport_kernel<false, true>(Params);
+template __global__ void transport_kernel<true, true>(Params);
 
 // One ray per thread: nearest hit through the BVH or over every triangle.
 __global__ void __launch_bounds__(256) query_kernel(QueryParams Q, const int* perm) {
@@ -901,10 +979,15 @@ __global__ void zero_kernel(unsigned long long* p, size_t n) {
 extern "C" {
 
 hipError_t xmc_launch_transport(const xmc::Params* p, int grid, int block, size_t lds, hipStream_t s, int trace) {
-    if (trace)
-        hipLaunchKernelGGL(xmc::transport_kernel<true>, dim3(grid), dim3(block), lds, s, *p);
+    const bool paths = p->paths.scattering != nullptr || p->paths.det_traj != nullptr;
+    if (trace && paths)
+        hipLaunchKernelGGL((xmc::transport_kernel<true, true>), dim3(grid), dim3(block), lds, s, *p);
+    else if (trace)
+        hipLaunchKernelGGL((xmc::transport_kernel<true, false>), dim3(grid), dim3(block), lds, s, *p);
+    else if (paths)
+        hipLaunchKernelGGL((xmc::transport_kernel<false, true>), dim3(grid), dim3(block), lds, s, *p);
     else
-        hipLaunchKernelGGL(xmc::transport_kernel<false>, dim3(grid), dim3(block), lds, s, *p);
+        hipLaunchKernelGGL((xmc::transport_kernel<false, false>), dim3(grid), dim3(block), lds, s, *p);
     return hipGetLastError();
 }
 
@@ -924,7 +1007,8 @@ hipError_t xmc_launch_zero(unsigned long long* p, size_t n, hipStream_t s) {
 }
 
 hipError_t xmc_transport_occupancy(int block, size_t lds, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, xmc::transport_kernel<false>, block, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, xmc::transport_kernel<false, false>, block,
+                                                        lds);
 }
 
 hipError_t xmc_launch_batch(const xmc::Params* p, const xmc::Params* cfgs, const uint64_t* prefix, int ncfg, int grid,

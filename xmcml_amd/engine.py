@@ -35,7 +35,7 @@ EXPORTS = [
     # surrogate model (include/xmcml_surrogate.h)
     "xmc_mlp_create", "xmc_mlp_destroy", "xmc_mlp_param_count", "xmc_mlp_get_params", "xmc_mlp_set_params",
     "xmc_mlp_forward", "xmc_mlp_backward", "xmc_mlp_get_grads", "xmc_mlp_adam", "xmc_mlp_train_step",
-    "xmc_mlp_gemm_check",
+    "xmc_mlp_gemm_check", "xmc_mlp_train_sampled", "xmc_dev_alloc", "xmc_dev_free", "xmc_dev_copy", "xmc_dev_sync",
 ]
 
 
@@ -96,6 +96,12 @@ def lib():
     sig("xmc_mlp_adam", _I, [_P, _F, _F, _F, _F, _F, _P])
     sig("xmc_mlp_train_step", _I, [_P, _P, _P, _P, _I, _F, _F, _F, _F, _F, _F, _P])
     sig("xmc_mlp_gemm_check", _I, [_PF, _PF, _I, _I, _I, _PF, _I])
+    sig("xmc_mlp_train_sampled", _I, [_P, _P, _P, _P, C.c_int64, _I, _U64, C.c_int64, _F, _F, _F, _F, _F, _F,
+                                      _PF, _P])
+    sig("xmc_dev_alloc", _I, [_I, _U64, C.POINTER(_P)])
+    sig("xmc_dev_free", _I, [_P])
+    sig("xmc_dev_copy", _I, [_P, _P, _U64, _I])
+    sig("xmc_dev_sync", _I, [_I])
     _lib = L
     return L
 
@@ -118,10 +124,13 @@ class Scene:
     """Geometry, optics and device tallies of one input, resident on the GPU."""
 
     def __init__(self, spec: SceneSpec, block_size: int = 0, blocks_per_cu: int = 0,
-                 absorption: bool = True, device: int = 0, distance_grid: bool = True):
+                 absorption: bool = True, device: int = 0, distance_grid: bool = True,
+                 scattering: bool = False, trajectories: bool = False, trajectory_capacity: int = 0):
         self.spec = spec
         self.input = InputHandle(spec)
-        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1, 0 if distance_grid else 1)
+        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1, 0 if distance_grid else 1,
+                      int(scattering), int(trajectories), trajectory_capacity)
+        self.scattering, self.trajectories = scattering, trajectories
         h = _P()
         _check(lib().xmc_scene_create(self.input.ptr, C.byref(opt), C.byref(h)), "xmc_scene_create")
         self._h = h
@@ -152,7 +161,8 @@ class Scene:
 
     def collect(self, out: Optional[OutputHandle] = None) -> OutputHandle:
         if out is None:
-            out = OutputHandle(self.spec, absorption=self.absorption)
+            out = OutputHandle(self.spec, absorption=self.absorption, scattering=self.scattering,
+                               trajectory=self.trajectories)
         _check(lib().xmc_scene_collect(self._h, out.ptr), "xmc_scene_collect")
         return out
 
@@ -204,6 +214,51 @@ class Scene:
         return dict(root=root.value, boxes=boxes, links=links, tsurf=ts, tidx=ti)
 
 
+class DeviceArray:
+    """A float32 array in device memory, allocated by the engine library
+    (xmc_dev_alloc); upload() / numpy() copy through host memory."""
+
+    def __init__(self, shape, data=None, device: int = 0):
+        self.shape = tuple(int(v) for v in shape)
+        self.device = device
+        self.nbytes = 4 * int(np.prod(self.shape)) if self.shape else 4
+        p = _P()
+        _check(lib().xmc_dev_alloc(device, self.nbytes, C.byref(p)), "xmc_dev_alloc")
+        self._p = p
+        if data is not None:
+            self.upload(data)
+
+    @property
+    def ptr(self) -> _P:
+        return self._p
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.shape != self.shape:
+            raise ValueError(f"shape {a.shape} != {self.shape}")
+        _check(lib().xmc_dev_copy(self._p, a.ctypes.data, self.nbytes, 1), "xmc_dev_copy")
+
+    def numpy(self) -> np.ndarray:
+        a = np.empty(self.shape, np.float32)
+        _check(lib().xmc_dev_copy(a.ctypes.data, self._p, self.nbytes, 2), "xmc_dev_copy")
+        return a
+
+    def close(self):
+        if getattr(self, "_p", None):
+            lib().xmc_dev_free(self._p)
+            self._p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_sync(device: int = 0):
+    _check(lib().xmc_dev_sync(device), "xmc_dev_sync")
+
+
 def distance_grid(spec: SceneSpec, target_cells: int = 0) -> dict:
     """The conservative distance grid a scene builds (host computation)."""
     inp = InputHandle(spec)
@@ -224,7 +279,7 @@ def launch(spec: SceneSpec, out: OutputHandle, seed: int, first: int, n: int, ab
     Returns the next photon index of the stream."""
     inp = InputHandle(spec, photons=n)
     rng = Rng(seed, first)
-    opt = Options(0, 0, 0, 0 if absorption else 1, 0)
+    opt = Options(0, 0, 0, 0 if absorption else 1, 0, 0, 0, 0)
     _check(lib().xmc_launch(inp.ptr, out.ptr, C.byref(rng), C.byref(opt)), "xmc_launch")
     return rng.nextPhoton
 
@@ -240,7 +295,7 @@ class Batch:
         self.inputs = [InputHandle(s) for s in self.specs]
         arr = (Input * len(self.inputs))(*[h.c for h in self.inputs])
         self._arr = arr
-        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1, 0)
+        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1, 0, 0, 0, 0)
         h = _P()
         _check(lib().xmc_batch_create(arr, len(self.inputs), C.byref(opt), C.byref(h)), "xmc_batch_create")
         self._h = h

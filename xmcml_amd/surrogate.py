@@ -130,7 +130,7 @@ def simulate(configs: Sequence[Sequence[Layer]], photons, seed: int, chunk: int 
 
 class MLP:
     """The matrix-core MLP of include/xmcml_surrogate.h.  Device arrays are
-    torch tensors (float32, contiguous, on the model's device)."""
+    engine.DeviceArray (float32, row-major)."""
 
     def __init__(self, inputs: int = N_INPUTS, hidden: int = 512, depth: int = 4, outputs: int = N_OUTPUTS,
                  max_batch: int = 4096, seed: int = 0, device: int = 0):
@@ -178,18 +178,17 @@ class MLP:
                                                       self.nparam), "xmc_mlp_set_params")
 
     @staticmethod
-    def _dev(t, cols, name):
-        import torch
-        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
-            raise TypeError(f"{name} must be a contiguous float32 CUDA tensor")
-        if t.dim() != 2 or t.shape[1] != cols:
+    def _dev(a, cols, name):
+        if not isinstance(a, engine.DeviceArray):
+            raise TypeError(f"{name} must be an engine.DeviceArray")
+        if len(a.shape) != 2 or a.shape[1] != cols:
             raise ValueError(f"{name} must have shape (batch, {cols})")
-        return C.c_void_p(t.data_ptr())
+        return a.ptr
 
     def forward(self, x, out=None):
-        import torch
+        """Head output (n, 2*outputs) of device rows x; returns the device array."""
         if out is None:
-            out = torch.empty((x.shape[0], 2 * self.outputs), dtype=torch.float32, device=x.device)
+            out = engine.DeviceArray((x.shape[0], 2 * self.outputs), device=self.device)
         engine._check(engine.lib().xmc_mlp_forward(self._h, self._dev(x, self.inputs, "x"), x.shape[0],
                                                    self._dev(out, 2 * self.outputs, "out"), None),
                       "xmc_mlp_forward")
@@ -211,6 +210,15 @@ class MLP:
             self._h, self._dev(x, self.inputs, "x"), self._dev(y, self.outputs, "y"),
             self._dev(mask, self.outputs, "mask"), x.shape[0], scale, lr, beta1, beta2, eps, weight_decay, None),
             "xmc_mlp_train_step")
+
+    def train_sampled(self, X, Y, M, batch, seed, step, scale, lr, beta1=0.9, beta2=0.999, eps=1e-8,
+                      weight_decay=0.0, want_loss=False) -> Optional[float]:
+        loss = C.c_float()
+        engine._check(engine.lib().xmc_mlp_train_sampled(
+            self._h, self._dev(X, self.inputs, "X"), self._dev(Y, self.outputs, "Y"),
+            self._dev(M, self.outputs, "M"), X.shape[0], batch, seed, step, scale, lr, beta1, beta2, eps,
+            weight_decay, C.byref(loss) if want_loss else None, None), "xmc_mlp_train_sampled")
+        return loss.value if want_loss else None
 
 
 def gemm_check(a: np.ndarray, b: np.ndarray, device: int = 0) -> np.ndarray:
@@ -256,49 +264,51 @@ class Surrogate:
     def fit(cls, X: np.ndarray, Y: np.ndarray, M: np.ndarray, steps: int = 2000, batch: int = 4096,
             lr: float = 2e-3, lr_final: float = 2e-5, weight_decay: float = 0.0, seed: int = 0,
             hidden: int = 512, depth: int = 4, log_every: int = 0, device: int = 0) -> Tuple["Surrogate", FitLog]:
-        import torch
-        dev = torch.device("cuda", device)
+        """AdamW with a cosine learning rate from lr to lr_final; each step
+        draws `batch` rows of the resident data set on the device."""
         mean, std = cls.standardisation(Y, M)
         Ys = np.where(M > 0, (Y - mean) / std, 0.0).astype(np.float32)
-        Xt = torch.from_numpy(np.ascontiguousarray(X, np.float32)).to(dev)
-        Yt = torch.from_numpy(Ys).to(dev)
-        Mt = torch.from_numpy(np.ascontiguousarray(M, np.float32)).to(dev)
         n = X.shape[0]
         batch = min(batch, n)
+        Xd = engine.DeviceArray(X.shape, X, device)
+        Yd = engine.DeviceArray(Ys.shape, Ys, device)
+        Md = engine.DeviceArray(M.shape, M, device)
         mlp = MLP(X.shape[1], hidden, depth, Y.shape[1], batch, seed=seed, device=device)
-        g = torch.Generator(device=dev)
-        g.manual_seed(seed)
         scale = 1.0 / (batch * Y.shape[1])
         t0 = time.perf_counter()
         first = last = float("nan")
         for step in range(steps):
             frac = step / max(1, steps - 1)
             lr_t = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * frac))
-            idx = torch.randint(0, n, (batch,), device=dev, generator=g)
-            xb, yb, mb = Xt[idx].contiguous(), Yt[idx].contiguous(), Mt[idx].contiguous()
-            if step == 0 or step == steps - 1 or (log_every and step % log_every == 0):
-                loss = mlp.backward(xb, yb, mb, scale, want_loss=True)
-                mlp.adam(lr_t, weight_decay=weight_decay)
+            want = step == 0 or step == steps - 1 or bool(log_every and step % log_every == 0)
+            loss = mlp.train_sampled(Xd, Yd, Md, batch, seed, step, scale, lr_t, weight_decay=weight_decay,
+                                     want_loss=want)
+            if want:
                 if step == 0:
                     first = loss
                 last = loss
                 if log_every:
                     print(f"surrogate fit: step {step} loss {loss:.5f}", flush=True)
-            else:
-                mlp.train_step(xb, yb, mb, scale, lr_t, weight_decay=weight_decay)
-        torch.cuda.synchronize(dev)
-        return cls(mlp, mean, std), FitLog(steps, time.perf_counter() - t0, first, last)
+        engine.device_sync(device)
+        seconds = time.perf_counter() - t0
+        for a in (Xd, Yd, Md):
+            a.close()
+        return cls(mlp, mean, std), FitLog(steps, seconds, first, last)
 
     # ---- inference ------------------------------------------------------
-    def predict_device(self, x):
+    def predict_device(self, x, out=None):
         """Raw head output for feature rows already on the device."""
-        return self.mlp.forward(x)
+        return self.mlp.forward(x, out)
 
     def predict(self, X: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
         """(ln-value means, ln-value standard deviations), each (n, 96)."""
-        import torch
-        x = torch.from_numpy(np.ascontiguousarray(X, np.float32)).to(torch.device("cuda", self.mlp.device))
-        out = self.mlp.forward(x).cpu().numpy()
+        if len(X) == 0:
+            return np.zeros((0, self.y_mean.size)), np.zeros((0, self.y_mean.size))
+        x = engine.DeviceArray(X.shape, X, self.mlp.device)
+        o = self.mlp.forward(x)
+        out = o.numpy()
+        x.close()
+        o.close()
         O = self.y_mean.size
         mu = out[:, :O] * self.y_std + self.y_mean
         sig = np.exp(0.5 * np.clip(out[:, O:], -10, 10)) * self.y_std
