@@ -26,6 +26,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 for p in (ROOT, ROOT / "tests"):
     if str(p) not in sys.path:
@@ -100,6 +102,73 @@ def copy_calibration(nbytes: int) -> None:
         hip.hipFree(dst)
 
 
+def surrogate_line(model_path: Path, queries: int, mc_configs: int, mc_photons: int) -> dict:
+    """The learned half of the metric: configurations/s of the surrogate's
+    inference kernels (features resident on the device) and its median
+    ring-reflectance error against 1e7-photon Monte Carlo on held-out
+    configurations, next to plain batched Monte Carlo's configurations/s at
+    mc_photons photons each, measured now on the same GPU."""
+    from xmcml_amd import engine
+    from xmcml_amd import surrogate as S
+    from xmcml_amd.train_surrogate import configs_from_array, evaluate
+
+    z = np.load(model_path, allow_pickle=False)
+    sur = S.Surrogate.load(model_path, max_batch=65536)
+    cfg = configs_from_array(z["eval_layers"], z["eval_count"])
+    Xe = S.encode_many(cfg)
+    ev = evaluate(sur, Xe, z["Wref"], z["Kref"], int(sur.meta.get("min_count", 1000)))
+    unc = ev.pop("uncertainty")
+    flagged = unc > sur.sigma_max
+    # inference throughput on a resident batch of `queries` feature rows
+    Xq = np.tile(Xe, (queries // len(Xe) + 1, 1))[:queries]
+    xq = engine.DeviceArray(Xq.shape, Xq)
+    out = sur.predict_device(xq)
+    engine.device_sync()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sur.predict_device(xq, out)
+    engine.device_sync()
+    t_inf = (time.perf_counter() - t0) / reps
+    # end to end from host configurations (encode, copy, infer, copy back, fallback check)
+    t0 = time.perf_counter()
+    mu, sig = sur.predict(S.encode_many(cfg))
+    _ = sur.uncertainty(sig) > sur.sigma_max
+    t_e2e = (time.perf_counter() - t0) / len(cfg)
+    # plain batched Monte Carlo on held-out configurations
+    t0 = time.perf_counter()
+    S.simulate(cfg[:mc_configs], mc_photons, seed=99, chunk=mc_configs)
+    t_mc = time.perf_counter() - t0
+    mc_rate = mc_configs / t_mc
+    inf_rate = queries / t_inf
+    h, d = sur.mlp.hidden, sur.mlp.depth
+    fl = 2.0 * (S.N_INPUTS * h + (d - 1) * h * h + h * 2 * S.N_OUTPUTS)  # algorithmic FLOP per configuration
+    return {
+        "configs_per_s": inf_rate,
+        "unit": "configs/s",
+        "median_ring_rel_err": ev["ring_median_rel_err"],
+        "p90_ring_rel_err": ev["ring_p90_rel_err"],
+        "tof_median_rel_err": ev["tof_median_rel_err"],
+        "judged_against": f"{len(cfg)} held-out configurations, {sur.meta.get('eval_photons', 10_000_000):.0e}-photon "
+                          f"MC, rings with >= {sur.meta.get('min_count', 1000)} detected packets "
+                          f"({ev['ring_entries']} entries)",
+        "flagged_for_mc_fraction": float(flagged.mean()),
+        "median_ring_rel_err_unflagged": float(np.median(S.ring_relative_error(
+            np.exp(mu[~flagged, :S.RINGS]), z["Wref"][~flagged, :S.RINGS],
+            z["Kref"][~flagged, :S.RINGS] >= int(sur.meta.get("min_count", 1000))))) if (~flagged).any() else None,
+        "queries_resident": queries,
+        "inference_ms": t_inf * 1e3,
+        "inference_tflops": fl * queries / t_inf / 1e12,
+        "inference_frac_of_bf16_peak": fl * queries / t_inf / 2.5e15,
+        "end_to_end_configs_per_s_host": 1.0 / t_e2e,
+        "plain_mc_configs_per_s": mc_rate,
+        "plain_mc_photons_per_config": mc_photons,
+        "speedup_vs_plain_mc": inf_rate / mc_rate,
+        "trained_on": f"{sur.meta.get('train_configs')} configurations x {sur.meta.get('train_photons')} photons "
+                      f"of batched GPU MC",
+    }
+
+
 def traffic_from_profiles():
     """HBM bytes per transport launch from the committed rocprofv3 PMC
     summary, when one exists."""
@@ -118,9 +187,16 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=int, default=10_000_000, help="photons per step")
+    ap.add_argument("--workload", choices=["heightfield", "head"], default="heightfield",
+                    help="heightfield: the 3-D surface configuration (the metric's workload); head: five nested "
+                         "noisy ellipsoids of 327,680 triangles each")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--seed", type=int, default=777)
+    ap.add_argument("--surrogate-model", type=Path, default=ROOT / "xmcml_amd" / "data" / "surrogate_layered.npz")
+    ap.add_argument("--surrogate-queries", type=int, default=1 << 20)
+    ap.add_argument("--mc-configs", type=int, default=64)
+    ap.add_argument("--mc-photons", type=int, default=1_000_000)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="before the steps, run one device-to-device copy of this many MiB (a known-size "
                          "kernel that scales the HBM counters of a profiler pass)")
@@ -133,7 +209,9 @@ def main(argv=None) -> int:
 
     from xmcml_amd import engine, scenes
 
-    spec = scenes.heightfield_scene(args.photons)
+    make = (lambda n: scenes.heightfield_scene(n)) if args.workload == "heightfield" else \
+        (lambda n: scenes.head_scene(n))
+    spec = make(args.photons)
     print(f"bench: building scene ({spec.num_triangles} triangles)", file=sys.stderr, flush=True)
     sc = engine.Scene(spec, absorption=False, block_size=args.block)
     if args.copy_calibration_mib > 0:
@@ -192,9 +270,12 @@ def main(argv=None) -> int:
         "dtype": "fp32 transport, fp64 tallies",
         "data": "synthetic",
         "config": {
-            "workload": "3-D surface MC: 3 conformal heightfield boundaries (128x128 grid, 40x40 mm) + flat "
-                        "bottom, 98,306 triangles; 3 layers; pencil beam; roulette 1e-4/0.1; 20 rings + "
-                        "256-bin TOF 0-2 ns",
+            "workload": ("3-D surface MC: 3 conformal heightfield boundaries (128x128 grid, 40x40 mm) + flat "
+                         "bottom, 98,306 triangles; 3 layers; pencil beam; roulette 1e-4/0.1; 20 rings + "
+                         "256-bin TOF 0-2 ns") if args.workload == "heightfield" else
+                        ("heavy geometry: 5 nested ellipsoids (90/75/65 mm outer, shells 5/7/2/3 mm), "
+                         "icosphere level 7 with 1 mm seeded radial noise, 1,638,400 triangles; scalp/skull/"
+                         "CSF/grey/white; pencil beam; roulette 1e-4/0.1; rings at 10-40 mm + 256-bin TOF 0-5 ns"),
             "photons_per_step": args.photons,
             "triangles": spec.num_triangles,
             "parallelism": "single GPU, persistent kernel",
@@ -206,12 +287,18 @@ def main(argv=None) -> int:
     sc.close()
     if args.cpu_seconds > 0:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
-        cb = cpu_baseline(scenes.heightfield_scene(0), args.cpu_seconds, args.seed + 1)
+        cb = cpu_baseline(make(0), args.cpu_seconds, args.seed + 1)
         line["cpu_baseline"] = cb
         line["speedup_vs_cpu"] = value / cb["value"]
     else:
         line["cpu_baseline"] = None
     line["roofline"] = roofline
+    if args.surrogate_model and args.surrogate_model.exists() and args.surrogate_queries > 0:
+        print("bench: surrogate inference and plain batched MC", file=sys.stderr, flush=True)
+        line["surrogate"] = surrogate_line(args.surrogate_model, args.surrogate_queries, args.mc_configs,
+                                           args.mc_photons)
+    else:
+        line["surrogate"] = None
     print(json.dumps(line), flush=True)
     return 0
 

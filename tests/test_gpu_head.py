@@ -1,0 +1,55 @@
+"""The heavy-geometry workload on the GPU: five nested noisy ellipsoids.
+Bit-identical tallies against the float oracle at a reduced mesh level, the
+full 1,638,400-triangle mesh against the float oracle on a photon sample,
+and the weight ledger and detector fall-off at a larger count."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from xmcml_amd import scenes
+from xmcml_amd.abi import InputHandle, OutputHandle
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(engine, spec, n, seed):
+    inp = InputHandle(spec)
+    with engine.Scene(spec, absorption=False) as sc:
+        sc.run(seed, 0, n)
+        g = sc.collect()
+        st = sc.stats()
+    o = OutputHandle(spec, absorption=False)
+    c = ol.run_oracle(spec, inp, o, seed, 0, n, 32, 8)
+    assert st["photons"] == n
+    assert st["scatterEvents"] == c.scatterEvents
+    assert st["boundaryEvents"] == c.boundaryEvents
+    assert np.array_equal(g.det_count(), o.det_count())
+    assert np.array_equal(g.bin_count(), o.bin_count())
+    np.testing.assert_allclose(g.det_weight(), o.det_weight(), rtol=1e-11, atol=1e-15)
+    return st
+
+
+def test_head_reduced_mesh_matches_float_oracle(engine):
+    _compare(engine, scenes.head_scene(0, level=4), 3000, 21)
+
+
+def test_head_full_mesh_matches_float_oracle(engine):
+    spec = scenes.head_scene(0, level=7)
+    assert spec.num_triangles == 1_638_400
+    st = _compare(engine, spec, 1500, 22)
+    assert st["boundaryEvents"] > 0
+
+
+def test_head_full_mesh_ledger_and_falloff(engine):
+    n = 200_000
+    spec = scenes.head_scene(n, level=7)
+    with engine.Scene(spec, absorption=False) as sc:
+        sc.run(23, 0, n)
+        out = sc.collect()
+        st = sc.stats()
+    total = (out.specular + (st["wAbsorbed"] + st["wEscaped"] + st["wLeftArea"] + st["wKilled"]
+                             + st["wRouletteLost"] + st["wOther"] - st["wRouletteGain"]) / n)
+    assert total == pytest.approx(1.0, abs=1e-5)
+    assert st["wLeftArea"] == 0.0 and st["wOther"] == 0.0
+    w = out.det_weight() / n
+    assert np.all(w > 0) and np.all(np.diff(w) < 0)  # reflectance falls with source distance

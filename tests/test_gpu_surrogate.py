@@ -28,7 +28,7 @@ def dev(engine):
 def test_gemm_matches_reference(dev):
     from xmcml_amd.surrogate import gemm_check
     rng = np.random.default_rng(0)
-    for (M, N, K) in [(64, 64, 32), (128, 192, 96), (256, 64, 512)]:
+    for (M, N, K) in [(64, 64, 64), (128, 192, 128), (256, 64, 512)]:
         a = rng.standard_normal((M, K)).astype(np.float32)
         b = rng.standard_normal((N, K)).astype(np.float32)
         c = gemm_check(a, b)

@@ -34,8 +34,8 @@ typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
 typedef float f32x16 __attribute__((__vector_size__(16 * sizeof(float))));
 typedef uint32_t u32x4 __attribute__((__vector_size__(16)));
 
-constexpr int BM = 64, BN = 64, BK = 32;
-constexpr int LDS_STRIDE = BK + 8;  // halfwords per LDS tile row (80 bytes: 16-byte aligned rows)
+constexpr int BM = 64, BN = 64, BK = 64;
+constexpr int LDS_STRIDE = BK + 8;  // halfwords per LDS tile row (144 bytes: 16-byte aligned rows)
 constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2/pi)
 constexpr float kGeluA = 0.044715f;
 constexpr float kLogVarClamp = 10.0f;
@@ -82,28 +82,42 @@ struct GemmArgs {
     float* outf;         // EPI_LINEAR: [m*ldo + n]; EPI_PARTIAL: outf + z*M*ldo + m*ldo + n
 };
 
+// A 64 x 64 (rows x k) operand tile is 512 16-byte pieces: two per thread.
+// AK: piece = (row, 8 consecutive k); otherwise piece = (k, 8 consecutive
+// rows), transposed on the LDS store.
 template <bool AK>
-__device__ inline void load_tile(uint16_t* lds, const uint16_t* src, long ld, int r0, int k0, int t) {
-    if constexpr (AK) {
-        // 64 rows x 32 k, contiguous in k: one 16-byte read per thread
-        int r = t >> 2, c = (t & 3) * 8;
-        u32x4 v = *(const u32x4*)(src + (long)(r0 + r) * ld + k0 + c);
-        *(u32x4*)(lds + r * LDS_STRIDE + c) = v;
-    } else {
-        // 32 k x 64 rows, contiguous in the row index: read along rows, transpose into LDS
-        int kr = t >> 3, c = (t & 7) * 8;
-        u32x4 v = *(const u32x4*)(src + (long)(k0 + kr) * ld + r0 + c);
-        uint16_t h[8];
-        std::memcpy(h, &v, 16);
+__device__ inline void load_pieces(u32x4 (&v)[2], const uint16_t* src, long ld, int r0, int k0, int t) {
 #pragma unroll
-        for (int i = 0; i < 8; i++) lds[(c + i) * LDS_STRIDE + kr] = h[i];
+    for (int j = 0; j < 2; j++) {
+        int p = t + 256 * j;
+        int a = p >> 3, c = (p & 7) * 8;
+        v[j] = AK ? *(const u32x4*)(src + (long)(r0 + a) * ld + k0 + c)
+                  : *(const u32x4*)(src + (long)(k0 + a) * ld + r0 + c);
+    }
+}
+
+template <bool AK>
+__device__ inline void store_pieces(uint16_t* lds, const u32x4 (&v)[2], int t) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        int p = t + 256 * j;
+        int a = p >> 3, c = (p & 7) * 8;
+        if constexpr (AK) {
+            *(u32x4*)(lds + a * LDS_STRIDE + c) = v[j];
+        } else {
+            uint16_t h[8];
+            std::memcpy(h, &v[j], 16);
+#pragma unroll
+            for (int i = 0; i < 8; i++) lds[(c + i) * LDS_STRIDE + a] = h[i];
+        }
     }
 }
 
 template <bool AK, bool BKC, int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[BM * LDS_STRIDE];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[BN * LDS_STRIDE];
+    // two LDS stages: the tile being multiplied and the next one being stored
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][BM * LDS_STRIDE];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN * LDS_STRIDE];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
     const int kb = blockIdx.z * g.kchunk, ke = kb + g.kchunk;
@@ -113,19 +127,35 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
 #pragma unroll
     for (int v = 0; v < 16; v++) acc[v] = 0.0f;
 
+    u32x4 ra[2], rb[2];
+    load_pieces<AK>(ra, g.A, g.lda, m0, kb, t);
+    load_pieces<BKC>(rb, g.B, g.ldb, n0, kb, t);
+    store_pieces<AK>(As[0], ra, t);
+    store_pieces<BKC>(Bs[0], rb, t);
+    __syncthreads();
+    int stage = 0;
     for (int k0 = kb; k0 < ke; k0 += BK) {
-        load_tile<AK>(As, g.A, g.lda, m0, k0, t);
-        load_tile<BKC>(Bs, g.B, g.ldb, n0, k0, t);
-        __syncthreads();
+        const bool more = k0 + BK < ke;
+        if (more) {  // next tile's global reads in flight during this tile's products
+            load_pieces<AK>(ra, g.A, g.lda, m0, k0 + BK, t);
+            load_pieces<BKC>(rb, g.B, g.ldb, n0, k0 + BK, t);
+        }
+        const uint16_t* as = As[stage];
+        const uint16_t* bs = Bs[stage];
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             // lane holds row (lane % 32), k = kk + 8*(lane / 32) .. +7 of each operand
-            u32x4 av = *(const u32x4*)(As + (wm + lr) * LDS_STRIDE + kk + lk);
-            u32x4 bv = *(const u32x4*)(Bs + (wn + lr) * LDS_STRIDE + kk + lk);
+            u32x4 av = *(const u32x4*)(as + (wm + lr) * LDS_STRIDE + kk + lk);
+            u32x4 bv = *(const u32x4*)(bs + (wn + lr) * LDS_STRIDE + kk + lk);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av),
                                                           __builtin_bit_cast(bf16x8, bv), acc, 0, 0, 0);
         }
+        if (more) {
+            store_pieces<AK>(As[stage ^ 1], ra, t);
+            store_pieces<BKC>(Bs[stage ^ 1], rb, t);
+        }
         __syncthreads();
+        stage ^= 1;
     }
 
     // accumulator element v of a lane: row 8*(v/4) + 4*(lane/32) + v%4, column lane%32
@@ -192,13 +222,23 @@ __global__ void loss_kernel(const float* out, const float* y, const float* mask,
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// dst[n] = sum over rows of src[r][n] (bf16 in, fp32 out), fixed order
-__global__ void colsum_kernel(const uint16_t* src, int rows, int cols, float* dst) {
-    int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= cols) return;
+// part[z][n] = sum of src[r][n] (bf16 in, fp32 out) over the rows of slice
+// z = blockIdx.y, in a fixed order: 64 columns per block, four row lanes
+// (coalesced 128-byte rows), then a fixed LDS combine
+constexpr int kColsumSlices = 32;
+__global__ void colsum_kernel(const uint16_t* src, int rows, int cols, float* part) {
+    __shared__ float red[4][64];
+    const int c = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + c;
+    const int per = (rows + gridDim.y - 1) / gridDim.y;
+    const int r0 = blockIdx.y * per, r1 = min(rows, r0 + per);
     float s = 0.0f;
-    for (int r = 0; r < rows; r++) s += bf2f(src[(long)r * cols + n]);
-    dst[n] = s;
+    if (n < cols)
+        for (int r = r0 + ty; r < r1; r += 4) s += bf2f(src[(long)r * cols + n]);
+    red[ty][c] = s;
+    __syncthreads();
+    if (ty == 0 && n < cols)
+        part[(long)blockIdx.y * cols + n] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
 }
 
 // g[i] = sum_z p[z*count + i], fixed order
@@ -232,17 +272,18 @@ __device__ inline uint64_t splitmix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-// batch row r <- data row splitmix64(key + r) mod rows
+// batch row r <- data row splitmix64(key + r) mod rows; one thread per
+// copied value, so consecutive lanes move consecutive columns of a row
 __global__ void gather_kernel(const float* X, const float* Y, const float* M, long rows, int nin, int nout,
                               int batch, uint64_t key, float* xb, float* yb, float* mb) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= batch) return;
+    const int width = nin + 2 * nout;
+    long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)batch * width) return;
+    int r = (int)(e / width), c = (int)(e % width);
     long i = (long)(splitmix64(key + (uint64_t)r) % (uint64_t)rows);
-    for (int c = 0; c < nin; c++) xb[(long)r * nin + c] = X[i * nin + c];
-    for (int c = 0; c < nout; c++) {
-        yb[(long)r * nout + c] = Y[i * nout + c];
-        mb[(long)r * nout + c] = M[i * nout + c];
-    }
+    if (c < nin) xb[(long)r * nin + c] = X[i * nin + c];
+    else if (c < nin + nout) yb[(long)r * nout + (c - nin)] = Y[i * nout + (c - nin)];
+    else mb[(long)r * nout + (c - nin - nout)] = M[i * nout + (c - nin - nout)];
 }
 
 __global__ void to_bf16_kernel(const float* src, uint16_t* dst, long count) {
@@ -279,6 +320,9 @@ int split_count(int k) {
         if (k % (s * BK) == 0 && k / s >= 256) return s;
     return 1;
 }
+
+// every reduction length handed to the kernel is a multiple of BK
+static_assert(BK == 64, "padded widths (input 64, head 2*outputs, hidden, batch rows) assume 64");
 
 }  // namespace
 
@@ -402,8 +446,12 @@ int weight_grad(xmc_mlp* m, int l, const uint16_t* dz, const uint16_t* in, int r
     hipLaunchKernelGGL(reduce_partials_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, m->part,
                        splits, cnt, m->g + m->woff[l]);
     MLP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(colsum_kernel, dim3((m->fout[l] + 255) / 256), dim3(256), 0, s, dz, rows, m->fout[l],
-                       m->g + m->boff[l]);
+    // bias gradient: column sums in slices (reusing the partial buffer), then their fixed-order sum
+    hipLaunchKernelGGL(colsum_kernel, dim3((m->fout[l] + 63) / 64, kColsumSlices), dim3(256), 0, s, dz, rows,
+                       m->fout[l], m->part);
+    MLP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3((m->fout[l] + 255) / 256), dim3(256), 0, s, m->part,
+                       kColsumSlices, (long)m->fout[l], m->g + m->boff[l]);
     MLP_TRY(hipGetLastError());
     return 0;
 }
@@ -687,7 +735,8 @@ int xmc_mlp_train_sampled(xmc_mlp* m, const float* X, const float* Y, const floa
     MLP_TRY(hipSetDevice(m->d.device));
     hipStream_t s = (hipStream_t)stream;
     uint64_t key = seed * 0x9e3779b97f4a7c15ull + (uint64_t)step * 0xd1b54a32d192ed03ull;
-    hipLaunchKernelGGL(gather_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, X, Y, M, (long)rows,
+    long elems = (long)batch * (m->d.inputs + 2 * m->d.outputs);
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)((elems + 255) / 256)), dim3(256), 0, s, X, Y, M, (long)rows,
                        m->d.inputs, m->d.outputs, batch, key, m->xb, m->yb, m->mb);
     MLP_TRY(hipGetLastError());
     int rc = backward_impl(m, m->xb, m->yb, m->mb, batch, scale, loss, s);
@@ -728,7 +777,7 @@ int xmc_dev_sync(int device) {
 int xmc_mlp_gemm_check(const float* a, const float* b, int M, int N, int K, float* c, int device) {
     if (!a || !b || !c) return fail(-1, "NULL argument");
     if (M <= 0 || N <= 0 || K <= 0 || M % BM || N % BN || K % BK)
-        return fail(-1, "M, N must be multiples of 64 and K of 32");
+        return fail(-1, "M, N and K must be multiples of 64");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(-4, "no GPU");
     MLP_TRY(hipSetDevice(device));

@@ -83,6 +83,128 @@ def heightfield(z0: float, height: SineHeight, cells: int = 128, half: float = 2
     return v, grid_triangles(cells)
 
 
+def icosphere(level: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere: the icosahedron with every face split into 4, `level`
+    times (20 * 4^level triangles, 10 * 4^level + 2 vertices)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], dtype=np.int64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(level):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        m = len(v) + inv.reshape(3, -1)  # midpoint of edge (0,1), (1,2), (2,0) of every face
+        v = np.concatenate([v, mid])
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        ab, bc, ca = m[0], m[1], m[2]
+        f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1), np.stack([c, ca, bc], 1),
+                            np.stack([ab, bc, ca], 1)])
+    return v, f.astype(np.int32)
+
+
+class SphereNoise:
+    """Smooth radial displacement on the unit sphere: a sum of `terms` seeded
+    plane waves sin(k . u + phi) with |k| ~ U[2, 6], scaled so that its
+    largest value over the mesh is `amplitude` mm."""
+
+    def __init__(self, seed: int, amplitude: float = 1.0, terms: int = 8):
+        rng = np.random.default_rng(seed)
+        d = rng.standard_normal((terms, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        self.k = d * rng.uniform(2.0, 6.0, (terms, 1))
+        self.phi = rng.uniform(0.0, 2 * math.pi, terms)
+        self.w = rng.uniform(0.5, 1.0, terms)
+        self.amplitude = amplitude
+
+    def raw(self, u):
+        return np.sin(u @ self.k.T + self.phi) @ self.w
+
+    def __call__(self, u):
+        r = self.raw(u)
+        return self.amplitude * r / np.max(np.abs(r))
+
+
+def ellipsoid(axes, unit_vertices, faces, noise: Optional[np.ndarray] = None):
+    """The ellipsoid x^2/a^2 + y^2/b^2 + z^2/c^2 = 1 through the icosphere's
+    directions, each vertex moved `noise` mm along its radial direction."""
+    p = unit_vertices * np.asarray(axes, dtype=np.float64)
+    if noise is not None:
+        r = np.linalg.norm(p, axis=1, keepdims=True)
+        p = p * (1.0 + noise[:, None] / r)
+    return p, faces
+
+
+def axis_hit_z(v: np.ndarray, f: np.ndarray) -> float:
+    """z of the lowest point where the z axis crosses the mesh (Moller-Trumbore
+    in double precision over every triangle)."""
+    a, b, c = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+    e1, e2 = b - a, c - a
+    d = np.array([0.0, 0.0, 1.0])
+    o = np.array([0.0, 0.0, -1e4])
+    pv = np.cross(d, e2)
+    det = np.einsum("ij,ij->i", e1, pv)
+    ok = np.abs(det) > 1e-15
+    inv = np.where(ok, 1.0 / np.where(ok, det, 1.0), 0.0)
+    tv = o - a
+    uu = np.einsum("ij,ij->i", tv, pv) * inv
+    qv = np.cross(tv, e1)
+    vv = (qv @ d) * inv
+    t = np.einsum("ij,ij->i", e2, qv) * inv
+    hit = ok & (uu >= 0) & (vv >= 0) & (uu + vv <= 1) & (t > 0)
+    if not hit.any():
+        raise ValueError("the z axis misses the mesh")
+    return float(o[2] + t[hit].min())
+
+
+# The heavy-geometry workload: outer semi-axes and the thickness of each
+# shell from the outside in (mm), and (n, mua, mus, g) of the five tissues.
+HEAD_AXES = (90.0, 75.0, 65.0)
+HEAD_SHELLS = [5.0, 7.0, 2.0, 3.0]
+HEAD_TISSUES = [(1.4, 0.02, 19.0, 0.9), (1.4, 0.01, 16.0, 0.9), (1.33, 0.004, 0.3, 0.9), (1.4, 0.02, 22.0, 0.9),
+                (1.4, 0.008, 91.0, 0.9)]
+
+
+def head_scene(photons: int = 1_000_000_000, level: int = 7, seed: int = 5, time_bins: int = 256,
+               time_ns: float = 5.0, detector_radii=(10.0, 20.0, 30.0, 40.0), ring_width: float = 2.0) -> SceneSpec:
+    """Five nested closed ellipsoids (scalp, skull, CSF, grey, white matter
+    from the outside in), each an icosphere of level `level` (327,680
+    triangles at level 7) with the same seeded 1 mm radial noise on every
+    shell, so the shells keep their thicknesses and never cross.  Pencil
+    beam entering at the bottom pole along +z; ring detectors of
+    `ring_width` mm centred at the given source distances, around the beam
+    axis; a time-of-flight histogram of time_bins bins over 0..time_ns."""
+    u, f = icosphere(level)
+    nz = SphereNoise(seed)(u)
+    surfaces = []
+    axes = np.array(HEAD_AXES)
+    for k in range(len(HEAD_SHELLS) + 1):
+        surfaces.append(ellipsoid(axes, u, f, nz))
+        if k < len(HEAD_SHELLS):
+            axes = axes - HEAD_SHELLS[k]
+    ns = len(surfaces)
+    layers = [LayerSpec(1.0, 0.0, 0.0, 0.0, [0])]
+    for i, (ri, mua, mus, g) in enumerate(HEAD_TISSUES):
+        ids = [i, i + 1] if i + 1 < ns else [i]
+        layers.append(LayerSpec(ri, mua, mus, g, ids))
+    # entry point: where the -z axis meets the outer mesh (on its flat triangle, not on the
+    # ideal ellipsoid, so that the launch offset puts the photon inside the scalp)
+    start_z = axis_hit_z(*surfaces[0])
+    half = [a + 3.0 for a in HEAD_AXES]
+    dets = [RingSpec(r - ring_width / 2, r + ring_width / 2, (0.0, 0.0, 0.0)) for r in detector_radii]
+    return SceneSpec(
+        layers=layers, surfaces=surfaces, area_corner=tuple(-h for h in half), area_length=tuple(2 * h for h in half),
+        area_partitions=(1, 1, 1), rings=dets, min_weight=1e-4, termination=TERMINATE_ROULETTE,
+        roulette_chance=0.1, time_start=0.0, time_finish=SPEED_OF_LIGHT_MM_PER_NS * time_ns, time_bins=time_bins,
+        start_position=(0.0, 0.0, start_z), photons=photons, name=f"head-5shell-l{level}")
+
+
 def rings(count: int, width: float, center=(0.0, 0.0, 0.0)) -> List[RingSpec]:
     return [RingSpec(i * width, (i + 1) * width, center) for i in range(count)]
 
