@@ -188,6 +188,7 @@ typedef struct xmc_stats {
     double wRouletteGain;
     double wOther;        /* glass misses, surfaces without a next layer */
     uint64_t trajectoryOverflows; /* detected photons whose trajectory exceeded trajectoryCapacity */
+    uint64_t entryWalks;          /* walks started at a cell's entry node instead of the root */
 } xmc_stats;
 
 /* One recorded step of a traced photon (debug / parity).  Positions and
@@ -225,6 +226,7 @@ typedef struct xmc_options {
     int trajectoryCapacity; /* voxels kept per photon, 0 = 20480 (MAX_TRAJECTORY_SIZE, mcml_kernel_types.h:8);
                              * a detected photon with more crossings counts only the first ones and is
                              * reported in xmc_stats.trajectoryOverflows */
+    int noEntryGrid;     /* 1 = short step-limited walks start at the BVH root, not at a cell's entry node */
 } xmc_options;
 
 typedef struct xmc_scene xmc_scene;

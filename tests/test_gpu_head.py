@@ -50,6 +50,9 @@ def test_head_full_mesh_ledger_and_falloff(engine):
     total = (out.specular + (st["wAbsorbed"] + st["wEscaped"] + st["wLeftArea"] + st["wKilled"]
                              + st["wRouletteLost"] + st["wOther"] - st["wRouletteGain"]) / n)
     assert total == pytest.approx(1.0, abs=1e-5)
-    assert st["wLeftArea"] == 0.0 and st["wOther"] == 0.0
+    # Moller-Trumbore is not watertight: a ray through a shared edge can miss both
+    # triangles (in the reference's double arithmetic too, more rarely); such a photon
+    # leaves the closed head and dies at the Area boundary
+    assert st["wLeftArea"] / n < 1e-4 and st["wOther"] == 0.0
     w = out.det_weight() / n
     assert np.all(w > 0) and np.all(np.diff(w) < 0)  # reflectance falls with source distance

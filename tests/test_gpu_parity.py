@@ -255,3 +255,30 @@ def test_heightfield_full_size_ledger_and_symmetry(engine):
     assert st["photons"] == n
     assert np.all(g.det_weight() > 0)
     assert 0.999 * g.det_count().sum() <= g.bin_count().sum() <= g.det_count().sum()
+
+
+@pytest.mark.parametrize("grids", [(True, True), (False, True), (True, False)])
+def test_walk_skipping_grids_change_nothing(engine, grids):
+    """The distance bound and the entry nodes only skip work: tallies and
+    event counts equal those of walks that always start at the root."""
+    dist, entry = grids
+    n = 6000
+    spec = scenes.small_scene("heightfield", n)
+    res = []
+    for opts in (dict(distance_grid=False, entry_grid=False), dict(distance_grid=dist, entry_grid=entry)):
+        with engine.Scene(spec, **opts) as sc:
+            sc.run(31, 0, n)
+            out = sc.collect()
+            res.append((out, sc.stats()))
+    (a, sa), (b, sb) = res
+    for k in ("scatterEvents", "boundaryEvents", "rouletteEvents"):
+        assert sa[k] == sb[k], k
+    assert sa["culledWalks"] == 0 and sa["entryWalks"] == 0
+    if entry:
+        assert sb["entryWalks"] > 0
+    if dist:
+        assert sb["culledWalks"] > 0
+    assert np.array_equal(a.det_count(), b.det_count())
+    assert np.array_equal(a.bin_count(), b.bin_count())
+    np.testing.assert_allclose(a.absorption, b.absorption, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(a.det_weight(), b.det_weight(), rtol=1e-11, atol=1e-12)

@@ -157,6 +157,7 @@ class Stats(C.Structure):
         ("wRouletteGain", C.c_double),
         ("wOther", C.c_double),
         ("trajectoryOverflows", C.c_uint64),
+        ("entryWalks", C.c_uint64),
     ]
 
 
@@ -198,6 +199,7 @@ class Options(C.Structure):
         ("scattering", C.c_int),
         ("trajectories", C.c_int),
         ("trajectoryCapacity", C.c_int),
+        ("noEntryGrid", C.c_int),
     ]
 
 

@@ -4,6 +4,7 @@
 //  base/xmcml/xmcmlLauncher/launcher_omp.cpp:10-163).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -16,6 +17,7 @@
 #include "../../include/xmcml.h"
 #include "xmc_bvh.h"
 #include "xmc_distance.h"
+#include "xmc_entry.h"
 #include "xmc_params.h"
 
 extern "C" {
@@ -112,7 +114,9 @@ struct xmc_scene {
            off_q = 0, slab_words = 0;
     DevBuf<int2> spill;
     DevBuf<int> perm_dev;
-    DevBuf<uint8_t> dist;
+    DevBuf<uint32_t> dist;  // walk-skipping grid cells
+    DevBuf<int> entry_ref;
+    DevBuf<float> entry_box;
     // voxel-plane crossings: scattering grid, detector trajectory counts and
     // the per-lane trajectory lists (allocated only when asked for)
     DevBuf<double> scat;
@@ -283,8 +287,10 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
         P.root_hi[a] = gb.root_hi[a];
     }
 
-    // distance grid that lets step-limited walks that cannot hit be skipped
-    if (nt > 256 && o.noDistanceGrid == 0) {
+    // walk-skipping grid: per cell a distance bound (skip walks that cannot hit) and an
+    // entry node (start short walks below the root); about 32 cells per triangle
+    const bool want_dist = o.noDistanceGrid == 0, want_entry = o.noEntryGrid == 0 && gb.root_ref >= 0;
+    if (nt > 256 && (want_dist || want_entry)) {
         xmc::DistanceGrid dg;
         double lo[3], hi[3];
         for (int a = 0; a < 3; a++) {
@@ -292,16 +298,35 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
             lo[a] = gb.root_lo[a] - pad;
             hi[a] = gb.root_hi[a] + pad;
         }
-        xmc::build_distance_grid(in->surface, in->numberOfSurfaces, lo, hi, (size_t)16 << 20, dg);
-        HIP_TRY(sc->dist.alloc(dg.q.size()));
-        HIP_TRY(hipMemcpy(sc->dist.p, dg.q.data(), dg.q.size(), hipMemcpyHostToDevice));
-        P.dist.q = sc->dist.p;
+        size_t target = std::min<size_t>(std::max<size_t>((size_t)nt * 32, (size_t)1 << 18), (size_t)16 << 20);
+        xmc::build_distance_grid(in->surface, in->numberOfSurfaces, lo, hi, target, dg);
+        xmc::EntryGrid eg;
+        if (want_entry) xmc::build_entry_grid(gb, dg.lo, dg.h, dg.dims, eg);
+        std::vector<uint32_t> cells(dg.q.size());
+        for (size_t c = 0; c < cells.size(); c++) {
+            uint32_t q = want_dist ? dg.q[c] : 0u;
+            uint32_t e = !want_entry ? xmc::kEntryRoot : (uint32_t)(eg.cell[c] + 1);
+            cells[c] = q | (e << 8);
+        }
+        if (eg.ref.size() >= xmc::kEntryRoot - 1) return fail(XMC_ERR_LIMIT, "too many BVH entry nodes");
+        HIP_TRY(sc->dist.alloc(cells.size()));
+        HIP_TRY(hipMemcpy(sc->dist.p, cells.data(), cells.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(sc->entry_ref.alloc(std::max<size_t>(eg.ref.size(), 1)));
+        HIP_TRY(sc->entry_box.alloc(std::max<size_t>(eg.box.size(), 1)));
+        if (!eg.ref.empty()) {
+            HIP_TRY(hipMemcpy(sc->entry_ref.p, eg.ref.data(), eg.ref.size() * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(sc->entry_box.p, eg.box.data(), eg.box.size() * 4, hipMemcpyHostToDevice));
+        }
+        P.dist.cells = sc->dist.p;
         for (int a = 0; a < 3; a++) {
             P.dist.lo[a] = dg.lo[a];
             P.dist.dims[a] = dg.dims[a];
         }
         P.dist.inv_h = 1.0f / dg.h;
         P.dist.step_q = dg.step_q;
+        P.dist.margin = want_entry ? eg.margin : 0.0f;
+        P.dist.entry_ref = sc->entry_ref.p;
+        P.dist.entry_box = sc->entry_box.p;
     }
 
     // layer boundary table: FindIntersectionLayer (mcml_kernel.cpp:566-580)
@@ -637,6 +662,7 @@ static int read_stats(unsigned long long* dev, hipStream_t stream, xmc_stats* st
     st->wRouletteGain = led[6];
     st->wOther = led[7];
     st->trajectoryOverflows = c[6];
+    st->entryWalks = c[7];
     HIP_TRY(xmc_launch_zero(dev, 16, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     return XMC_OK;

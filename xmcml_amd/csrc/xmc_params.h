@@ -49,7 +49,7 @@ struct Tallies {
     double* bin_weight;      // ndet * T
     double* bin_target;
     unsigned long long* bin_count;
-    unsigned long long* stats;  // 6 event counters, trajectory overflows, -, then 8 ledger doubles from word 8
+    unsigned long long* stats;  // 6 event counters, trajectory overflows, entry walks, then 8 ledger doubles from word 8
 };
 
 // Voxel-centre plane crossings of the photon path (UpdatePhotonTrajectory,
@@ -79,12 +79,23 @@ struct Trace {
 };
 
 // Conservative distance grid (see xmc_distance.h); q == nullptr disables it.
+// Per-cell word of the walk-skipping grid: bits 0-7 hold q, a lower bound
+// q * step_q on the distance from any point of the cell to any triangle;
+// bits 8-31 hold the cell's entry for walks that move at most `margin`
+// along every axis: 0 = no triangle within reach, kEntryRoot = start at the
+// root, else 1 + an index into entry_ref / entry_box (node reference and its
+// box, 6 floats).
+constexpr uint32_t kEntryRoot = 0xFFFFFFu;
+
 struct DistGrid {
-    const uint8_t* q;
+    const uint32_t* cells;  // null when the scene has no grid
     float lo[3];
     float inv_h;
     float step_q;
+    float margin;
     int dims[3];
+    const int* entry_ref;
+    const float* entry_box;
 };
 
 struct Params {

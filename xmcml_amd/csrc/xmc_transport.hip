@@ -27,6 +27,9 @@
 namespace xmc {
 
 constexpr float kInf = __builtin_inff();
+#ifndef XMC_VISITS_PER_ROUND
+#define XMC_VISITS_PER_ROUND 4
+#endif
 constexpr float kEpsF = 1.0e-6f;                 // EPSILON
 constexpr float kCosZeroF = 1.0f - 1.0e-6f;      // float "cosine of zero"
 constexpr float kCos90F = 1.0e-12f;              // COSINE_OF_90D
@@ -233,99 +236,133 @@ struct Hit {
 };
 
 // ComputeBVHIntersection (limited = true, :354-466) or
-// ComputeBVHIntersectionWithoutStep (:468-584).
-__device__ Hit walk(const Geo& g, const Ray& r, float step, bool limited, Stack& st) {
-    Hit h;
+// ComputeBVHIntersectionWithoutStep (:468-584), as a resumable walk:
+// walk_begin tests the start node's box, walk_visit processes one node or
+// leaf and finds the next one.  Both return false when the walk is over; h
+// then holds the result.  Run to completion, they are the reference's loop.
+__device__ __forceinline__ bool walk_begin(const Geo& g, const Ray& r, float step, bool limited, Stack& st, Hit& h,
+                                           int& cur, int& best) {
     h.dist = kInf;
     h.tri = -1;
     h.found = false;
-    float tn1, tn2;
-    if (!box_hit(r, g.rlo[0], g.rlo[1], g.rlo[2], g.rhi[0], g.rhi[1], g.rhi[2], tn1)) return h;
-    if (limited && step < tn1) return h;
+    best = -1;
+    float tn;
+    if (!box_hit(r, g.rlo[0], g.rlo[1], g.rlo[2], g.rhi[0], g.rhi[1], g.rhi[2], tn)) return false;
+    if (limited && step < tn) return false;
     st.sp = 0;
-    int cur = g.root_ref;
-    while (true) {
-        bool use_stack;
-        if (cur < 0) {
-            int i = ~cur, best = -1, last;
-            do {
-                float t = tri_distance(g.tris, i, r, last);
-                if (t >= 0.0f && t < h.dist) {
-                    h.dist = t;
-                    best = i;
-                }
-                i++;
-            } while (!last);
-            if (best >= 0 && (!limited || h.dist <= step)) {
-                h.found = true;
-                h.tri = best;
-            }
-            use_stack = true;
-        } else {
-            const float4* N = reinterpret_cast<const float4*>(g.nodes) + 4 * cur;
-            float4 a = N[0], b = N[1], c = N[2];
-            int4 d = reinterpret_cast<const int4*>(N)[3];
-            bool f1 = box_hit(r, a.x, a.y, a.z, a.w, b.x, b.y, tn1);
-            bool f2 = box_hit(r, b.z, b.w, c.x, c.y, c.z, c.w, tn2);
-            int left = d.x, right = d.y;
-            use_stack = false;
-            if (limited) {
-                if (f1 && f2) {
-                    if (tn1 <= tn2) {
-                        if (tn1 <= step) {
-                            cur = left;
-                            if (tn2 <= step) st.push(right, tn2);
-                        } else {
-                            use_stack = true;
-                        }
-                    } else {
-                        if (tn2 <= step) {
-                            cur = right;
-                            if (tn1 <= step) st.push(left, tn1);
-                        } else {
-                            use_stack = true;
-                        }
-                    }
-                } else if (f1) {
-                    if (tn1 <= step) cur = left;
-                    else use_stack = true;
-                } else {
-                    if (f2 && tn2 <= step) cur = right;
-                    else use_stack = true;
-                }
-            } else {
-                if (f1 && f2) {
-                    if (tn1 <= tn2) {
-                        cur = left;
-                        st.push(right, tn2);
-                    } else {
-                        cur = right;
-                        st.push(left, tn1);
-                    }
-                } else if (f1) {
-                    cur = left;
-                } else if (f2) {
-                    cur = right;
-                } else {
-                    use_stack = true;
-                }
-            }
+    cur = g.root_ref;
+    return true;
+}
+
+// A visit is one node (both child boxes) or one triangle of a leaf: inside a
+// leaf, cur = ~(next triangle) and best is the leaf's closest so far, so a
+// lane testing a triangle and a lane testing a node cost about the same.
+__device__ __forceinline__ bool walk_visit(const GpuNode* nodes, const GpuTri* tris, const Ray& r, float step,
+                                           bool limited, Stack& st, Hit& h, int& cur, int& best) {
+    bool use_stack;
+    float tn1, tn2;
+    if (cur < 0) {
+        int i = ~cur, last;
+        float t = tri_distance(tris, i, r, last);
+        if (t >= 0.0f && t < h.dist) {
+            h.dist = t;
+            best = i;
         }
-        if (use_stack) {
-            if (st.sp == 0) break;
-            int2 e = st.pop();
-            float tn = __int_as_float(e.y);
-            cur = e.x;
-            if (h.dist < tn) {
-                while (st.sp > 0 && h.dist < tn) {
-                    e = st.pop();
-                    tn = __int_as_float(e.y);
+        if (!last) {
+            cur = ~(i + 1);
+            return true;
+        }
+        if (best >= 0 && (!limited || h.dist <= step)) {
+            h.found = true;
+            h.tri = best;
+        }
+        best = -1;
+        use_stack = true;
+    } else {
+        const float4* N = reinterpret_cast<const float4*>(nodes) + 4 * cur;
+        float4 a = N[0], b = N[1], c = N[2];
+        int4 d = reinterpret_cast<const int4*>(N)[3];
+        bool f1 = box_hit(r, a.x, a.y, a.z, a.w, b.x, b.y, tn1);
+        bool f2 = box_hit(r, b.z, b.w, c.x, c.y, c.z, c.w, tn2);
+        int left = d.x, right = d.y;
+        use_stack = false;
+        if (limited) {
+            if (f1 && f2) {
+                if (tn1 <= tn2) {
+                    if (tn1 <= step) {
+                        cur = left;
+                        if (tn2 <= step) st.push(right, tn2);
+                    } else {
+                        use_stack = true;
+                    }
+                } else {
+                    if (tn2 <= step) {
+                        cur = right;
+                        if (tn1 <= step) st.push(left, tn1);
+                    } else {
+                        use_stack = true;
+                    }
                 }
-                if (h.dist < tn || (limited && step < tn)) break;
-                cur = e.x;
+            } else if (f1) {
+                if (tn1 <= step) cur = left;
+                else use_stack = true;
+            } else {
+                if (f2 && tn2 <= step) cur = right;
+                else use_stack = true;
+            }
+        } else {
+            if (f1 && f2) {
+                if (tn1 <= tn2) {
+                    cur = left;
+                    st.push(right, tn2);
+                } else {
+                    cur = right;
+                    st.push(left, tn1);
+                }
+            } else if (f1) {
+                cur = left;
+            } else if (f2) {
+                cur = right;
+            } else {
+                use_stack = true;
             }
         }
     }
+    if (use_stack) {
+        if (st.sp == 0) return false;
+        int2 e = st.pop();
+        float tn = __int_as_float(e.y);
+        cur = e.x;
+        if (h.dist < tn) {
+            while (st.sp > 0 && h.dist < tn) {
+                e = st.pop();
+                tn = __int_as_float(e.y);
+            }
+            if (h.dist < tn || (limited && step < tn)) return false;
+            cur = e.x;
+        }
+    }
+    return true;
+}
+
+// One visit as a call (ray queries).  Inlined into the loop of walk() below,
+// this build's compiler produced wrong hits for unlimited walks on gfx950
+// (misses where the same source, compiled for the host or called out of
+// line, finds the hit); the transport loop inlines walk_visit and is held to
+// the float oracle bit for bit.
+__device__ __attribute__((noinline)) bool walk_visit_call(const GpuNode* nodes, const GpuTri* tris, const Ray& r,
+                                                          float step, bool limited, Stack& st, Hit& h, int& cur,
+                                                          int& best) {
+    return walk_visit(nodes, tris, r, step, limited, st, h, cur, best);
+}
+
+// The whole walk at once (ray queries).
+__device__ __forceinline__ Hit walk(const Geo& g, const Ray& r, float step, bool limited, Stack& st) {
+    Hit h;
+    int cur, best;
+    if (walk_begin(g, r, step, limited, st, h, cur, best))
+        while (walk_visit_call(g.nodes, g.tris, r, step, limited, st, h, cur, best)) {
+        }
     return h;
 }
 
@@ -532,20 +569,27 @@ __device__ __forceinline__ int time_bin(const Detectors& D, double t) {
     return -1;
 }
 
-// True when no triangle can lie within the segment of length step * |d|
-// from p: the distance grid's lower bound for p's cell exceeds it by a
-// margin that covers float rounding of the cell lookup and the triangle test.
-__device__ __forceinline__ bool no_hit_possible(const DistGrid& G, V3 p, V3 d, float step) {
+// Where a step-limited walk from p along d over `step` starts: -1 nothing
+// can be hit (skip the walk), -2 at the root, e >= 0 at entry e.  Skipping
+// needs either the cell's distance bound to exceed the step by a margin that
+// covers float rounding of the lookup and of the triangle test, or a step that
+// moves at most the grid's margin along every axis from a cell with no
+// triangle within reach; a short step from a cell with an entry starts there.
+__device__ __forceinline__ int walk_start(const DistGrid& G, V3 p, V3 d, float step) {
     float fx = (p.x - G.lo[0]) * G.inv_h;
     float fy = (p.y - G.lo[1]) * G.inv_h;
     float fz = (p.z - G.lo[2]) * G.inv_h;
-    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f)) return false;
+    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f)) return -2;
     int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-    if (ix >= G.dims[0] || iy >= G.dims[1] || iz >= G.dims[2]) return false;
-    float L = (float)G.q[((size_t)iz * G.dims[1] + iy) * G.dims[0] + ix] * G.step_q;
-    float margin = 1.0e-4f + 16.0f * offset_scale(p);
-    float room = L - margin;
-    return room > 0.0f && step * step * dot(d, d) < room * room;
+    if (ix >= G.dims[0] || iy >= G.dims[1] || iz >= G.dims[2]) return -2;
+    uint32_t word = G.cells[((size_t)iz * G.dims[1] + iy) * G.dims[0] + ix];
+    float L = (float)(word & 0xFFu) * G.step_q;
+    float room = L - (1.0e-4f + 16.0f * offset_scale(p));
+    if (room > 0.0f && step * step * dot(d, d) < room * room) return -1;
+    float reach = step * fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z)));
+    uint32_t e = word >> 8;
+    if (!(reach <= G.margin) || e == kEntryRoot) return -2;
+    return (int)e - 1;
 }
 
 template <bool TRACE>
@@ -585,6 +629,13 @@ enum { kAbs = 0, kEsc, kDet, kArea, kKill, kRLost, kRGain, kOther, kLedger };
 // spill, trace, counters) come from P; per-configuration fields (geometry,
 // optics, detectors, tallies, source, termination) from CFG(), which is P
 // itself for a single scene and the lane's entry of `cfgs` in a batch.
+// Lane phases of the persistent loop.  Each round a lane either starts an
+// interaction (step and walk start), advances its BVH walk by a few visits,
+// or resolves the interaction (boundary or scatter, termination) -- so a
+// long walk in one lane no longer holds the wave's other lanes.
+constexpr int kBegin = 0, kWalk = 1, kResolve = 2;
+constexpr int kVisitsPerRound = XMC_VISITS_PER_ROUND;
+
 template <bool TRACE, bool BATCHED, bool PATHS>
 __device__ __forceinline__ void transport_body(const Params& P, const Params* __restrict__ cfgs,
                                                const uint64_t* __restrict__ prefix, int ncfg) {
@@ -617,9 +668,19 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     uint32_t draws = 0;
     Recorder<TRACE> rec{nullptr, 0, 0};
     unsigned long long n_scatter = 0, n_boundary = 0, n_roulette = 0, n_photons = 0, n_culled = 0;
-    unsigned long long n_overflow = 0;
+    unsigned long long n_overflow = 0, n_entry = 0;
     uint32_t* vox = nullptr;  // this lane's trajectory (voxel indices)
     int nvox = 0;
+    // the interaction in progress: its step, walk and hit
+    int phase = kBegin;
+    float step = kInf;
+    Hit h;
+    h.found = false;
+    h.dist = kInf;
+    h.tri = -1;
+    V3 rinv = {0.0f, 0.0f, 0.0f};  // the walking ray is (pos, dir) with these inverses and axis flags
+    uint32_t raxes = 0;
+    int cur = 0, best = -1;
     if constexpr (PATHS) {
         if (P.paths.det_traj)
             vox = P.paths.lane_voxels + (size_t)(blockIdx.x * blockDim.x + tid) * (size_t)P.paths.capacity;
@@ -680,6 +741,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                     n_photons++;
                     draws = 0;
                     nvox = 0;
+                    phase = kBegin;
                     if (TRACE) {
                         uint64_t k = pid - P.trace.first;
                         rec.ev = (k < P.trace.n) ? P.trace.events + k * (uint64_t)P.trace.max_events : nullptr;
@@ -696,34 +758,63 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
         if (__ballot(alive) == 0) break;
         if (!alive) continue;
 
-        if (draws > kDrawLimit) {  // stream block exhausted
-            alive = false;
-            atomicAdd(ledger + kOther, (double)w);
-            continue;
-        }
-        // ---- one interaction (mcml_kernel.cpp:47-124)
-        float step = kInf;
-        if (!Lglass) {
-            float u;
-            do {
-                u = next_u(x);
-                draws++;
-            } while (u == 0.0f);
-            step = -xlogf(u) / Lmut;
-        }
-        const Params& C = CFG();
-        Geo geo{C.nodes, C.tris, C.root_ref, {C.root_lo[0], C.root_lo[1], C.root_lo[2]},
-                {C.root_hi[0], C.root_hi[1], C.root_hi[2]}};
-        Hit h;
-        if (!Lglass && C.dist.q != nullptr && no_hit_possible(C.dist, pos, dir, step)) {
-            h.found = false;  // the walk could not find anything within the step
+        if (phase == kBegin) {
+            if (draws > kDrawLimit) {  // stream block exhausted
+                alive = false;
+                atomicAdd(ledger + kOther, (double)w);
+                continue;
+            }
+            // ---- one interaction (mcml_kernel.cpp:47-124): the step, then the walk
+            step = kInf;
+            if (!Lglass) {
+                float u;
+                do {
+                    u = next_u(x);
+                    draws++;
+                } while (u == 0.0f);
+                step = -xlogf(u) / Lmut;
+            }
+            const Params& C = CFG();
+            h.found = false;  // until a walk finds something within the step
             h.dist = kInf;
             h.tri = -1;
-            n_culled++;
-        } else {
-            Ray ray = make_ray(pos, dir);
-            h = walk(geo, ray, step, !Lglass, st);
+            int e = (!Lglass && C.dist.cells != nullptr) ? walk_start(C.dist, pos, dir, step) : -2;
+            if (e == -1) {
+                n_culled++;
+                phase = kResolve;
+            } else {
+                Geo geo{C.nodes, C.tris, C.root_ref, {C.root_lo[0], C.root_lo[1], C.root_lo[2]},
+                        {C.root_hi[0], C.root_hi[1], C.root_hi[2]}};
+                if (e >= 0) {  // start below the root: same leaves, same order, same hit
+                    geo.root_ref = C.dist.entry_ref[e];
+                    const float* b = C.dist.entry_box + 6 * e;
+                    geo.rlo[0] = b[0];
+                    geo.rlo[1] = b[1];
+                    geo.rlo[2] = b[2];
+                    geo.rhi[0] = b[3];
+                    geo.rhi[1] = b[4];
+                    geo.rhi[2] = b[5];
+                    n_entry++;
+                }
+                Ray ray = make_ray(pos, dir);
+                rinv = ray.inv;
+                raxes = (ray.ax ? 1u : 0u) | (ray.ay ? 2u : 0u) | (ray.az ? 4u : 0u);
+                phase = walk_begin(geo, ray, step, !Lglass, st, h, cur, best) ? kWalk : kResolve;
+            }
         }
+        // ---- up to kVisitsPerRound BVH visits for the lanes still walking; a lane
+        // whose walk is long keeps walking next round while the others carry on
+        for (int k = 0; k < kVisitsPerRound; ++k) {
+            if (phase == kWalk) {
+                const Params& C = CFG();
+                Ray ray{pos, dir, rinv, (raxes & 1u) != 0, (raxes & 2u) != 0, (raxes & 4u) != 0};
+                if (!walk_visit(C.nodes, C.tris, ray, step, !Lglass, st, h, cur, best)) phase = kResolve;
+            }
+            if (__ballot(phase == kWalk) == 0) break;
+        }
+        if (phase != kResolve) continue;
+        phase = kBegin;
+        const Params& C = CFG();
         if (h.found) {
             step = h.dist - 0.01f * kMinDistF;
             tim += step * Ln;
@@ -734,7 +825,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             pos.y += step * dir.y;
             pos.z += step * dir.z;
             V3 hit = pos;
-            V3 nrm = tri_normal(geo.tris, h.tri);
+            V3 nrm = tri_normal(C.tris, h.tri);
             float ofs = offset_scale(hit);
             if (dot(dir, nrm) < 0.0f) ofs = -ofs;
             pos.x = hit.x - ofs * nrm.x;
@@ -746,7 +837,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                 // CrossBoundary (:229-284)
                 float ci = dot(dir, nrm);
                 if (ci < 0.0f) ci = -ci;
-                int surface = __float_as_int(reinterpret_cast<const float4*>(geo.tris)[3 * h.tri].w);
+                int surface = __float_as_int(reinterpret_cast<const float4*>(C.tris)[3 * h.tri].w);
                 int nxt = C.next_layer[layer * C.n_surfaces + surface];
                 if (nxt < 0) {
                     alive = false;
@@ -895,6 +986,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     if (st.spills) atomicAdd(P.tally.stats + 4, (unsigned long long)st.spills);
     if (n_culled) atomicAdd(P.tally.stats + 5, n_culled);
     if (n_overflow) atomicAdd(P.tally.stats + 6, n_overflow);
+    if (n_entry) atomicAdd(P.tally.stats + 7, n_entry);
     atomicAdd(ledger + kAbs, w_abs);
     __syncthreads();
     if (tid < kLedger) unsafeAtomicAdd(reinterpret_cast<double*>(P.tally.stats + 8) + tid, ledger[tid]);

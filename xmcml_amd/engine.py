@@ -125,11 +125,12 @@ class Scene:
 
     def __init__(self, spec: SceneSpec, block_size: int = 0, blocks_per_cu: int = 0,
                  absorption: bool = True, device: int = 0, distance_grid: bool = True,
-                 scattering: bool = False, trajectories: bool = False, trajectory_capacity: int = 0):
+                 scattering: bool = False, trajectories: bool = False, trajectory_capacity: int = 0,
+                 entry_grid: bool = True):
         self.spec = spec
         self.input = InputHandle(spec)
         opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1, 0 if distance_grid else 1,
-                      int(scattering), int(trajectories), trajectory_capacity)
+                      int(scattering), int(trajectories), trajectory_capacity, 0 if entry_grid else 1)
         self.scattering, self.trajectories = scattering, trajectories
         h = _P()
         _check(lib().xmc_scene_create(self.input.ptr, C.byref(opt), C.byref(h)), "xmc_scene_create")
@@ -279,7 +280,7 @@ def launch(spec: SceneSpec, out: OutputHandle, seed: int, first: int, n: int, ab
     Returns the next photon index of the stream."""
     inp = InputHandle(spec, photons=n)
     rng = Rng(seed, first)
-    opt = Options(0, 0, 0, 0 if absorption else 1, 0, 0, 0, 0)
+    opt = Options(0, 0, 0, 0 if absorption else 1, 0, 0, 0, 0, 0)
     _check(lib().xmc_launch(inp.ptr, out.ptr, C.byref(rng), C.byref(opt)), "xmc_launch")
     return rng.nextPhoton
 
@@ -295,7 +296,7 @@ class Batch:
         self.inputs = [InputHandle(s) for s in self.specs]
         arr = (Input * len(self.inputs))(*[h.c for h in self.inputs])
         self._arr = arr
-        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1, 0, 0, 0, 0)
+        opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1, 0, 0, 0, 0, 0)
         h = _P()
         _check(lib().xmc_batch_create(arr, len(self.inputs), C.byref(opt), C.byref(h)), "xmc_batch_create")
         self._h = h
