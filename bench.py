@@ -169,13 +169,14 @@ def surrogate_line(model_path: Path, queries: int, mc_configs: int, mc_photons: 
     }
 
 
-def traffic_from_profiles():
-    """HBM bytes per transport launch from the committed rocprofv3 PMC
-    summary, when one exists."""
-    f = ROOT / "profiles" / "transport_heightfield_pmc.json"
+def traffic_from_profiles(photons_per_launch: int, workload: str):
+    """HBM bytes per transport launch: the committed rocprofv3 FETCH_SIZE +
+    WRITE_SIZE summary (per photon, copy-calibrated) times the photons of one
+    launch; None for workloads without a profile."""
+    f = ROOT / "profiles" / f"transport_{workload}_pmc.json"
     if f.exists():
         try:
-            return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+            return json.loads(f.read_text())["hbm_bytes_per_photon"] * photons_per_launch
         except Exception:
             return None
     return None
@@ -249,8 +250,8 @@ def main(argv=None) -> int:
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
         "frac_of_nonpacked_peak": achieved / PEAK_FP32_NONPACKED_TFLOPS,
-        "traffic": traffic_from_profiles(),
-        "kernel": "xmc::transport_kernel<false>",
+        "traffic": traffic_from_profiles(args.photons, args.workload),
+        "kernel": "xmc::transport_kernel<false, false>",
         "kernel_ms_per_launch": launch_s * 1e3,
         "flop_per_launch": flop / args.steps,
         "scatter_events_per_photon": st["scatterEvents"] / photons,
