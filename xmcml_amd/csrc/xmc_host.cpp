@@ -25,6 +25,7 @@ hipError_t xmc_launch_transport(const xmc::Params* p, int grid, int block, size_
 hipError_t xmc_launch_query(const xmc::QueryParams* q, const int* perm, int block, size_t lds, hipStream_t s);
 hipError_t xmc_launch_zero(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t xmc_transport_occupancy(int block, size_t lds, int* blocks_per_cu);
+hipError_t xmc_transport_occupancy_plain(int block, size_t lds, int* blocks_per_cu);
 hipError_t xmc_launch_batch(const xmc::Params* p, const xmc::Params* cfgs, const uint64_t* prefix, int ncfg, int grid,
                             int block, size_t lds, hipStream_t s);
 hipError_t xmc_batch_occupancy(int block, size_t lds, int* blocks_per_cu);
@@ -219,6 +220,12 @@ static int configure_launch(int device, int depth, int block, int blocks_per_cu,
     HIP_TRY(spill.alloc(P.spill_depth > 0 ? (size_t)P.spill_depth * lanes : 1));
     P.spill = spill.p;
     return XMC_OK;
+}
+
+// The occupancy query of the transport kernel xmc_launch_transport picks for
+// this scene (interleaved when it has a walk-skipping grid).
+static auto transport_occupancy_of(const xmc::Params& P) -> hipError_t (*)(int, size_t, int*) {
+    return P.dist.cells != nullptr ? xmc_transport_occupancy : xmc_transport_occupancy_plain;
 }
 
 // One trajectory list of traj_cap voxel indices per persistent lane.
@@ -503,7 +510,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
 
     sc->block = o.blockSize > 0 ? o.blockSize : 256;
     if (launchable) {
-        rc = configure_launch(sc->device, sc->ref.depth, sc->block, o.blocksPerCU, xmc_transport_occupancy,
+        rc = configure_launch(sc->device, sc->ref.depth, sc->block, o.blocksPerCU, transport_occupancy_of(P),
                               &sc->blocks, sc->spill, P);
         if (rc) return rc;
         if ((rc = alloc_lane_voxels(sc))) return rc;
@@ -518,8 +525,8 @@ int xmc_scene_set_launch(xmc_scene* sc, int block, int blocks_per_cu) {
     HIP_TRY(hipSetDevice(sc->device));
     HIP_TRY(hipStreamSynchronize(sc->stream));
     int blocks = 0;
-    int rc = configure_launch(sc->device, sc->ref.depth, block, blocks_per_cu, xmc_transport_occupancy, &blocks,
-                              sc->spill, sc->base);
+    int rc = configure_launch(sc->device, sc->ref.depth, block, blocks_per_cu, transport_occupancy_of(sc->base),
+                              &blocks, sc->spill, sc->base);
     if (rc) return rc;
     sc->block = block;
     sc->blocks = blocks;

@@ -345,24 +345,101 @@ __device__ __forceinline__ bool walk_visit(const GpuNode* nodes, const GpuTri* t
     return true;
 }
 
-// One visit as a call (ray queries).  Inlined into the loop of walk() below,
-// this build's compiler produced wrong hits for unlimited walks on gfx950
-// (misses where the same source, compiled for the host or called out of
-// line, finds the hit); the transport loop inlines walk_visit and is held to
-// the float oracle bit for bit.
-__device__ __attribute__((noinline)) bool walk_visit_call(const GpuNode* nodes, const GpuTri* tris, const Ray& r,
-                                                          float step, bool limited, Stack& st, Hit& h, int& cur,
-                                                          int& best) {
-    return walk_visit(nodes, tris, r, step, limited, st, h, cur, best);
-}
-
-// The whole walk at once (ray queries).
+// The whole walk in one loop: the same visits as walk_begin/walk_visit, with
+// no state kept between them (ray queries, and transport scenes whose walks
+// are a handful of visits).
 __device__ __forceinline__ Hit walk(const Geo& g, const Ray& r, float step, bool limited, Stack& st) {
     Hit h;
-    int cur, best;
-    if (walk_begin(g, r, step, limited, st, h, cur, best))
-        while (walk_visit_call(g.nodes, g.tris, r, step, limited, st, h, cur, best)) {
+    h.dist = kInf;
+    h.tri = -1;
+    h.found = false;
+    float tn1, tn2;
+    if (!box_hit(r, g.rlo[0], g.rlo[1], g.rlo[2], g.rhi[0], g.rhi[1], g.rhi[2], tn1)) return h;
+    if (limited && step < tn1) return h;
+    st.sp = 0;
+    int cur = g.root_ref;
+    while (true) {
+        bool use_stack;
+        if (cur < 0) {
+            int i = ~cur, best = -1, last;
+            do {
+                float t = tri_distance(g.tris, i, r, last);
+                if (t >= 0.0f && t < h.dist) {
+                    h.dist = t;
+                    best = i;
+                }
+                i++;
+            } while (!last);
+            if (best >= 0 && (!limited || h.dist <= step)) {
+                h.found = true;
+                h.tri = best;
+            }
+            use_stack = true;
+        } else {
+            const float4* N = reinterpret_cast<const float4*>(g.nodes) + 4 * cur;
+            float4 a = N[0], b = N[1], c = N[2];
+            int4 d = reinterpret_cast<const int4*>(N)[3];
+            bool f1 = box_hit(r, a.x, a.y, a.z, a.w, b.x, b.y, tn1);
+            bool f2 = box_hit(r, b.z, b.w, c.x, c.y, c.z, c.w, tn2);
+            int left = d.x, right = d.y;
+            use_stack = false;
+            if (limited) {
+                if (f1 && f2) {
+                    if (tn1 <= tn2) {
+                        if (tn1 <= step) {
+                            cur = left;
+                            if (tn2 <= step) st.push(right, tn2);
+                        } else {
+                            use_stack = true;
+                        }
+                    } else {
+                        if (tn2 <= step) {
+                            cur = right;
+                            if (tn1 <= step) st.push(left, tn1);
+                        } else {
+                            use_stack = true;
+                        }
+                    }
+                } else if (f1) {
+                    if (tn1 <= step) cur = left;
+                    else use_stack = true;
+                } else {
+                    if (f2 && tn2 <= step) cur = right;
+                    else use_stack = true;
+                }
+            } else {
+                if (f1 && f2) {
+                    if (tn1 <= tn2) {
+                        cur = left;
+                        st.push(right, tn2);
+                    } else {
+                        cur = right;
+                        st.push(left, tn1);
+                    }
+                } else if (f1) {
+                    cur = left;
+                } else if (f2) {
+                    cur = right;
+                } else {
+                    use_stack = true;
+                }
+            }
         }
+        if (use_stack) {
+            if (st.sp == 0) break;
+            int2 e = st.pop();
+            float tn = __int_as_float(e.y);
+            cur = e.x;
+            if (h.dist < tn) {
+                while (st.sp > 0 && h.dist < tn) {
+                    e = st.pop();
+                    tn = __int_as_float(e.y);
+                }
+                if (h.dist < tn || (limited && step < tn)) break;
+                cur = e.x;
+            }
+        }
+    }
     return h;
 }
 
@@ -632,11 +709,14 @@ enum { kAbs = 0, kEsc, kDet, kArea, kKill, kRLost, kRGain, kOther, kLedger };
 // Lane phases of the persistent loop.  Each round a lane either starts an
 // interaction (step and walk start), advances its BVH walk by a few visits,
 // or resolves the interaction (boundary or scatter, termination) -- so a
-// long walk in one lane no longer holds the wave's other lanes.
+// long walk in one lane no longer holds the wave's other lanes.  Scenes whose
+// walks are a handful of visits (no walk-skipping grid) run the walks to the
+// end within the round instead (INTERLEAVE = false): fewer registers live
+// across rounds, more waves per SIMD.
 constexpr int kBegin = 0, kWalk = 1, kResolve = 2;
 constexpr int kVisitsPerRound = XMC_VISITS_PER_ROUND;
 
-template <bool TRACE, bool BATCHED, bool PATHS>
+template <bool TRACE, bool BATCHED, bool PATHS, bool INTERLEAVE>
 __device__ __forceinline__ void transport_body(const Params& P, const Params* __restrict__ cfgs,
                                                const uint64_t* __restrict__ prefix, int ncfg) {
     extern __shared__ int2 lds_stack[];
@@ -667,8 +747,10 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     bool Lglass = false, Ltarget = false;
     uint32_t draws = 0;
     Recorder<TRACE> rec{nullptr, 0, 0};
-    unsigned long long n_scatter = 0, n_boundary = 0, n_roulette = 0, n_photons = 0, n_culled = 0;
-    unsigned long long n_overflow = 0, n_entry = 0;
+    // per-lane event counters, 32-bit (a lane sees far fewer than 2^32 events per launch,
+    // and flush() empties them long before): two registers fewer each than 64-bit ones
+    uint32_t n_scatter = 0, n_boundary = 0, n_roulette = 0, n_photons = 0, n_culled = 0;
+    uint32_t n_overflow = 0, n_entry = 0;
     uint32_t* vox = nullptr;  // this lane's trajectory (voxel indices)
     int nvox = 0;
     // the interaction in progress: its step, walk and hit
@@ -758,6 +840,19 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
         if (__ballot(alive) == 0) break;
         if (!alive) continue;
 
+        if constexpr (!INTERLEAVE) {
+            // every round starts an interaction: nothing of the last one is carried over,
+            // which tells the compiler that none of its registers live across rounds
+            phase = kBegin;
+            step = kInf;
+            h.found = false;
+            h.dist = kInf;
+            h.tri = -1;
+            rinv = {0.0f, 0.0f, 0.0f};
+            raxes = 0;
+            cur = 0;
+            best = -1;
+        }
         if (phase == kBegin) {
             if (draws > kDrawLimit) {  // stream block exhausted
                 alive = false;
@@ -797,20 +892,27 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                     n_entry++;
                 }
                 Ray ray = make_ray(pos, dir);
-                rinv = ray.inv;
-                raxes = (ray.ax ? 1u : 0u) | (ray.ay ? 2u : 0u) | (ray.az ? 4u : 0u);
-                phase = walk_begin(geo, ray, step, !Lglass, st, h, cur, best) ? kWalk : kResolve;
+                if constexpr (INTERLEAVE) {
+                    rinv = ray.inv;
+                    raxes = (ray.ax ? 1u : 0u) | (ray.ay ? 2u : 0u) | (ray.az ? 4u : 0u);
+                    phase = walk_begin(geo, ray, step, !Lglass, st, h, cur, best) ? kWalk : kResolve;
+                } else {
+                    h = walk(geo, ray, step, !Lglass, st);
+                    phase = kResolve;
+                }
             }
         }
         // ---- up to kVisitsPerRound BVH visits for the lanes still walking; a lane
         // whose walk is long keeps walking next round while the others carry on
-        for (int k = 0; k < kVisitsPerRound; ++k) {
-            if (phase == kWalk) {
-                const Params& C = CFG();
-                Ray ray{pos, dir, rinv, (raxes & 1u) != 0, (raxes & 2u) != 0, (raxes & 4u) != 0};
-                if (!walk_visit(C.nodes, C.tris, ray, step, !Lglass, st, h, cur, best)) phase = kResolve;
+        if constexpr (INTERLEAVE) {
+            for (int k = 0; k < kVisitsPerRound; ++k) {
+                if (phase == kWalk) {
+                    const Params& C = CFG();
+                    Ray ray{pos, dir, rinv, (raxes & 1u) != 0, (raxes & 2u) != 0, (raxes & 4u) != 0};
+                    if (!walk_visit(C.nodes, C.tris, ray, step, !Lglass, st, h, cur, best)) phase = kResolve;
+                }
+                if (__ballot(phase == kWalk) == 0) break;
             }
-            if (__ballot(phase == kWalk) == 0) break;
         }
         if (phase != kResolve) continue;
         phase = kBegin;
@@ -979,22 +1081,22 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             P.trace.counts[k] = rec.n;
         }
     }
-    if (n_photons) atomicAdd(P.tally.stats + 0, n_photons);
-    if (n_scatter) atomicAdd(P.tally.stats + 1, n_scatter);
-    if (n_boundary) atomicAdd(P.tally.stats + 2, n_boundary);
-    if (n_roulette) atomicAdd(P.tally.stats + 3, n_roulette);
+    if (n_photons) atomicAdd(P.tally.stats + 0, (unsigned long long)n_photons);
+    if (n_scatter) atomicAdd(P.tally.stats + 1, (unsigned long long)n_scatter);
+    if (n_boundary) atomicAdd(P.tally.stats + 2, (unsigned long long)n_boundary);
+    if (n_roulette) atomicAdd(P.tally.stats + 3, (unsigned long long)n_roulette);
     if (st.spills) atomicAdd(P.tally.stats + 4, (unsigned long long)st.spills);
-    if (n_culled) atomicAdd(P.tally.stats + 5, n_culled);
-    if (n_overflow) atomicAdd(P.tally.stats + 6, n_overflow);
-    if (n_entry) atomicAdd(P.tally.stats + 7, n_entry);
+    if (n_culled) atomicAdd(P.tally.stats + 5, (unsigned long long)n_culled);
+    if (n_overflow) atomicAdd(P.tally.stats + 6, (unsigned long long)n_overflow);
+    if (n_entry) atomicAdd(P.tally.stats + 7, (unsigned long long)n_entry);
     atomicAdd(ledger + kAbs, w_abs);
     __syncthreads();
     if (tid < kLedger) unsafeAtomicAdd(reinterpret_cast<double*>(P.tally.stats + 8) + tid, ledger[tid]);
 }
 
-template <bool TRACE, bool PATHS>
+template <bool TRACE, bool PATHS, bool INTERLEAVE>
 __global__ void __launch_bounds__(256) transport_kernel(Params P) {
-    transport_body<TRACE, false, PATHS>(P, nullptr, nullptr, 0);
+    transport_body<TRACE, false, PATHS, INTERLEAVE>(P, nullptr, nullptr, 0);
 }
 
 // Many configurations in one launch: P carries the run-wide fields and the
@@ -1002,13 +1104,17 @@ __global__ void __launch_bounds__(256) transport_kernel(Params P) {
 // tallies of configuration c.
 __global__ void __launch_bounds__(256) batch_kernel(Params P, const Params* __restrict__ cfgs,
                                                     const uint64_t* __restrict__ prefix, int ncfg) {
-    transport_body<false, true, false>(P, cfgs, prefix, ncfg);
+    transport_body<false, true, false, false>(P, cfgs, prefix, ncfg);
 }
 
-template __global__ void transport_kernel<false, false>(Params);
-template __global__ void transport_kernel<true, false>(Params);
-template __global__ void transport_kernel<false, true>(Params);
-template __global__ void transport_kernel<true, true>(Params);
+template __global__ void transport_kernel<false, false, true>(Params);
+template __global__ void transport_kernel<true, false, true>(Params);
+template __global__ void transport_kernel<false, true, true>(Params);
+template __global__ void transport_kernel<true, true, true>(Params);
+template __global__ void transport_kernel<false, false, false>(Params);
+template __global__ void transport_kernel<true, false, false>(Params);
+template __global__ void transport_kernel<false, true, false>(Params);
+template __global__ void transport_kernel<true, true, false>(Params);
 
 // One ray per thread: nearest hit through the BVH or over every triangle.
 __global__ void __launch_bounds__(256) query_kernel(QueryParams Q, const int* perm) {
@@ -1067,19 +1173,27 @@ __global__ void zero_kernel(unsigned long long* p, size_t n) {
 
 }  // namespace xmc
 
+template <bool TRACE, bool PATHS>
+static void launch_transport(const xmc::Params* p, bool interleave, int grid, int block, size_t lds, hipStream_t s) {
+    if (interleave)
+        hipLaunchKernelGGL((xmc::transport_kernel<TRACE, PATHS, true>), dim3(grid), dim3(block), lds, s, *p);
+    else
+        hipLaunchKernelGGL((xmc::transport_kernel<TRACE, PATHS, false>), dim3(grid), dim3(block), lds, s, *p);
+}
+
+
 // Host-callable launchers (the host translation unit has no kernel syntax).
 extern "C" {
 
+// The interleaved loop for scenes with a walk-skipping grid (long walks),
+// walks run to the end within the round otherwise.
 hipError_t xmc_launch_transport(const xmc::Params* p, int grid, int block, size_t lds, hipStream_t s, int trace) {
     const bool paths = p->paths.scattering != nullptr || p->paths.det_traj != nullptr;
-    if (trace && paths)
-        hipLaunchKernelGGL((xmc::transport_kernel<true, true>), dim3(grid), dim3(block), lds, s, *p);
-    else if (trace)
-        hipLaunchKernelGGL((xmc::transport_kernel<true, false>), dim3(grid), dim3(block), lds, s, *p);
-    else if (paths)
-        hipLaunchKernelGGL((xmc::transport_kernel<false, true>), dim3(grid), dim3(block), lds, s, *p);
-    else
-        hipLaunchKernelGGL((xmc::transport_kernel<false, false>), dim3(grid), dim3(block), lds, s, *p);
+    const bool interleave = p->dist.cells != nullptr;
+    if (trace && paths) launch_transport<true, true>(p, interleave, grid, block, lds, s);
+    else if (trace) launch_transport<true, false>(p, interleave, grid, block, lds, s);
+    else if (paths) launch_transport<false, true>(p, interleave, grid, block, lds, s);
+    else launch_transport<false, false>(p, interleave, grid, block, lds, s);
     return hipGetLastError();
 }
 
@@ -1099,8 +1213,13 @@ hipError_t xmc_launch_zero(unsigned long long* p, size_t n, hipStream_t s) {
 }
 
 hipError_t xmc_transport_occupancy(int block, size_t lds, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, xmc::transport_kernel<false, false>, block,
-                                                        lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, xmc::transport_kernel<false, false, true>,
+                                                        block, lds);
+}
+
+hipError_t xmc_transport_occupancy_plain(int block, size_t lds, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, xmc::transport_kernel<false, false, false>,
+                                                        block, lds);
 }
 
 hipError_t xmc_launch_batch(const xmc::Params* p, const xmc::Params* cfgs, const uint64_t* prefix, int ncfg, int grid,
