@@ -710,9 +710,16 @@ template <class R> struct Scene {
 
     // GetAreaIndexVector / GetAreaIndex (:190-227); returns -1 outside
     int area_index(const V3<R>& p, int v[3]) const {
-        R ix = area_part[0] * (p.x - area_corner.x) / area_length.x;
-        R iy = area_part[1] * (p.y - area_corner.y) / area_length.y;
-        R iz = area_part[2] * (p.z - area_corner.z) / area_length.z;
+        R ix, iy, iz;
+        if constexpr (sizeof(R) == 4) {  // float: partitions per unit length once per scene
+            ix = (p.x - area_corner.x) * (area_part[0] / area_length.x);
+            iy = (p.y - area_corner.y) * (area_part[1] / area_length.y);
+            iz = (p.z - area_corner.z) * (area_part[2] / area_length.z);
+        } else {
+            ix = area_part[0] * (p.x - area_corner.x) / area_length.x;
+            iy = area_part[1] * (p.y - area_corner.y) / area_length.y;
+            iz = area_part[2] * (p.z - area_corner.z) / area_length.z;
+        }
         bool inside = !((ix < (R)0.0 || ix >= area_part[0]) || (iy < (R)0.0 || iy >= area_part[1]) ||
                         (iz < (R)0.0 || iz >= area_part[2]));
         uint8_t bx = 255, by = 255, bz = 255;
@@ -849,8 +856,14 @@ template <class R> V3<R> new_direction(V3<R> d, R g, Mcg59& rng) {  // :442-473
         tmp = rotate ? std::sqrt(Prec<R>::one_minus_sq(uz)) : (R)0;
     }
     if (rotate) {
-        r.x = st * (ux * uz * cp - uy * sp) / tmp + ux * ct;
-        r.y = st * (uy * uz * cp + ux * sp) / tmp + uy * ct;
+        if constexpr (sizeof(R) == 4) {  // float: one reciprocal for both components
+            R itmp = (R)1.0 / tmp;
+            r.x = st * (ux * uz * cp - uy * sp) * itmp + ux * ct;
+            r.y = st * (uy * uz * cp + ux * sp) * itmp + uy * ct;
+        } else {
+            r.x = st * (ux * uz * cp - uy * sp) / tmp + ux * ct;
+            r.y = st * (uy * uz * cp + ux * sp) / tmp + uy * ct;
+        }
         r.z = -st * cp * tmp + uz * ct;
     } else {
         R tmp = (uz >= (R)0.0) ? (R)1.0 : (R)-1.0;
@@ -874,6 +887,7 @@ template <class R> R step_size(R mua, R mus, Mcg59& rng) {  // :618-631
     do {
         u = Prec<R>::uniform(rng.next());
     } while (u == (R)0.0);
+    if constexpr (sizeof(R) == 4) return -Prec<R>::log_(u) * ((R)1.0 / (mua + mus));  // float: reciprocal per layer
     return -Prec<R>::log_(u) / (mua + mus);
 }
 
@@ -1238,7 +1252,8 @@ template <class R> struct Walker {
             move(p);
             int idx = sc.area_index(p.pos, v);
             if (idx >= 0) {
-                R dw = p.weight * L.mua / (L.mua + L.mus);  // :633-642
+                // :633-642; float: the layer's absorbed fraction mua / (mua + mus) once per layer
+                R dw = sizeof(R) == 4 ? p.weight * (L.mua / (L.mua + L.mus)) : p.weight * L.mua / (L.mua + L.mus);
                 p.weight -= dw;
                 if (!tl.absorption.empty()) tl.absorption[idx] += (double)dw;
                 tl.w_abs += (double)dw;

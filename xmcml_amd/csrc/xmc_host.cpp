@@ -295,7 +295,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     }
 
     // walk-skipping grid: per cell a distance bound (skip walks that cannot hit) and an
-    // entry node (start short walks below the root); about 32 cells per triangle
+    // entry node (start short walks below the root); about 128 cells per triangle
     const bool want_dist = o.noDistanceGrid == 0, want_entry = o.noEntryGrid == 0 && gb.root_ref >= 0;
     if (nt > 256 && (want_dist || want_entry)) {
         xmc::DistanceGrid dg;
@@ -305,7 +305,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
             lo[a] = gb.root_lo[a] - pad;
             hi[a] = gb.root_hi[a] + pad;
         }
-        size_t target = std::min<size_t>(std::max<size_t>((size_t)nt * 32, (size_t)1 << 18), (size_t)16 << 20);
+        size_t target = std::min<size_t>(std::max<size_t>((size_t)nt * 128, (size_t)1 << 18), (size_t)16 << 20);
         xmc::build_distance_grid(in->surface, in->numberOfSurfaces, lo, hi, target, dg);
         xmc::EntryGrid eg;
         if (want_entry) xmc::build_entry_grid(gb, dg.lo, dg.h, dg.dims, eg);
@@ -380,7 +380,10 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     P.area.part[0] = part.x;
     P.area.part[1] = part.y;
     P.area.part[2] = part.z;
-    for (int a = 0; a < 3; a++) P.area.partf[a] = (float)P.area.part[a];
+    for (int a = 0; a < 3; a++) {
+        P.area.partf[a] = (float)P.area.part[a];
+        P.area.scale[a] = P.area.partf[a] / P.area.length[a];
+    }
 
     // detectors (cube bounds in float arithmetic as the float walk evaluates them)
     std::vector<float> cube, ring;

@@ -67,6 +67,7 @@ struct Area {
     float corner[3];
     float length[3];
     float partf[3];
+    float scale[3];  // partf / length, in float
     int part[3];
 };
 

@@ -520,8 +520,9 @@ __device__ __forceinline__ V3 spin(V3 d, float g, uint64_t& x, uint32_t& draws) 
     float s2 = ux * ux + uy * uy;
     if (s2 > 2e-12f) {
         float tmp = __builtin_sqrtf(s2);
-        r.x = st * (ux * uz * cp - uy * sp) / tmp + ux * c;
-        r.y = st * (uy * uz * cp + ux * sp) / tmp + uy * c;
+        float itmp = 1.0f / tmp;
+        r.x = st * (ux * uz * cp - uy * sp) * itmp + ux * c;
+        r.y = st * (uy * uz * cp + ux * sp) * itmp + uy * c;
         r.z = -st * cp * tmp + uz * c;
     } else {
         float tmp = (uz >= 0.0f) ? 1.0f : -1.0f;
@@ -538,11 +539,12 @@ __device__ __forceinline__ V3 spin(V3 d, float g, uint64_t& x, uint32_t& draws) 
     return r;
 }
 
-// GetAreaIndex (:190-227) in float; -1 outside.  Partitions are <= 255.
+// GetAreaIndex (:190-227) in float, with partitions per unit length (scale)
+// in place of the division; -1 outside.  Partitions are <= 255.
 __device__ __forceinline__ int area_index(const Area& A, V3 p) {
-    float ix = A.partf[0] * (p.x - A.corner[0]) / A.length[0];
-    float iy = A.partf[1] * (p.y - A.corner[1]) / A.length[1];
-    float iz = A.partf[2] * (p.z - A.corner[2]) / A.length[2];
+    float ix = (p.x - A.corner[0]) * A.scale[0];
+    float iy = (p.y - A.corner[1]) * A.scale[1];
+    float iz = (p.z - A.corner[2]) * A.scale[2];
     bool inside = !((ix < 0.0f || ix >= A.partf[0]) || (iy < 0.0f || iy >= A.partf[1]) ||
                     (iz < 0.0f || iz >= A.partf[2]));
     if (!inside) return -1;
@@ -743,7 +745,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     uint32_t visited = 0;
     uint64_t x = 0;
     bool alive = false, done = false;
-    float Ln = 1.0f, Lmua = 0.0f, Lmut = 0.0f, Lg = 0.0f;
+    float Ln = 1.0f, Linv = 0.0f, Labs = 0.0f, Lg = 0.0f;  // n, 1/(mua+mus), mua/(mua+mus), g
     bool Lglass = false, Ltarget = false;
     uint32_t draws = 0;
     Recorder<TRACE> rec{nullptr, 0, 0};
@@ -771,8 +773,9 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     auto load_layer = [&](int l) {
         const Layers& L = CFG().layers;
         Ln = L.n[l];
-        Lmua = L.mua[l];
-        Lmut = L.mua[l] + L.mus[l];
+        float mut = L.mua[l] + L.mus[l];
+        Linv = 1.0f / mut;
+        Labs = L.mua[l] / mut;
         Lg = L.g[l];
         Lglass = (L.glass_mask >> l) & 1u;
         Ltarget = (L.target_mask >> l) & 1u;
@@ -867,7 +870,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                     u = next_u(x);
                     draws++;
                 } while (u == 0.0f);
-                step = -xlogf(u) / Lmut;
+                step = -xlogf(u) * Linv;
             }
             const Params& C = CFG();
             h.found = false;  // until a walk finds something within the step
@@ -1034,7 +1037,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             if constexpr (PATHS) path_crossings(C, prev, pos, w, vox, nvox);
             int idx = area_index(C.area, pos);
             if (idx >= 0) {
-                float dw = w * Lmua / Lmut;  // ComputeDroppedWeightOfPhoton (:633-642)
+                float dw = w * Labs;  // ComputeDroppedWeightOfPhoton (:633-642)
                 w -= dw;
                 if (C.tally.absorption) unsafeAtomicAdd(C.tally.absorption + idx, (double)dw);
                 w_abs += (double)dw;
