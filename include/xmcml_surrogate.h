@@ -96,9 +96,10 @@ int xmc_mlp_train_step(xmc_mlp* m, const float* x_dev, const float* y_dev, const
 
 /* C[M][N] = sum_k A[m][k] * B[n][k] with bf16 operands given as fp32 host
  * arrays (rounded to bf16 on upload) and fp32 accumulation on the matrix
- * cores; M, N and K multiples of 64.  A check of the matrix
- * kernel alone. */
-int xmc_mlp_gemm_check(const float* a, const float* b, int M, int N, int K, float* c, int device);
+ * cores; M, N and K multiples of 64.  tile = 128 forces the 128 x 128 tile
+ * kernel (M, N multiples of 128), 0 the automatic choice.  A check of the
+ * matrix kernels alone. */
+int xmc_mlp_gemm_check(const float* a, const float* b, int M, int N, int K, float* c, int tile, int device);
 
 #ifdef __cplusplus
 }

@@ -25,20 +25,24 @@ def dev(engine):
     return engine.DeviceArray
 
 
-def test_gemm_matches_reference(dev):
+@pytest.mark.parametrize("tile", [0, 128])
+def test_gemm_matches_reference(dev, tile):
     from xmcml_amd.surrogate import gemm_check
     rng = np.random.default_rng(0)
-    for (M, N, K) in [(64, 64, 64), (128, 192, 128), (256, 64, 512)]:
+    shapes = [(64, 64, 64), (128, 192, 128), (256, 64, 512)] if tile == 0 else \
+        [(128, 128, 64), (256, 384, 128), (128, 256, 512)]
+    for (M, N, K) in shapes:
         a = rng.standard_normal((M, K)).astype(np.float32)
         b = rng.standard_normal((N, K)).astype(np.float32)
-        c = gemm_check(a, b)
+        c = gemm_check(a, b, tile)
         ref = R.bf16(a).astype(np.float64) @ R.bf16(b).astype(np.float64).T
         bound = (np.abs(R.bf16(a)).astype(np.float64) @ np.abs(R.bf16(b)).astype(np.float64).T) * K * 2.0 ** -24
         assert np.all(np.abs(c - ref) <= bound + 1e-30), (M, N, K, np.max(np.abs(c - ref) / bound))
         # exact on integer-valued operands (every partial sum is representable)
         ai = rng.integers(-8, 9, (M, K)).astype(np.float32)
         bi = rng.integers(-8, 9, (N, K)).astype(np.float32)
-        assert np.array_equal(gemm_check(ai, bi), (ai.astype(np.float64) @ bi.T.astype(np.float64)).astype(np.float32))
+        assert np.array_equal(gemm_check(ai, bi, tile),
+                              (ai.astype(np.float64) @ bi.T.astype(np.float64)).astype(np.float32))
 
 
 def _model(hidden=128, depth=2, outputs=32, max_batch=256, seed=3):
@@ -76,6 +80,29 @@ def test_forward_matches_reference(dev):
     out2 = m.forward(dev(big.shape, big)).numpy()
     assert np.array_equal(out2[:200], out) and np.array_equal(out2[200:400], out)
     assert np.array_equal(out2[400:], out[:100])
+
+
+def test_large_batch_forward_and_backward(dev):
+    """Batches large enough for the 128 x 128 tile kernel (bias + GELU forward,
+    GELU' backward with a transposed weight operand) against the reference."""
+    from xmcml_amd.surrogate import MLP
+    n, hidden = 16384, 256
+    m = MLP(30, hidden, 2, 32, n, seed=5)
+    p = m.get_params()
+    x, y, mask, (xt, yt, mt) = _batch(dev, n, 32, seed=6)
+    out = m.forward(xt).numpy()
+    ref_net = R.RefMLP(p, 30, hidden, 2, 32)
+    ref, _ = ref_net.forward(x)
+    assert np.linalg.norm(out - ref) <= 5e-3 * np.linalg.norm(ref)
+    scale = 1.0 / mask.sum()
+    loss = m.backward(xt, yt, mt, scale)
+    rloss, _, rgrads = ref_net.loss_and_grads(x, y, mask, scale)
+    assert loss == pytest.approx(rloss, rel=2e-3)
+    for l, ((gW, gb), (rW, rb)) in enumerate(zip(R.unpack(m.get_grads(), R.layer_shapes(30, hidden, 2, 32)),
+                                                 rgrads)):
+        for name, a, b in (("W", gW, rW), ("b", gb, rb)):
+            rel, cos = _cmp(a, b)
+            assert rel <= 5e-3 and cos >= 0.9999, (l, name, rel, cos)
 
 
 def _cmp(g, r):

@@ -54,7 +54,13 @@ __host__ __device__ inline float bf2f(uint16_t h) {
     return f;
 }
 
-__device__ inline float tanh_f(float y) { return 1.0f - 2.0f / (1.0f + expf(2.0f * y)); }
+// tanh(y) = 1 - 2 / (1 + e^{2y}) with the hardware exponential and
+// reciprocal (v_exp_f32, v_rcp_f32, about 1 ulp each): the GELU epilogues
+// run once per output element, and exact libm calls there cost more VALU
+// time than the matrix products they follow
+__device__ inline float tanh_f(float y) {
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * y));
+}
 
 __device__ inline float gelu(float z) { return 0.5f * z * (1.0f + tanh_f(kGeluC * (z + kGeluA * z * z * z))); }
 
@@ -113,6 +119,33 @@ __device__ inline void store_pieces(uint16_t* lds, const u32x4 (&v)[2], int t) {
     }
 }
 
+// The accumulator tile of one wave (32 x 32, v_mfma_f32_32x32x16_bf16 layout:
+// element v of a lane is row 8*(v/4) + 4*(lane/32) + v%4, column lane%32)
+// through the epilogue, at rows m_base.., columns n_base..
+template <int EPI>
+__device__ __forceinline__ void epilogue(const GemmArgs& g, const f32x16& acc, int m_base, int n_base, int lane) {
+    const int n = n_base + (lane & 31);
+    float bias = 0.0f;
+    if constexpr (EPI == EPI_GELU || EPI == EPI_LINEAR) bias = g.bias[n];
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int m = m_base + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const long o = (long)m * g.ldo + n;
+        float x = acc[v];
+        if constexpr (EPI == EPI_GELU) {
+            float z = x + bias;
+            g.zout[o] = f2bf(z);
+            g.out[o] = f2bf(gelu(z));
+        } else if constexpr (EPI == EPI_LINEAR) {
+            g.outf[o] = x + bias;
+        } else if constexpr (EPI == EPI_DGELU) {
+            g.out[o] = f2bf(x * gelu_grad(bf2f(g.zin[o])));
+        } else {
+            g.outf[(long)blockIdx.z * g.M * g.ldo + o] = x;
+        }
+    }
+}
+
 template <bool AK, bool BKC, int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     // two LDS stages: the tile being multiplied and the next one being stored
@@ -158,27 +191,106 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
         stage ^= 1;
     }
 
-    // accumulator element v of a lane: row 8*(v/4) + 4*(lane/32) + v%4, column lane%32
-    const int n = n0 + wn + lr;
-    float bias = 0.0f;
-    if constexpr (EPI == EPI_GELU || EPI == EPI_LINEAR) bias = g.bias[n];
+    epilogue<EPI>(g, acc, m0 + wm, n0 + wn, lane);
+}
+
+// 128 x 128 tiles for the large products (both dimensions multiples of 128):
+// each wave computes a 64 x 64 quarter as 2 x 2 MFMA tiles, so every
+// operand fragment read from LDS feeds two MFMAs.  32-deep k stages, two LDS
+// buffers (41 KB), next stage's global reads in flight during the products.
+constexpr int BM2 = 128, BN2 = 128, BK2 = 32, LDS_STRIDE2 = BK2 + 8;
+
+template <bool AK>
+__device__ inline void load_pieces2(u32x4 (&v)[2], const uint16_t* src, long ld, int r0, int k0, int t) {
+    // 128 rows x 32 k = 512 pieces of 8: two per thread
 #pragma unroll
-    for (int v = 0; v < 16; v++) {
-        const int m = m0 + wm + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        const long o = (long)m * g.ldo + n;
-        float x = acc[v];
-        if constexpr (EPI == EPI_GELU) {
-            float z = x + bias;
-            g.zout[o] = f2bf(z);
-            g.out[o] = f2bf(gelu(z));
-        } else if constexpr (EPI == EPI_LINEAR) {
-            g.outf[o] = x + bias;
-        } else if constexpr (EPI == EPI_DGELU) {
-            g.out[o] = f2bf(x * gelu_grad(bf2f(g.zin[o])));
+    for (int j = 0; j < 2; j++) {
+        int p = t + 256 * j;
+        if constexpr (AK) {
+            int a = p >> 2, c = (p & 3) * 8;
+            v[j] = *(const u32x4*)(src + (long)(r0 + a) * ld + k0 + c);
         } else {
-            g.outf[(long)blockIdx.z * g.M * g.ldo + o] = x;
+            int a = p >> 4, c = (p & 15) * 8;  // 32 k-rows of 16 pieces along the rows
+            v[j] = *(const u32x4*)(src + (long)(k0 + a) * ld + r0 + c);
         }
     }
+}
+
+template <bool AK>
+__device__ inline void store_pieces2(uint16_t* lds, const u32x4 (&v)[2], int t) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        int p = t + 256 * j;
+        if constexpr (AK) {
+            int a = p >> 2, c = (p & 3) * 8;
+            *(u32x4*)(lds + a * LDS_STRIDE2 + c) = v[j];
+        } else {
+            int a = p >> 4, c = (p & 15) * 8;
+            uint16_t h[8];
+            std::memcpy(h, &v[j], 16);
+#pragma unroll
+            for (int i = 0; i < 8; i++) lds[(c + i) * LDS_STRIDE2 + a] = h[i];
+        }
+    }
+}
+
+template <bool AK, bool BKC, int EPI>
+__global__ __launch_bounds__(256) void gemm128_kernel(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][BM2 * LDS_STRIDE2];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN2 * LDS_STRIDE2];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m0 = blockIdx.y * BM2, n0 = blockIdx.x * BN2;
+    const int kb = blockIdx.z * g.kchunk, ke = kb + g.kchunk;
+    const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+    const int lr = lane & 31, lk = (lane >> 5) * 8;
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[i][j][v] = 0.0f;
+
+    u32x4 ra[2], rb[2];
+    load_pieces2<AK>(ra, g.A, g.lda, m0, kb, t);
+    load_pieces2<BKC>(rb, g.B, g.ldb, n0, kb, t);
+    store_pieces2<AK>(As[0], ra, t);
+    store_pieces2<BKC>(Bs[0], rb, t);
+    __syncthreads();
+    int stage = 0;
+    for (int k0 = kb; k0 < ke; k0 += BK2) {
+        const bool more = k0 + BK2 < ke;
+        if (more) {
+            load_pieces2<AK>(ra, g.A, g.lda, m0, k0 + BK2, t);
+            load_pieces2<BKC>(rb, g.B, g.ldb, n0, k0 + BK2, t);
+        }
+        const uint16_t* as = As[stage];
+        const uint16_t* bs = Bs[stage];
+#pragma unroll
+        for (int kk = 0; kk < BK2; kk += 16) {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                a[i] = __builtin_bit_cast(bf16x8, *(const u32x4*)(as + (wm + 32 * i + lr) * LDS_STRIDE2 + kk + lk));
+                b[i] = __builtin_bit_cast(bf16x8, *(const u32x4*)(bs + (wn + 32 * i + lr) * LDS_STRIDE2 + kk + lk));
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            store_pieces2<AK>(As[stage ^ 1], ra, t);
+            store_pieces2<BKC>(Bs[stage ^ 1], rb, t);
+        }
+        __syncthreads();
+        stage ^= 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) epilogue<EPI>(g, acc[i][j], m0 + wm + 32 * i, n0 + wn + 32 * j, lane);
 }
 
 // x[b][0..nin) fp32 -> bf16 rows of width ldx, zero padding beyond nin and batch
@@ -308,10 +420,18 @@ int fail(int code, const std::string& msg) { return xmc::set_error(code, msg); }
 
 inline int round_up(int x, int q) { return (x + q - 1) / q * q; }
 
+// 128 x 128 tiles when both dimensions allow and there are enough tiles to
+// fill the CUs, 64 x 64 otherwise
 template <bool AK, bool BKC, int EPI>
-hipError_t launch_gemm(const GemmArgs& g, int splits, hipStream_t s) {
-    dim3 grid(g.N / BN, g.M / BM, splits);
-    hipLaunchKernelGGL((gemm_kernel<AK, BKC, EPI>), grid, dim3(256), 0, s, g);
+hipError_t launch_gemm(const GemmArgs& g, int splits, hipStream_t s, bool force128 = false) {
+    if (EPI != EPI_PARTIAL && g.M % BM2 == 0 && g.N % BN2 == 0 && (g.kchunk % BK2) == 0 &&
+        (long)(g.M / BM2) * (g.N / BN2) * splits >= (force128 ? 1 : 256)) {
+        dim3 grid(g.N / BN2, g.M / BM2, splits);
+        hipLaunchKernelGGL((gemm128_kernel<AK, BKC, EPI>), grid, dim3(256), 0, s, g);
+    } else {
+        dim3 grid(g.N / BN, g.M / BM, splits);
+        hipLaunchKernelGGL((gemm_kernel<AK, BKC, EPI>), grid, dim3(256), 0, s, g);
+    }
     return hipGetLastError();
 }
 
@@ -390,7 +510,9 @@ int refresh_mirror(xmc_mlp* m, hipStream_t s) {
 }
 
 // forward of `batch` rows (<= rows) already packed into a0
-int forward_packed(xmc_mlp* m, int rows, hipStream_t s) {
+// head_out: where the fp32 head rows go (the model's buffer, or the caller's
+// when every one of the `rows` rows is the caller's)
+int forward_packed(xmc_mlp* m, int rows, hipStream_t s, float* head_out) {
     const uint16_t* prev = m->a0;
     long ldp = m->in_pad;
     for (int l = 0; l < m->layers; l++) {
@@ -411,7 +533,7 @@ int forward_packed(xmc_mlp* m, int rows, hipStream_t s) {
             prev = m->a[l];
             ldp = m->d.hidden;
         } else {
-            g.outf = m->out;
+            g.outf = head_out;
             g.ldo = m->head;
             MLP_TRY((launch_gemm<true, true, EPI_LINEAR>(g, 1, s)));
         }
@@ -463,7 +585,7 @@ int backward_impl(xmc_mlp* m, const float* x, const float* y, const float* mask,
     int rows = round_up(batch, BM);
     int rc = pack(m, x, batch, rows, s);
     if (rc) return rc;
-    if ((rc = forward_packed(m, rows, s))) return rc;
+    if ((rc = forward_packed(m, rows, s, m->out))) return rc;
     long n = (long)rows * m->d.outputs;
     int blocks = (int)((n + 255) / 256);
     hipLaunchKernelGGL(loss_kernel, dim3(blocks), dim3(256), 0, s, m->out, y, mask, batch, m->d.outputs, rows,
@@ -697,9 +819,10 @@ int xmc_mlp_forward(xmc_mlp* m, const float* x, int batch, float* out, void* str
         int rows = round_up(nb, BM);
         int rc = pack(m, x + (long)b0 * m->d.inputs, nb, rows, s);
         if (rc) return rc;
-        if ((rc = forward_packed(m, rows, s))) return rc;
-        MLP_TRY(hipMemcpyAsync(out + (long)b0 * m->head, m->out, (size_t)nb * m->head * 4,
-                               hipMemcpyDeviceToDevice, s));
+        float* dst = out + (long)b0 * m->head;
+        if ((rc = forward_packed(m, rows, s, nb == rows ? dst : m->out))) return rc;
+        if (nb != rows)
+            MLP_TRY(hipMemcpyAsync(dst, m->out, (size_t)nb * m->head * 4, hipMemcpyDeviceToDevice, s));
     }
     return 0;
 }
@@ -774,7 +897,7 @@ int xmc_dev_sync(int device) {
     return 0;
 }
 
-int xmc_mlp_gemm_check(const float* a, const float* b, int M, int N, int K, float* c, int device) {
+int xmc_mlp_gemm_check(const float* a, const float* b, int M, int N, int K, float* c, int tile, int device) {
     if (!a || !b || !c) return fail(-1, "NULL argument");
     if (M <= 0 || N <= 0 || K <= 0 || M % BM || N % BN || K % BK)
         return fail(-1, "M, N and K must be multiples of 64");
@@ -806,7 +929,8 @@ int xmc_mlp_gemm_check(const float* a, const float* b, int M, int N, int K, floa
         g.bias = bias;
         g.outf = dc;
         g.ldo = N;
-        e = launch_gemm<true, true, EPI_LINEAR>(g, 1, nullptr);
+        if (tile == 128 && (M % BM2 || N % BN2)) e = hipErrorInvalidValue;
+        else e = launch_gemm<true, true, EPI_LINEAR>(g, 1, nullptr, tile == 128);
     }
     if (e == hipSuccess) e = hipMemcpy(c, dc, (size_t)M * N * 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess) rc = fail(-2, std::string("gemm check: ") + hipGetErrorString(e));

@@ -95,7 +95,7 @@ def lib():
     sig("xmc_mlp_get_grads", _I, [_P, _PF, C.c_int64])
     sig("xmc_mlp_adam", _I, [_P, _F, _F, _F, _F, _F, _P])
     sig("xmc_mlp_train_step", _I, [_P, _P, _P, _P, _I, _F, _F, _F, _F, _F, _F, _P])
-    sig("xmc_mlp_gemm_check", _I, [_PF, _PF, _I, _I, _I, _PF, _I])
+    sig("xmc_mlp_gemm_check", _I, [_PF, _PF, _I, _I, _I, _PF, _I, _I])
     sig("xmc_mlp_train_sampled", _I, [_P, _P, _P, _P, C.c_int64, _I, _U64, C.c_int64, _F, _F, _F, _F, _F, _F,
                                       _PF, _P])
     sig("xmc_dev_alloc", _I, [_I, _U64, C.POINTER(_P)])

@@ -221,8 +221,9 @@ class MLP:
         return loss.value if want_loss else None
 
 
-def gemm_check(a: np.ndarray, b: np.ndarray, device: int = 0) -> np.ndarray:
-    """C = A B^T on the matrix-core kernel (bf16 operands, fp32 accumulate)."""
+def gemm_check(a: np.ndarray, b: np.ndarray, tile: int = 0, device: int = 0) -> np.ndarray:
+    """C = A B^T on the matrix-core kernel (bf16 operands, fp32 accumulate);
+    tile = 128 forces the 128 x 128 tile kernel."""
     a = np.ascontiguousarray(a, np.float32)
     b = np.ascontiguousarray(b, np.float32)
     M, K = a.shape
@@ -230,7 +231,7 @@ def gemm_check(a: np.ndarray, b: np.ndarray, device: int = 0) -> np.ndarray:
     c = np.zeros((M, N), np.float32)
     pf = C.POINTER(C.c_float)
     engine._check(engine.lib().xmc_mlp_gemm_check(a.ctypes.data_as(pf), b.ctypes.data_as(pf), M, N, K,
-                                                  c.ctypes.data_as(pf), device), "xmc_mlp_gemm_check")
+                                                  c.ctypes.data_as(pf), tile, device), "xmc_mlp_gemm_check")
     return c
 
 
