@@ -60,7 +60,14 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 
 // dynamic LDS of the transport kernels: short stacks, then the block's ledger
-size_t transport_lds(int block) { return (size_t)xmc::kStackLds * block * sizeof(int2) + 8 * sizeof(double); }
+size_t transport_lds(int block, int tile_count = 0) {
+    return (size_t)xmc::kStackLds * block * sizeof(int2) + 8 * sizeof(double) + (size_t)tile_count * sizeof(double);
+}
+
+// LDS absorption tile: at most this many voxels per block (24 kB; four 256-thread
+// blocks per CU with their stacks -- measured faster on the slab than 13.5 kB at
+// five blocks: 4.2e7 vs 3.8e7 photons/s, against 2.3e7 without a tile)
+constexpr int kAbsTileVoxels = 3072;
 
 template <class T>
 struct DevBuf {
@@ -205,7 +212,7 @@ static int configure_launch(int device, int depth, int block, int blocks_per_cu,
                             DevBuf<int2>& spill, xmc::Params& P) {
     if (block <= 0) block = 256;
     if (block % 64 != 0 || block > 256) return fail(XMC_ERR_ARG, "blockSize must be 64, 128, 192 or 256");
-    size_t lds = transport_lds(block);
+    size_t lds = transport_lds(block, P.abs_tile.count);
     int per_cu = 0;
     HIP_TRY(kernel_occupancy(block, lds, &per_cu));
     if (per_cu < 1) per_cu = 1;
@@ -476,6 +483,30 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     P.tally.stats = s + sc->off_st;
     P.queue = s + sc->off_q;
 
+    // LDS absorption tile: up to kAbsTileVoxels voxels from the beam's entry voxel down,
+    // centred on it across (most of the weight is deposited within a few voxels of it)
+    std::memset(&P.abs_tile, 0, sizeof(P.abs_tile));
+    if (launchable && sc->absorption && P.dist.cells == nullptr) {  // the kernel variant without the grid
+        const xmc_area* ar = in->area;
+        const int pn[3] = {part.x, part.y, part.z};
+        const double sp[3] = {in->startPosition.x, in->startPosition.y, in->startPosition.z};
+        const double co[3] = {ar->corner.x, ar->corner.y, ar->corner.z};
+        const double ln[3] = {ar->length.x, ar->length.y, ar->length.z};
+        int size[3];
+        size[2] = std::min(pn[2], 12);
+        int across = (int)std::sqrt((double)kAbsTileVoxels / size[2]);
+        size[0] = std::min(pn[0], across);
+        size[1] = std::min(pn[1], across);
+        for (int a = 0; a < 3; a++) {
+            int c = (int)std::floor(pn[a] * (sp[a] - co[a]) / ln[a]);
+            c = std::min(std::max(c, 0), pn[a] - 1);
+            int lo = a < 2 ? c - size[a] / 2 : c - 1;
+            P.abs_tile.lo[a] = std::min(std::max(lo, 0), pn[a] - size[a]);
+            P.abs_tile.size[a] = size[a];
+        }
+        P.abs_tile.count = size[0] * size[1] * size[2];
+    }
+
     // scattering grid and detector trajectory maps (mcml_kernel.cpp:750-810)
     if (o.trajectoryCapacity < 0) return fail(XMC_ERR_ARG, "trajectoryCapacity must be >= 0");
     std::memset(&P.paths, 0, sizeof(P.paths));
@@ -567,7 +598,7 @@ static int run_impl(xmc_scene* sc, uint64_t seed, uint64_t first, uint64_t n, hi
     P.seed_value = (2 * seed + 1) & xmc::kMcgMask;
     if (tr) P.trace = *tr;
     HIP_TRY(xmc_launch_zero(P.queue, 1, s));
-    size_t lds = transport_lds(sc->block);
+    size_t lds = transport_lds(sc->block, P.abs_tile.count);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

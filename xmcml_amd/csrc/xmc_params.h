@@ -99,6 +99,15 @@ struct DistGrid {
     const float* entry_box;
 };
 
+// A box of absorption voxels (around the beam entry, where most weight is
+// deposited) that each block accumulates in LDS and adds to the grid once at
+// the end; deposits elsewhere go straight to the grid.  count = 0: no tile.
+struct AbsTile {
+    int lo[3];
+    int size[3];
+    int count;
+};
+
 struct Params {
     const GpuNode* nodes;
     const GpuTri* tris;
@@ -126,6 +135,7 @@ struct Params {
     int spill_depth;  // entries per lane in the spill
     Trace trace;
     Paths paths;
+    AbsTile abs_tile;
 };
 
 struct QueryParams {

@@ -541,15 +541,22 @@ __device__ __forceinline__ V3 spin(V3 d, float g, uint64_t& x, uint32_t& draws) 
 
 // GetAreaIndex (:190-227) in float, with partitions per unit length (scale)
 // in place of the division; -1 outside.  Partitions are <= 255.
-__device__ __forceinline__ int area_index(const Area& A, V3 p) {
+__device__ __forceinline__ int area_index(const Area& A, V3 p, int& bx, int& by, int& bz) {
     float ix = (p.x - A.corner[0]) * A.scale[0];
     float iy = (p.y - A.corner[1]) * A.scale[1];
     float iz = (p.z - A.corner[2]) * A.scale[2];
     bool inside = !((ix < 0.0f || ix >= A.partf[0]) || (iy < 0.0f || iy >= A.partf[1]) ||
                     (iz < 0.0f || iz >= A.partf[2]));
     if (!inside) return -1;
-    int bx = (int)ix, by = (int)iy, bz = (int)iz;
+    bx = (int)ix;
+    by = (int)iy;
+    bz = (int)iz;
     return bx * A.part[1] * A.part[2] + by * A.part[2] + bz;
+}
+
+__device__ __forceinline__ int area_index(const Area& A, V3 p) {
+    int bx, by, bz;
+    return area_index(A, p, bx, by, bz);
 }
 
 // GetPlaneSegmentIntersectionPoint (mcml_math.cpp:59-94) for the plane
@@ -729,6 +736,12 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
              P.spill_depth, 0, 0};
     double* ledger = reinterpret_cast<double*>(lds_stack + kStackLds * blockDim.x);
     if (tid < kLedger) ledger[tid] = 0.0;
+    // the block's absorption tile follows the ledger in LDS
+    double* tile = ledger + kLedger;
+    const AbsTile& T = P.abs_tile;
+    constexpr bool kTile = !BATCHED && !INTERLEAVE;  // the slab-like scenes where deposits crowd
+    if constexpr (kTile)
+        for (int i = tid; i < T.count; i += blockDim.x) tile[i] = 0.0;
     __syncthreads();
     double w_abs = 0.0;
     // never take P's address: that would move the kernel-argument struct
@@ -1035,11 +1048,21 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             pos.y += step * dir.y;
             pos.z += step * dir.z;
             if constexpr (PATHS) path_crossings(C, prev, pos, w, vox, nvox);
-            int idx = area_index(C.area, pos);
+            int bx, by, bz;
+            int idx = area_index(C.area, pos, bx, by, bz);
             if (idx >= 0) {
                 float dw = w * Labs;  // ComputeDroppedWeightOfPhoton (:633-642)
                 w -= dw;
-                if (C.tally.absorption) unsafeAtomicAdd(C.tally.absorption + idx, (double)dw);
+                if (C.tally.absorption) {
+                    bool in_tile = false;
+                    if constexpr (kTile) {
+                        unsigned tx = (unsigned)(bx - T.lo[0]), ty = (unsigned)(by - T.lo[1]),
+                                 tz = (unsigned)(bz - T.lo[2]);
+                        in_tile = tx < (unsigned)T.size[0] && ty < (unsigned)T.size[1] && tz < (unsigned)T.size[2];
+                        if (in_tile) atomicAdd(tile + ((int)tz * T.size[1] + (int)ty) * T.size[0] + (int)tx, (double)dw);
+                    }
+                    if (!in_tile) unsafeAtomicAdd(C.tally.absorption + idx, (double)dw);
+                }
                 w_abs += (double)dw;
                 dir = spin(dir, Lg, x, draws);
                 n_scatter++;
@@ -1095,6 +1118,16 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     atomicAdd(ledger + kAbs, w_abs);
     __syncthreads();
     if (tid < kLedger) unsafeAtomicAdd(reinterpret_cast<double*>(P.tally.stats + 8) + tid, ledger[tid]);
+    if constexpr (kTile) {
+        const Area& A = P.area;
+        for (int i = tid; i < T.count; i += blockDim.x) {
+            double v = tile[i];
+            if (v == 0.0) continue;
+            int tx = i % T.size[0], ty = (i / T.size[0]) % T.size[1], tz = i / (T.size[0] * T.size[1]);
+            int g = ((T.lo[0] + tx) * A.part[1] + (T.lo[1] + ty)) * A.part[2] + (T.lo[2] + tz);
+            unsafeAtomicAdd(P.tally.absorption + g, v);
+        }
+    }
 }
 
 template <bool TRACE, bool PATHS, bool INTERLEAVE>
