@@ -41,6 +41,68 @@ PEAK_FP32_NONPACKED_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
+WORKLOADS = {
+    "heightfield": "3-D surface MC: 3 conformal heightfield boundaries (128x128 grid, 40x40 mm) + flat bottom, "
+                   "98,306 triangles; 3 layers; pencil beam; roulette 1e-4/0.1; 20 rings + 256-bin TOF 0-2 ns",
+    "head": "heavy geometry: 5 nested ellipsoids (90/75/65 mm outer, shells 5/7/2/3 mm), icosphere level 7 with "
+            "1 mm seeded radial noise, 1,638,400 triangles; scalp/skull/CSF/grey/white; pencil beam; roulette "
+            "1e-4/0.1; rings at 10-40 mm + 256-bin TOF 0-5 ns",
+    "slab": "planar 3-layer slab (0.1/2/20 mm), pencil beam, roulette 1e-4/0.1, 20 rings of 0.5 mm, 100^3 "
+            "absorption grid over 20x20x20 mm, 50-bin TOF",
+    "batch": "1024 layered configurations of 2-5 layers (parameters and log-uniform photon budgets in [1e5, 1e7] "
+             "from seed 3), 32 rings + 64 TOF bins each, one persistent launch replayed as a HIP graph",
+}
+
+
+def bench_batch(args) -> int:
+    """The batched configurations: every budget in one persistent launch (a
+    captured HIP graph); value = photons/s of the launch, plus configs/s."""
+    from xmcml_amd import engine, scenes
+    import oracle_lib as ol
+    from xmcml_amd.abi import InputHandle, OutputHandle
+
+    specs = scenes.batched_configs(1024, photon_scale=args.batch_scale)
+    total = sum(s.photons for s in specs)
+    print(f"bench: batch of {len(specs)} configurations, {total:.3e} photons", file=sys.stderr, flush=True)
+    b = engine.Batch(specs)
+    b.run(args.seed)  # warm-up launch (and graph capture); the timed launch replays the same graph
+    b.collect()
+    b.stats()
+    t0 = time.perf_counter()
+    b.run(args.seed)
+    outs = b.collect()
+    elapsed = time.perf_counter() - t0
+    st = b.stats()
+    b.close()
+    value = total / elapsed
+    line = {
+        "metric": "photons/s (batched configurations)", "value": value, "unit": "photons/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": elapsed * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 transport, fp64 tallies", "data": "synthetic",
+        "config": {"workload": WORKLOADS["batch"], "configurations": len(specs), "photons": total},
+        "configs_per_s": len(specs) / elapsed, "kernel_ms": st["kernelMs"],
+        "scatter_events_per_photon": st["scatterEvents"] / total,
+    }
+    if args.cpu_seconds > 0:
+        # the double oracle on a sample: 32 configurations, 20,000 photons each, all host threads
+        threads = host_threads()
+        photons = seconds = 0.0
+        t_end = time.perf_counter() + args.cpu_seconds
+        k = 0
+        while time.perf_counter() < t_end and k < 256:
+            sp = specs[(k * 37) % len(specs)]
+            o = OutputHandle(sp, absorption=False)
+            c = ol.run_oracle(sp, InputHandle(sp), o, args.seed, 0, 20000, 64, threads)
+            photons += c.photons
+            seconds += c.seconds
+            k += 1
+        line["cpu_baseline"] = {"value": photons / seconds, "unit": "photons/s", "cores": threads, "kind": "port",
+                                "sample": f"{k} configurations of the batch x 20,000 photons, double precision"}
+        line["speedup_vs_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def host_threads() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -48,7 +110,7 @@ def host_threads() -> int:
         return os.cpu_count() or 1
 
 
-def cpu_baseline(spec, target_seconds: float, seed: int) -> dict:
+def cpu_baseline(spec, target_seconds: float, seed: int, absorption: bool = False) -> dict:
     """The double-precision oracle on all host threads, on a bounded sample."""
     import oracle_lib as ol
     from xmcml_amd.abi import InputHandle, OutputHandle
@@ -56,11 +118,11 @@ def cpu_baseline(spec, target_seconds: float, seed: int) -> dict:
     threads = host_threads()
     inp = InputHandle(spec)
     n = max(200, 20 * threads)
-    out = OutputHandle(spec, absorption=False)
+    out = OutputHandle(spec, absorption=absorption)
     c = ol.run_oracle(spec, inp, out, seed, 0, n, 64, threads)  # warm-up, not counted
     rate = c.photons / max(c.seconds, 1e-9)
     first, photons, seconds, scatter = n, 0, 0.0, 0
-    out = OutputHandle(spec, absorption=False)
+    out = OutputHandle(spec, absorption=absorption)
     while seconds < target_seconds:
         chunk = int(max(n, rate * (target_seconds - seconds)))
         c = ol.run_oracle(spec, inp, out, seed, first + photons, chunk, 64, threads)
@@ -188,9 +250,11 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=int, default=10_000_000, help="photons per step")
-    ap.add_argument("--workload", choices=["heightfield", "head"], default="heightfield",
+    ap.add_argument("--workload", choices=["heightfield", "head", "slab", "batch"], default="heightfield",
                     help="heightfield: the 3-D surface configuration (the metric's workload); head: five nested "
-                         "noisy ellipsoids of 327,680 triangles each")
+                         "noisy ellipsoids of 327,680 triangles each; slab: the planar 3-layer configuration with "
+                         "its 100^3 absorption grid; batch: the 1024 layered configurations in one launch")
+    ap.add_argument("--batch-scale", type=float, default=1.0, help="batch workload: photon budgets times this")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--seed", type=int, default=777)
@@ -210,11 +274,13 @@ def main(argv=None) -> int:
 
     from xmcml_amd import engine, scenes
 
-    make = (lambda n: scenes.heightfield_scene(n)) if args.workload == "heightfield" else \
-        (lambda n: scenes.head_scene(n))
+    if args.workload == "batch":
+        return bench_batch(args)
+    make = {"heightfield": scenes.heightfield_scene, "head": scenes.head_scene, "slab": scenes.slab_scene}[args.workload]
+    absorption = args.workload == "slab"
     spec = make(args.photons)
     print(f"bench: building scene ({spec.num_triangles} triangles)", file=sys.stderr, flush=True)
-    sc = engine.Scene(spec, absorption=False, block_size=args.block)
+    sc = engine.Scene(spec, absorption=absorption, block_size=args.block)
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib << 20)
         print(f"bench: calibration copy of {args.copy_calibration_mib} MiB done", file=sys.stderr, flush=True)
@@ -271,12 +337,7 @@ def main(argv=None) -> int:
         "dtype": "fp32 transport, fp64 tallies",
         "data": "synthetic",
         "config": {
-            "workload": ("3-D surface MC: 3 conformal heightfield boundaries (128x128 grid, 40x40 mm) + flat "
-                         "bottom, 98,306 triangles; 3 layers; pencil beam; roulette 1e-4/0.1; 20 rings + "
-                         "256-bin TOF 0-2 ns") if args.workload == "heightfield" else
-                        ("heavy geometry: 5 nested ellipsoids (90/75/65 mm outer, shells 5/7/2/3 mm), "
-                         "icosphere level 7 with 1 mm seeded radial noise, 1,638,400 triangles; scalp/skull/"
-                         "CSF/grey/white; pencil beam; roulette 1e-4/0.1; rings at 10-40 mm + 256-bin TOF 0-5 ns"),
+            "workload": WORKLOADS[args.workload],
             "photons_per_step": args.photons,
             "triangles": spec.num_triangles,
             "parallelism": "single GPU, persistent kernel",
@@ -288,7 +349,7 @@ def main(argv=None) -> int:
     sc.close()
     if args.cpu_seconds > 0:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
-        cb = cpu_baseline(make(0), args.cpu_seconds, args.seed + 1)
+        cb = cpu_baseline(make(0), args.cpu_seconds, args.seed + 1, absorption)
         line["cpu_baseline"] = cb
         line["speedup_vs_cpu"] = value / cb["value"]
     else:

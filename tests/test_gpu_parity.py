@@ -282,3 +282,51 @@ def test_walk_skipping_grids_change_nothing(engine, grids):
     assert np.array_equal(a.bin_count(), b.bin_count())
     np.testing.assert_allclose(a.absorption, b.absorption, rtol=1e-10, atol=1e-12)
     np.testing.assert_allclose(a.det_weight(), b.det_weight(), rtol=1e-11, atol=1e-12)
+
+
+def test_heightfield_statistics_at_1e8_photons(engine):
+    """The 3-D surface configuration at its full 1e8 photons on the GPU (seed
+    1, twenty launches of 5e6) against the double oracle (seed 2, forty
+    samples of 10,000 photons), independent streams.  Ring bins, total
+    reflectance and absorption: within three combined standard errors, R and A
+    within 0.5 %.  The 80-odd populated TOF bins (summed over rings) are tested
+    as a family: with standard errors estimated from 20 and 40 samples, a few
+    |z| above 3 are expected by chance (a t-like tail), so at least 97 % must
+    lie within 3, none beyond 4.5, and their mean z^2 must be chi-square-like
+    (< 1.5)."""
+    spec = scenes.heightfield_scene(0)
+    inp = InputHandle(spec)
+    nb_g, n_gpu, nb_c, n_cpu = 20, 5_000_000, 40, 10_000
+
+    def vec(det_w, bins_w, absorbed, n):
+        return np.concatenate([det_w / n, bins_w.sum(axis=0) / n, [det_w.sum() / n, absorbed / n]])
+
+    G = []
+    with engine.Scene(spec, absorption=False) as sc:
+        for i in range(nb_g):
+            sc.run(1, i * n_gpu, n_gpu)
+            g = sc.collect()
+            st = sc.stats()
+            G.append(vec(g.det_weight(), g.bin_weight(), st["wAbsorbed"], n_gpu))
+    Cc = []
+    for i in range(nb_c):
+        o = OutputHandle(spec, absorption=False)
+        c = ol.run_oracle(spec, inp, o, 2, i * n_cpu, n_cpu, 64, 0)
+        Cc.append(vec(o.det_weight(), o.bin_weight(), c.wAbsorbed, n_cpu))
+    G, Cc = np.array(G), np.array(Cc)
+    mg, mc = G.mean(0), Cc.mean(0)
+    se = np.sqrt(G.var(0, ddof=1) / nb_g + Cc.var(0, ddof=1) / nb_c)
+    populated = (mc > 1e-5) & ((Cc > 0).sum(axis=0) >= nb_c // 2)
+    z = np.abs(mg - mc) / np.where(se > 0, se, 1.0)
+    rings, tof = np.arange(20), np.arange(20, 20 + spec.time_bins)
+    zr = z[rings][populated[rings]]
+    zt = z[tof][populated[tof]]
+    print(f"max |z|: rings {zr.max():.2f}, TOF {zt.max():.2f} over {zt.size} bins "
+          f"({np.mean(zt < 3):.3f} within 3, mean z^2 {np.mean(zt ** 2):.2f}), R {z[-2]:.2f}, A {z[-1]:.2f}; "
+          f"R {mg[-2]:.5f} vs {mc[-2]:.5f}, A {mg[-1]:.5f} vs {mc[-1]:.5f}")
+    assert zr.size == 20 and np.all(zr < 3.0)
+    assert z[-2] < 3.0 and z[-1] < 3.0
+    assert zt.size > 40
+    assert np.mean(zt < 3.0) >= 0.97 and zt.max() < 4.5 and np.mean(zt ** 2) < 1.5
+    assert abs(mg[-2] / mc[-2] - 1) < 5e-3
+    assert abs(mg[-1] / mc[-1] - 1) < 5e-3
