@@ -298,8 +298,10 @@ int xmc_batch_set_launch(xmc_batch* batch, int blockSize, int blocksPerCU, int u
 int xmc_batch_get_launch(xmc_batch* batch, int* blockSize, int* blocks);
 
 /* The conservative distance grid the scene would build (host only; for
- * tests): every point of cell (x, y, z) is at least q[(z*dims[1]+y)*dims[0]+x]
- * * stepQ away from every triangle.  q may be NULL to query the sizes. */
+ * tests): the centre lo + ((x, y, z) + 1/2) * h of cell (x, y, z) is at least
+ * q[(z*dims[1]+y)*dims[0]+x] * stepQ away from every triangle, so a point p
+ * is at least that less |p - centre| away.  q may be NULL to query the
+ * sizes. */
 int xmc_distance_grid(const xmc_input* input, uint64_t targetCells, float* lo, float* h, float* stepQ,
                       int* dims, uint8_t* q);
 

@@ -1,5 +1,6 @@
 """The distance grid used to skip BVH walks must never overstate the distance
-to the nearest triangle (host computation, no GPU)."""
+to the nearest triangle, and near the mesh it must be the exact distance from
+each cell centre (host computation, no GPU)."""
 import numpy as np
 
 from xmcml_amd import engine, scenes
@@ -35,28 +36,58 @@ def closest_point_distance(p, a, b, c):
     return np.sqrt(((p - q) ** 2).sum(1))
 
 
-def test_bound_never_exceeds_true_distance():
+def _grid_and_triangles(target=200_000):
     spec = scenes.small_scene("heightfield", 10)
-    g = engine.distance_grid(spec, target_cells=200_000)
+    g = engine.distance_grid(spec, target_cells=target)
     tri = np.concatenate([v[t] for v, t in spec.surfaces])  # (T, 3, 3)
-    a, b, c = tri[:, 0], tri[:, 1], tri[:, 2]
+    return g, tri[:, 0], tri[:, 1], tri[:, 2]
+
+
+def test_bound_never_exceeds_true_distance():
+    """For any point p, q * step_q of its cell less |p - centre| never
+    exceeds p's distance to the nearest triangle (brute force)."""
+    g, a, b, c = _grid_and_triangles()
     rng = np.random.default_rng(1)
     dims = np.array(g["dims"])
     hi = g["lo"] + dims * g["h"]
     pts = rng.uniform(g["lo"], hi, size=(3000, 3))
     # points concentrated near the surfaces too
-    pts[:1000, 2] = rng.uniform(-1.5, 3.5, 1000)
+    pts[:1500, 2] = rng.uniform(-1.5, 3.5, 1500)
     idx = np.floor((pts - g["lo"]) / g["h"]).astype(int)
     ok = np.all((idx >= 0) & (idx < dims), axis=1)
     pts, idx = pts[ok], idx[ok]
-    bound = g["q"][idx[:, 2], idx[:, 1], idx[:, 0]].astype(np.float64) * g["step_q"]
+    centre = g["lo"].astype(np.float64) + (idx + 0.5) * g["h"]
+    bound = (g["q"][idx[:, 2], idx[:, 1], idx[:, 0]].astype(np.float64) * g["step_q"]
+             - np.sqrt(((pts - centre) ** 2).sum(1)))
     culled = 0
     for p, L in zip(pts, bound):
-        if L == 0:
+        if L <= 0:
             continue
         d = closest_point_distance(np.broadcast_to(p, a.shape), a, b, c).min()
-        assert L <= d, (p, L, d)
+        assert L <= d + 1e-9, (p, L, d)
         culled += 1
-    assert culled > 500  # the bound is useful: most random points are far from every surface
-    # and reasonably tight: within 2 sqrt(3) h + a quantum of the truth on average
-    assert g["q"].max() > 0
+    assert culled > 1000  # the bound is useful: most random points are far from every surface
+
+
+def test_centre_distance_exact_near_the_surfaces():
+    """Within a few cells of the mesh the stored bound is the exact centre
+    distance floored to the quantum (a box distance or a misplaced centre
+    would be off by far more), and farther out it never undercuts the old
+    point-of-cell bound by more than a cell."""
+    g, a, b, c = _grid_and_triangles()
+    dims = np.array(g["dims"])
+    rng = np.random.default_rng(2)
+    idx = np.stack([rng.integers(0, dims[0], 4000), rng.integers(0, dims[1], 4000),
+                    rng.integers(0, dims[2], 4000)], axis=1)
+    centre = g["lo"].astype(np.float64) + (idx + 0.5) * g["h"]
+    keep = (centre[:, 2] > -1.5) & (centre[:, 2] < 3.5)
+    idx, centre = idx[keep][:800], centre[keep][:800]
+    stored = g["q"][idx[:, 2], idx[:, 1], idx[:, 0]].astype(np.float64) * g["step_q"]
+    true = np.array([closest_point_distance(np.broadcast_to(p, a.shape), a, b, c).min() for p in centre])
+    assert np.all(stored <= true + 1e-9)
+    near = true < 3.5 * g["h"]
+    assert near.sum() > 50
+    # float storage of the distance loses at most a few ulps before flooring
+    assert np.all(stored[near] > true[near] - g["step_q"] - 1e-5)
+    far = ~near & (true < 200 * g["step_q"])
+    assert np.all(stored[far] > true[far] - 1.8 * g["h"])

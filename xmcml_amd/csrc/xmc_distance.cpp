@@ -1,18 +1,30 @@
 // Conservative distance grid (see xmc_distance.h).
 //
-// Marking: every cell that intersects the bounding box of a triangle is a
-// source, so every triangle point lies in a source cell.  For a point p in
-// cell C and a triangle point t in source cell M,
-//   |p - t| >= |c_C - c_M| - sqrt(3) h
-// (each point is within sqrt(3) h / 2 of its cell centre).  The exact
-// Euclidean distance transform of the source set (Felzenszwalb &
-// Huttenlocher, "Distance Transforms of Sampled Functions", 2012: separable
-// lower envelopes of parabolas, one pass per axis) gives min_M |c_C - c_M|.
+// Each cell stores a lower bound D_c on the distance from its centre c to
+// the nearest triangle.  For any point p, |p - t| >= D_c - |p - c| for every
+// triangle point t (triangle inequality), so the kernel, which knows p,
+// subtracts |p - c| itself and never relies on p lying inside the cell.
+//
+// Two bounds are combined, D_c = max(near, far):
+//   near: min(R, exact distance from c to every triangle whose bounding box
+//         grown by R contains c).  When the nearest triangle is within R of
+//         c it is one of those, so the minimum is exact; otherwise every
+//         triangle is farther than R and R itself is a bound.  Closest point
+//         on a triangle by Voronoi regions (Ericson, Real-Time Collision
+//         Detection, 2005, 5.1.5).
+//   far:  every cell that intersects the bounding box of a triangle is a
+//         source; a triangle point in source cell M is within sqrt(3) h / 2
+//         of its centre c_M, so |c - t| >= |c - c_M| - sqrt(3) h / 2.  The
+//         exact Euclidean distance transform of the source set (Felzenszwalb
+//         & Huttenlocher, "Distance Transforms of Sampled Functions", 2012:
+//         separable lower envelopes of parabolas, one pass per axis) gives
+//         min_M |c - c_M|.
 #include "xmc_distance.h"
 
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <thread>
 
 namespace xmc {
 namespace {
@@ -45,6 +57,64 @@ void edt_1d(const double* f, double* d, int n, std::vector<int>& v, std::vector<
         double dq = q - v[k];
         d[q] = dq * dq + f[v[k]];
     }
+}
+
+struct D3 {
+    double x, y, z;
+};
+inline D3 sub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 axpy(D3 a, double t, D3 b) { return {a.x + t * b.x, a.y + t * b.y, a.z + t * b.z}; }
+
+// Squared distance from p to triangle (a, b, c): the closest point is a
+// vertex, a point of an edge or of the face, told apart by the signs of the
+// projections of p onto the edges (the triangle's Voronoi regions).
+double point_triangle_sq(D3 p, D3 a, D3 b, D3 c) {
+    D3 ab = sub(b, a), ac = sub(c, a), ap = sub(p, a);
+    double d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0 && d2 <= 0.0) return dot(ap, ap);
+    D3 bp = sub(p, b);
+    double d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0 && d4 <= d3) return dot(bp, bp);
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        D3 q = axpy(a, d1 / (d1 - d3), ab);
+        D3 e = sub(p, q);
+        return dot(e, e);
+    }
+    D3 cp = sub(p, c);
+    double d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0 && d5 <= d6) return dot(cp, cp);
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        D3 q = axpy(a, d2 / (d2 - d6), ac);
+        D3 e = sub(p, q);
+        return dot(e, e);
+    }
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        D3 q = axpy(b, (d4 - d3) / ((d4 - d3) + (d5 - d6)), sub(c, b));
+        D3 e = sub(p, q);
+        return dot(e, e);
+    }
+    double den = va + vb + vc;
+    if (!(den > 0.0)) {  // degenerate (zero-area) triangle: nearest of its three edges
+        double best = std::min({dot(ap, ap), dot(bp, bp), dot(cp, cp)});
+        const D3 ends[3][2] = {{a, b}, {a, c}, {b, c}};
+        for (const auto& s : ends) {
+            D3 e = sub(s[1], s[0]);
+            double ee = dot(e, e);
+            if (ee <= 0.0) continue;
+            double t = std::clamp(dot(sub(p, s[0]), e) / ee, 0.0, 1.0);
+            D3 r = sub(p, axpy(s[0], t, e));
+            best = std::min(best, dot(r, r));
+        }
+        return best;
+    }
+    double v = vb / den, w = vc / den;
+    D3 q = axpy(axpy(a, v, ab), w, ac);
+    D3 e = sub(p, q);
+    return dot(e, e);
 }
 
 }  // namespace
@@ -86,40 +156,119 @@ void build_distance_grid(const xmc_surface* s, int ns, const double lo[3], const
                     for (int x = c0[0]; x <= c1[0]; x++) f[idx(x, y, z)] = 0.0;
         }
     }
-    // separable squared EDT along x, y, z
-    std::vector<int> v;
-    std::vector<double> zb, in, res;
+    // separable squared EDT along x, y, z; the rows of one pass are independent
+    const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     for (int a = 0; a < 3; a++) {
-        int n = dims[a];
-        in.resize(n);
-        res.resize(n);
-        int o1 = (a + 1) % 3, o2 = (a + 2) % 3;
-        for (int p = 0; p < dims[o1]; p++)
-            for (int r = 0; r < dims[o2]; r++) {
-                int c[3];
-                c[o1] = p;
-                c[o2] = r;
-                for (int i = 0; i < n; i++) {
-                    c[a] = i;
-                    in[i] = f[idx(c[0], c[1], c[2])];
+        const int n = dims[a], o1 = (a + 1) % 3, o2 = (a + 2) % 3;
+        const int rows = dims[o1] * dims[o2];
+        std::vector<std::thread> pool;
+        for (unsigned th = 0; th < nthreads; th++)
+            pool.emplace_back([&, th]() {
+                std::vector<int> v;
+                std::vector<double> zb, in(n), res(n);
+                for (int row = (int)th; row < rows; row += (int)nthreads) {
+                    int c[3];
+                    c[o1] = row / dims[o2];
+                    c[o2] = row % dims[o2];
+                    for (int i = 0; i < n; i++) {
+                        c[a] = i;
+                        in[i] = f[idx(c[0], c[1], c[2])];
+                    }
+                    edt_1d(in.data(), res.data(), n, v, zb);
+                    for (int i = 0; i < n; i++) {
+                        c[a] = i;
+                        f[idx(c[0], c[1], c[2])] = res[i];
+                    }
                 }
-                edt_1d(in.data(), res.data(), n, v, zb);
-                for (int i = 0; i < n; i++) {
-                    c[a] = i;
-                    f[idx(c[0], c[1], c[2])] = res[i];
-                }
+            });
+        for (auto& t : pool) t.join();
+    }
+    // near bound: exact centre-to-triangle distances within R of each triangle.  R is
+    // 4 cells, less when that would cost more than ~4e8 distance evaluations.
+    struct Tri {
+        D3 v[3];
+        double lo[3], hi[3];
+        int c0[3], c1[3];
+    };
+    std::vector<Tri> tris;
+    for (int i = 0; i < ns; i++)
+        for (int j = 0; j < s[i].numberOfTriangles; j++) {
+            const xmc_int3& t = s[i].triangles[j];
+            const xmc_double3* v = s[i].vertices;
+            Tri T;
+            T.v[0] = {v[t.x].x, v[t.x].y, v[t.x].z};
+            T.v[1] = {v[t.y].x, v[t.y].y, v[t.y].z};
+            T.v[2] = {v[t.z].x, v[t.z].y, v[t.z].z};
+            tris.push_back(T);
+        }
+    auto cell_range = [&](double R) {
+        double work = 0.0;
+        for (Tri& T : tris) {
+            const double tl[3] = {std::min({T.v[0].x, T.v[1].x, T.v[2].x}), std::min({T.v[0].y, T.v[1].y, T.v[2].y}),
+                                  std::min({T.v[0].z, T.v[1].z, T.v[2].z})};
+            const double th[3] = {std::max({T.v[0].x, T.v[1].x, T.v[2].x}), std::max({T.v[0].y, T.v[1].y, T.v[2].y}),
+                                  std::max({T.v[0].z, T.v[1].z, T.v[2].z})};
+            double cells = 1.0;
+            for (int a = 0; a < 3; a++) {
+                T.lo[a] = tl[a];
+                T.hi[a] = th[a];
+                // centres lo + (i + 1/2) h within [tl - R, th + R]
+                T.c0[a] = std::max(0, (int)std::ceil((tl[a] - R - lo[a]) / h - 0.5));
+                T.c1[a] = std::min(dims[a] - 1, (int)std::floor((th[a] + R - lo[a]) / h - 0.5));
+                cells *= std::max(0, T.c1[a] - T.c0[a] + 1);
             }
+            work += cells;
+        }
+        return work;
+    };
+    // cell centres as the kernel forms them: from the float origin
+    const double flo[3] = {(double)(float)lo[0], (double)(float)lo[1], (double)(float)lo[2]};
+    double R = 4.0 * h;
+    while (R > h && cell_range(R) > 4e8) R -= 0.5 * h;
+    cell_range(R);
+    std::vector<float> near(total, (float)R);
+    {
+        const unsigned nth = nthreads;
+        std::vector<std::thread> pool;
+        for (unsigned th = 0; th < nth; th++)
+            pool.emplace_back([&, th]() {
+                // this thread owns the z-slices z = th (mod nth): the surfaces may fill only a
+                // few slices, and interleaving shares those among all threads
+                for (const Tri& T : tris) {
+                    int za = T.c0[2] + (int)((th + nth - (unsigned)T.c0[2] % nth) % nth);
+                    for (int z = za; z <= T.c1[2]; z += (int)nth)
+                        for (int y = T.c0[1]; y <= T.c1[1]; y++)
+                            for (int x = T.c0[0]; x <= T.c1[0]; x++) {
+                                D3 c = {flo[0] + (x + 0.5) * h, flo[1] + (y + 0.5) * h, flo[2] + (z + 0.5) * h};
+                                float& m = near[idx(x, y, z)];
+                                // the triangle's bounding box is no farther than the triangle:
+                                // skip it when the box is already beyond the cell's minimum
+                                double bx = std::max({T.lo[0] - c.x, 0.0, c.x - T.hi[0]});
+                                double by = std::max({T.lo[1] - c.y, 0.0, c.y - T.hi[1]});
+                                double bz = std::max({T.lo[2] - c.z, 0.0, c.z - T.hi[2]});
+                                if (bx * bx + by * by + bz * bz >= (double)m * (double)m) continue;
+                                double d = std::sqrt(point_triangle_sq(c, T.v[0], T.v[1], T.v[2]));
+                                // stored rounded down, so that the float never overstates d
+                                float df = (float)d;
+                                if ((double)df > d) df = std::nextafter(df, 0.0f);
+                                if (df < m) m = df;
+                            }
+                }
+            });
+        for (auto& t : pool) t.join();
     }
     out.dims[0] = dims[0];
     out.dims[1] = dims[1];
     out.dims[2] = dims[2];
     for (int a = 0; a < 3; a++) out.lo[a] = (float)lo[a];
     out.h = hf;
-    out.step_q = hf * 0.25f;
+    out.step_q = hf * 0.125f;
+    out.reach = (float)R;
     out.q.assign(total, 0);
-    const double sq3 = std::sqrt(3.0);
+    const double half_diag = 0.5 * std::sqrt(3.0);
     for (size_t c = 0; c < total; c++) {
-        double d = f[c] >= kInf * 0.5 ? 1e9 : (std::sqrt(f[c]) - sq3) * h;
+        double far = f[c] >= kInf * 0.5 ? 1e9 : (std::sqrt(f[c]) - half_diag) * h;
+        double d = std::max((double)near[c], far);
         if (d <= 0.0) continue;
         double qv = std::floor(d / (double)out.step_q);
         out.q[c] = (uint8_t)std::min(qv, 255.0);

@@ -9,21 +9,24 @@
 namespace xmc {
 
 // Cell c of a dims[0] x dims[1] x dims[2] grid with origin `lo` and cubic
-// cells of edge h holds q[c] such that every point of the cell is at least
-// q[c] * step_q away from every triangle.  A step-limited query from inside
-// cell c cannot hit anything when its length is below that bound.
+// cells of edge h holds q[c] such that the centre of the cell,
+// lo + (i + 1/2) h, is at least q[c] * step_q away from every triangle.  A
+// point p is then at least q[c] * step_q - |p - centre| away, and a
+// step-limited query from p shorter than that cannot hit anything.
 struct DistanceGrid {
     float lo[3] = {0, 0, 0};
     float h = 1.0f;
-    float step_q = 0.25f;  // distance quantum (h / 4)
+    float step_q = 0.125f;  // distance quantum (h / 8)
+    float reach = 0.0f;     // R: distances up to R are exact (before quantisation)
     int dims[3] = {0, 0, 0};
     std::vector<uint8_t> q;
 };
 
 // Builds the grid over [lo, hi] with about `target_cells` cells.  The bound
-// is |c_C - c_M| - sqrt(3) h minimised over cells M that intersect some
-// triangle's bounding box (exact Euclidean distance transform of cell
-// centres), floored to the quantum.
+// is the exact distance from the cell centre to the nearest triangle when
+// that is below R (4 cells, or fewer on very large meshes), otherwise the
+// larger of R and the distance transform of the cells that meet triangle
+// bounding boxes minus half a cell diagonal; floored to the quantum.
 void build_distance_grid(const xmc_surface* surfaces, int n_surfaces, const double lo[3], const double hi[3],
                          size_t target_cells, DistanceGrid& out);
 

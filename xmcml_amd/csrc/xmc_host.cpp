@@ -336,6 +336,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
             P.dist.lo[a] = dg.lo[a];
             P.dist.dims[a] = dg.dims[a];
         }
+        P.dist.h = dg.h;
         P.dist.inv_h = 1.0f / dg.h;
         P.dist.step_q = dg.step_q;
         P.dist.margin = want_entry ? eg.margin : 0.0f;

@@ -81,7 +81,7 @@ struct Trace {
 
 // Conservative distance grid (see xmc_distance.h); q == nullptr disables it.
 // Per-cell word of the walk-skipping grid: bits 0-7 hold q, a lower bound
-// q * step_q on the distance from any point of the cell to any triangle;
+// q * step_q on the distance from the cell's centre to any triangle;
 // bits 8-31 hold the cell's entry for walks that move at most `margin`
 // along every axis: 0 = no triangle within reach, kEntryRoot = start at the
 // root, else 1 + an index into entry_ref / entry_box (node reference and its
@@ -91,6 +91,7 @@ constexpr uint32_t kEntryRoot = 0xFFFFFFu;
 struct DistGrid {
     const uint32_t* cells;  // null when the scene has no grid
     float lo[3];
+    float h;
     float inv_h;
     float step_q;
     float margin;

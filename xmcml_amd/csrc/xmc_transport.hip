@@ -656,11 +656,13 @@ __device__ __forceinline__ int time_bin(const Detectors& D, double t) {
 }
 
 // Where a step-limited walk from p along d over `step` starts: -1 nothing
-// can be hit (skip the walk), -2 at the root, e >= 0 at entry e.  Skipping
-// needs either the cell's distance bound to exceed the step by a margin that
-// covers float rounding of the lookup and of the triangle test, or a step that
-// moves at most the grid's margin along every axis from a cell with no
-// triangle within reach; a short step from a cell with an entry starts there.
+// can be hit (skip the walk), -2 at the root, e >= 0 at entry e.  The cell
+// bounds the distance from its centre to every triangle, so p is at least
+// that bound less |p - centre| away (whichever cell float rounding files p
+// under).  Skipping needs that to exceed the step by a margin that covers
+// float rounding of the lookup and of the triangle test, or a step that moves
+// at most the grid's margin along every axis from a cell with no triangle
+// within reach; a short step from a cell with an entry starts there.
 __device__ __forceinline__ int walk_start(const DistGrid& G, V3 p, V3 d, float step) {
     float fx = (p.x - G.lo[0]) * G.inv_h;
     float fy = (p.y - G.lo[1]) * G.inv_h;
@@ -669,8 +671,12 @@ __device__ __forceinline__ int walk_start(const DistGrid& G, V3 p, V3 d, float s
     int ix = (int)fx, iy = (int)fy, iz = (int)fz;
     if (ix >= G.dims[0] || iy >= G.dims[1] || iz >= G.dims[2]) return -2;
     uint32_t word = G.cells[((size_t)iz * G.dims[1] + iy) * G.dims[0] + ix];
+    float ox = (fx - ((float)ix + 0.5f)) * G.h;
+    float oy = (fy - ((float)iy + 0.5f)) * G.h;
+    float oz = (fz - ((float)iz + 0.5f)) * G.h;
+    float off = __builtin_sqrtf(ox * ox + oy * oy + oz * oz);
     float L = (float)(word & 0xFFu) * G.step_q;
-    float room = L - (1.0e-4f + 16.0f * offset_scale(p));
+    float room = L - off - (1.0e-4f + 16.0f * offset_scale(p));
     if (room > 0.0f && step * step * dot(d, d) < room * room) return -1;
     float reach = step * fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z)));
     uint32_t e = word >> 8;
