@@ -317,7 +317,7 @@ def main(argv=None) -> int:
         "frac": achieved / PEAK_FP32_TFLOPS,
         "frac_of_nonpacked_peak": achieved / PEAK_FP32_NONPACKED_TFLOPS,
         "traffic": traffic_from_profiles(args.photons, args.workload),
-        "kernel": "xmc::transport_kernel<false, false>",
+        "kernel": "xmc::transport_kernel<false, false, true>" if spec.num_triangles > 256 else "xmc::transport_kernel<false, false, false>",
         "kernel_ms_per_launch": launch_s * 1e3,
         "flop_per_launch": flop / args.steps,
         "scatter_events_per_photon": st["scatterEvents"] / photons,

@@ -17,7 +17,7 @@ ROOT = Path(__file__).resolve().parents[1]
 
 def declared(header: Path):
     text = header.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|void|double|const char\*|uint64_t|float)\s+\*?(\w+)\s*\(",
+    return sorted(set(re.findall(r"^\s*(?:int|void|double|const char\*|uint64_t|float|xmc_input\s*\*)\s*\*?(\w+)\s*\(",
                                  text, re.M)))
 
 

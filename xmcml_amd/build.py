@@ -22,7 +22,8 @@ REFERENCE = Path("/root/reference")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("XMC_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["xmc_transport.hip", "xmc_mlp.hip", "xmc_host.cpp", "xmc_bvh.cpp", "xmc_distance.cpp", "xmc_entry.cpp"]
+SOURCES = ["xmc_transport.hip", "xmc_mlp.hip", "xmc_host.cpp", "xmc_bvh.cpp", "xmc_distance.cpp", "xmc_entry.cpp",
+           "xmc_io.cpp"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-fno-fast-math", "-munsafe-fp-atomics",

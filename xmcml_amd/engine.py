@@ -36,6 +36,10 @@ EXPORTS = [
     "xmc_mlp_create", "xmc_mlp_destroy", "xmc_mlp_param_count", "xmc_mlp_get_params", "xmc_mlp_set_params",
     "xmc_mlp_forward", "xmc_mlp_backward", "xmc_mlp_get_grads", "xmc_mlp_adam", "xmc_mlp_train_step",
     "xmc_mlp_gemm_check", "xmc_mlp_train_sampled", "xmc_dev_alloc", "xmc_dev_free", "xmc_dev_copy", "xmc_dev_sync",
+    # file formats and backups (include/xmcml_io.h)
+    "xmc_read_input_file", "xmc_read_surface_file", "xmc_document_input", "xmc_document_biasing",
+    "xmc_document_free", "xmc_write_surface_file", "xmc_write_output_file", "xmc_write_backup_file",
+    "xmc_backup_file_name", "xmc_read_backup_file", "xmc_launch_with_backups",
 ]
 
 
@@ -102,6 +106,18 @@ def lib():
     sig("xmc_dev_free", _I, [_P])
     sig("xmc_dev_copy", _I, [_P, _P, _U64, _I])
     sig("xmc_dev_sync", _I, [_I])
+    _S = C.c_char_p
+    sig("xmc_read_input_file", _I, [_S, C.POINTER(_P)])
+    sig("xmc_read_surface_file", _I, [_P, _S])
+    sig("xmc_document_input", _P, [_P])
+    sig("xmc_document_biasing", _I, [_P, _P])
+    sig("xmc_document_free", None, [_P])
+    sig("xmc_write_surface_file", _I, [_P, _I, _S])
+    sig("xmc_write_output_file", _I, [_P, _P, _S])
+    sig("xmc_write_backup_file", _I, [_P, _P, _P, _S])
+    sig("xmc_backup_file_name", _I, [_U64, C.c_char_p, _I])
+    sig("xmc_read_backup_file", _I, [_S, _P, _P])
+    sig("xmc_launch_with_backups", _I, [_P, _P, _P, _P, _U64, _S])
     _lib = L
     return L
 
