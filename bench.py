@@ -1,14 +1,14 @@
 """Photon-transport throughput on one MI355X, next to the CPU path on the
 host's cores.
 
-Workload (one step): `--photons` photons of the 3-D surface configuration --
+Workload (one step): `--photons` photons (default 1e8, the whole configuration, in one
+persistent launch) of the 3-D surface configuration --
 three conformal seeded heightfield boundaries triangulated on a 128 x 128 grid
 over 40 x 40 mm plus a flat bottom (98,306 triangles in the BVH), the
 3-layer tissue of the slab configuration, pencil beam, roulette at 1e-4 with
 survival 0.1, 20 ring detectors of 0.5 mm, a 256-bin time-of-flight
-histogram over 0-2 ns per ring.  Ten steps are the 1e8 photons of that
-configuration.  Geometry, optics and tallies are resident in HBM before the
-timed region; photon streams continue from step to step.
+histogram over 0-2 ns per ring.  Geometry, optics and tallies are resident in
+HBM before the timed region; photon streams continue from step to step.
 
 Output: one JSON line.  `value` is photons/s over the timed steps;
 `roofline` prices the transport kernel's algorithmic FP32 work (80 FLOP per
@@ -247,9 +247,10 @@ def traffic_from_profiles(photons_per_launch: int, workload: str):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--photons", type=int, default=10_000_000, help="photons per step")
+    ap.add_argument("--photons", type=int, default=None,
+                    help="photons per step, one launch (default: 1e8, the configuration's count; 1e7 for head)")
     ap.add_argument("--workload", choices=["heightfield", "head", "slab", "batch"], default="heightfield",
                     help="heightfield: the 3-D surface configuration (the metric's workload); head: five nested "
                          "noisy ellipsoids of 327,680 triangles each; slab: the planar 3-layer configuration with "
@@ -274,6 +275,8 @@ def main(argv=None) -> int:
 
     from xmcml_amd import engine, scenes
 
+    if args.photons is None:
+        args.photons = 10_000_000 if args.workload == "head" else 100_000_000
     if args.workload == "batch":
         return bench_batch(args)
     make = {"heightfield": scenes.heightfield_scene, "head": scenes.head_scene, "slab": scenes.slab_scene}[args.workload]

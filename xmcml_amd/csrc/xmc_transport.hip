@@ -881,7 +881,10 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                 atomicAdd(ledger + kOther, (double)w);
                 continue;
             }
-            // ---- one interaction (mcml_kernel.cpp:47-124): the step, then the walk
+            // ---- one interaction (mcml_kernel.cpp:47-124): the step, then the walk.  A glass
+            // layer's walk is unlimited (ComputeBVHIntersectionWithoutStep, :468-584): the
+            // step-limited walk with step = +inf takes every branch the unlimited one takes
+            // (each tn <= step holds, each step < tn fails), so one walk serves both.
             step = kInf;
             if (!Lglass) {
                 float u;
@@ -917,9 +920,9 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                 if constexpr (INTERLEAVE) {
                     rinv = ray.inv;
                     raxes = (ray.ax ? 1u : 0u) | (ray.ay ? 2u : 0u) | (ray.az ? 4u : 0u);
-                    phase = walk_begin(geo, ray, step, !Lglass, st, h, cur, best) ? kWalk : kResolve;
+                    phase = walk_begin(geo, ray, step, true, st, h, cur, best) ? kWalk : kResolve;
                 } else {
-                    h = walk(geo, ray, step, !Lglass, st);
+                    h = walk(geo, ray, step, true, st);
                     phase = kResolve;
                 }
             }
@@ -931,7 +934,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                 if (phase == kWalk) {
                     const Params& C = CFG();
                     Ray ray{pos, dir, rinv, (raxes & 1u) != 0, (raxes & 2u) != 0, (raxes & 4u) != 0};
-                    if (!walk_visit(C.nodes, C.tris, ray, step, !Lglass, st, h, cur, best)) phase = kResolve;
+                    if (!walk_visit(C.nodes, C.tris, ray, step, true, st, h, cur, best)) phase = kResolve;
                 }
                 if (__ballot(phase == kWalk) == 0) break;
             }
