@@ -28,7 +28,7 @@ namespace xmc {
 
 constexpr float kInf = __builtin_inff();
 #ifndef XMC_VISITS_PER_ROUND
-#define XMC_VISITS_PER_ROUND 4
+#define XMC_VISITS_PER_ROUND 2
 #endif
 constexpr float kEpsF = 1.0e-6f;                 // EPSILON
 constexpr float kCosZeroF = 1.0f - 1.0e-6f;      // float "cosine of zero"
