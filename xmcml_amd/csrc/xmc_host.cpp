@@ -112,6 +112,7 @@ struct xmc_scene {
     std::vector<int> perm;  // (surface, index) order -> leaf-order position
     DevBuf<xmc::GpuNode> nodes;
     DevBuf<xmc::GpuTri> tris;
+    DevBuf<float> normals;  // 4 per triangle
     DevBuf<int8_t> next_layer;
     DevBuf<float> cube, ring;
     DevBuf<uint32_t> filter;
@@ -290,11 +291,29 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     HIP_TRY(hipMemcpy(sc->tris.p, gb.tris.data(), nt * sizeof(xmc::GpuTri), hipMemcpyHostToDevice));
     HIP_TRY(sc->perm_dev.alloc(nt));
     HIP_TRY(hipMemcpy(sc->perm_dev.p, sc->perm.data(), nt * sizeof(int), hipMemcpyHostToDevice));
+    {
+        // unit normals with the kernel's float operations (cross, sqrt, three divisions; no
+        // FMA contraction): the same bits the kernel would compute at each boundary hit
+        std::vector<float> nrm(4 * nt, 0.0f);
+        for (size_t k = 0; k < nt; k++) {
+            const xmc::GpuTri& t = gb.tris[k];
+            float cx = t.e1[1] * t.e2[2] - t.e1[2] * t.e2[1];
+            float cy = t.e1[2] * t.e2[0] - t.e1[0] * t.e2[2];
+            float cz = t.e1[0] * t.e2[1] - t.e1[1] * t.e2[0];
+            float l = std::sqrt(cx * cx + cy * cy + cz * cz);
+            nrm[4 * k] = cx / l;
+            nrm[4 * k + 1] = cy / l;
+            nrm[4 * k + 2] = cz / l;
+        }
+        HIP_TRY(sc->normals.alloc(4 * nt));
+        HIP_TRY(hipMemcpy(sc->normals.p, nrm.data(), nrm.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
 
     xmc::Params& P = sc->base;
     std::memset(&P, 0, sizeof(P));
     P.nodes = sc->nodes.p;
     P.tris = sc->tris.p;
+    P.normals = sc->normals.p;
     P.root_ref = gb.root_ref;
     for (int a = 0; a < 3; a++) {
         P.root_lo[a] = gb.root_lo[a];
@@ -369,6 +388,10 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
             P.layers.mua[l] = (float)L.absorptionCoefficient;
             P.layers.mus[l] = (float)L.scatteringCoefficient;
             P.layers.g[l] = (float)L.anisotropy;
+            // per-layer constants of the step and the deposit, in the kernel's float operations
+            float mut = P.layers.mua[l] + P.layers.mus[l];
+            P.layers.inv_mut[l] = 1.0f / mut;
+            P.layers.abs_frac[l] = P.layers.mua[l] / mut;
             if (L.absorptionCoefficient == 0.0 && L.scatteringCoefficient == 0.0) P.layers.glass_mask |= 1u << l;
         } else {
             P.layers.n[l] = 1.0f;

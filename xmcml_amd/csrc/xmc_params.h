@@ -24,6 +24,8 @@ struct Layers {
     float mua[kMaxLayers];
     float mus[kMaxLayers];
     float g[kMaxLayers];
+    float inv_mut[kMaxLayers];            // 1 / (mua + mus), in float
+    float abs_frac[kMaxLayers];           // mua / (mua + mus), in float
     float crit[kMaxLayers][kMaxLayers];  // critical cosine for (from, to)
     uint32_t glass_mask;                  // bit l: layer l has mua == mus == 0
     uint32_t target_mask;                 // bit l: targetRangeLayers[l]
@@ -112,6 +114,7 @@ struct AbsTile {
 struct Params {
     const GpuNode* nodes;
     const GpuTri* tris;
+    const float* normals;  // 4 floats per triangle: the unit normal normalize(cross(e1, e2)), in float
     DistGrid dist;
     int root_ref;
     float root_lo[3], root_hi[3];

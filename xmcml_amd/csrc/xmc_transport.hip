@@ -792,9 +792,8 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     auto load_layer = [&](int l) {
         const Layers& L = CFG().layers;
         Ln = L.n[l];
-        float mut = L.mua[l] + L.mus[l];
-        Linv = 1.0f / mut;
-        Labs = L.mua[l] / mut;
+        Linv = L.inv_mut[l];   // 1 / (mua + mus) and mua / (mua + mus), computed once per scene
+        Labs = L.abs_frac[l];
         Lg = L.g[l];
         Lglass = (L.glass_mask >> l) & 1u;
         Ltarget = (L.target_mask >> l) & 1u;
@@ -952,7 +951,8 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             pos.y += step * dir.y;
             pos.z += step * dir.z;
             V3 hit = pos;
-            V3 nrm = tri_normal(C.tris, h.tri);
+            const float4 nv = reinterpret_cast<const float4*>(C.normals)[h.tri];  // tri_normal(), once per scene
+            V3 nrm = {nv.x, nv.y, nv.z};
             float ofs = offset_scale(hit);
             if (dot(dir, nrm) < 0.0f) ofs = -ofs;
             pos.x = hit.x - ofs * nrm.x;
