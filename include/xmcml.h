@@ -21,6 +21,9 @@
  *   ComputeSpecularReflectance() (base/xmcml/xmcml/mcml_kernel.h:10)
  *       -> xmc_specular_reflectance().
  *
+ * Attractive-vector biasing (InputInfo.useBiasing..targetPoint) runs on the
+ * GPU; its weight-integral tables are computed there too (xmc_weight_table_compute).
+ *
  * Differences from the reference seam, all deliberate:
  *   - Functions return an int status (XMC_OK or a negative code) instead of
  *     void; xmc_last_error() gives the message.  LaunchOMP reports nothing.
@@ -106,8 +109,17 @@ typedef struct xmc_ring_detector {
 #define XMC_TERMINATE_ROULETTE 1 /* MCML roulette: below minWeight survive with
                                     probability rouletteChance, weight /= chance */
 
-/* InputInfo, mcml_kernel_types.h:64-91, minus the fields of the biasing
- * extension (useBiasing..targetPoint), which the GPU path does not implement. */
+/* WeightIntegralTable, mcml_kernel_types.h:47-52: for one anisotropy g,
+ * elements[j] = the weight integral at dot = -1 + 2 j / (numberOfElements - 1)
+ * (mcml_integration.cpp:7-82, weight_integral.cpp:33-64). */
+typedef struct xmc_weight_table {
+    double anisotropy;
+    int numberOfElements;
+    const double* elements;
+} xmc_weight_table;
+
+/* InputInfo, mcml_kernel_types.h:64-91.  The biasing fields follow the
+ * others (appended, so that callers which zero the struct get no biasing). */
 typedef struct xmc_input {
     uint64_t numberOfPhotons; /* photons for this call (launch) */
     const xmc_area* area;
@@ -128,6 +140,22 @@ typedef struct xmc_input {
     uint8_t targetRangeLayers[XMC_MAX_LAYERS];
     int terminationMode; /* XMC_TERMINATE_* */
     double rouletteChance;
+    /* Attractive-vector biasing (ComputePhotonDirectionWithBiasing,
+     * mcml_kernel.cpp:325-440): each scattering draws directions until one is
+     * accepted with probability (0.1 + 0.45 (1 + d.a))^attractiveFactor, a the
+     * attractive vector towards targetPoint, and multiplies the weight by the
+     * weight integral of the old direction over that probability.  The tables
+     * (one per distinct anisotropy of layers 1.., GetDifferentAnisotropies,
+     * weight_integral.cpp:67-90) are the caller's, or, when weightTable is
+     * NULL, computed on the GPU by xmc_scene_create with weightIntegralPrecision
+     * and weightTablePrecision (defaults 128 and 262144 when 0, parser.cpp:10-11). */
+    int useBiasing;
+    double attractiveFactor;
+    xmc_double3 targetPoint;
+    int weightIntegralPrecision;
+    int weightTablePrecision;
+    const xmc_weight_table* weightTable;
+    int numberOfWeightTables;
 } xmc_input;
 
 /* TimeInfo, mcml_kernel_types.h:10-17. */
@@ -304,6 +332,23 @@ int xmc_batch_get_launch(xmc_batch* batch, int* blockSize, int* blocks);
  * sizes. */
 int xmc_distance_grid(const xmc_input* input, uint64_t targetCells, float* lo, float* h, float* stepQ,
                       int* dims, uint8_t* q);
+
+/* --- Weight integrals of the biasing (FP64, on the GPU). --- */
+/* ComputeWeightIntegral (mcml_integration.cpp:7-82) at numberOfDots values of
+ * dot: 2-point Gauss-Legendre on precision x precision cells of
+ * (theta, phi) in [0, pi] x [0, 2 pi].  The sines, cosines and powers that do
+ * not depend on dot come from the host's libm (as in the reference); the sums
+ * run on the GPU, one dot per thread, in the reference's order, so the values
+ * equal the reference's. */
+int xmc_weight_integrals(const double* dots, int numberOfDots, double anisotropy, double attractiveFactor,
+                         int precision, double* out, int device);
+/* The table ComputeWeightIntegral (weight_integral.cpp:33-64) builds for one
+ * anisotropy: elements[j] at dot = -1 + 2 j / tablePrecision, j = 0..tablePrecision,
+ * with the integral precision scaled by GetWeightIntegralPrecision
+ * (weight_integral.cpp:92-103: x16 above g = 0.98, x8 above 0.97, x4 above
+ * 0.95, x2 above 0.90).  elements holds tablePrecision + 1 doubles. */
+int xmc_weight_table_compute(double anisotropy, double attractiveFactor, int integralPrecision, int tablePrecision,
+                             double* elements, int device);
 
 /* --- The LaunchOMP replacement. --- */
 /* Builds a scene, runs input->numberOfPhotons photons starting at

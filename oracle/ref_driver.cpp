@@ -14,6 +14,7 @@
  */
 #include "mcml_kernel.h"
 #include "mcml_math.h"
+#include "mcml_integration.h"
 #include "launcher_omp.h"
 
 #include <omp.h>
@@ -58,6 +59,7 @@ struct RefInput {
     std::vector<RingDetector> rings;
     WeightIntegralTable table;
     double table_elem;
+    std::vector<WeightIntegralTable> tables;
 
     explicit RefInput(const xmc_input* x, bool with_bvh = true) {
         std::memset(&in, 0, sizeof(in));
@@ -115,14 +117,28 @@ struct RefInput {
         in.timeScaleSize = x->timeScaleSize;
         in.startPosition = {{x->startPosition.x, x->startPosition.y, x->startPosition.z}};
         in.startDirection = {{x->startDirection.x, x->startDirection.y, x->startDirection.z}};
-        in.useBiasing = 0;
-        // main.cpp:146-152 gives a one-element table when biasing is off
-        table.anisotropy = 0.0;
-        table.numberOfElements = 1;
-        table_elem = 0.0;
-        table.elements = &table_elem;
-        in.weightTable = &table;
-        in.numberOfWeightTables = 1;
+        in.useBiasing = x->useBiasing ? 1 : 0;
+        if (x->useBiasing) {  // the caller's tables (the reference computes them in weight_integral.cpp)
+            in.attractiveFactor = x->attractiveFactor;
+            in.targetPoint = {{x->targetPoint.x, x->targetPoint.y, x->targetPoint.z}};
+            for (int t = 0; t < x->numberOfWeightTables; t++) {
+                WeightIntegralTable w;
+                w.anisotropy = x->weightTable[t].anisotropy;
+                w.numberOfElements = x->weightTable[t].numberOfElements;
+                w.elements = const_cast<double*>(x->weightTable[t].elements);
+                tables.push_back(w);
+            }
+            in.weightTable = tables.data();
+            in.numberOfWeightTables = (int)tables.size();
+        } else {
+            // main.cpp:146-152 gives a one-element table when biasing is off
+            table.anisotropy = 0.0;
+            table.numberOfElements = 1;
+            table_elem = 0.0;
+            table.elements = &table_elem;
+            in.weightTable = &table;
+            in.numberOfWeightTables = 1;
+        }
         for (int j = 0; j < MAX_LAYERS; j++) in.targetRangeLayers[j] = x->targetRangeLayers[j];
         in.numberOfPhotons = x->numberOfPhotons;
         in.bvhTree = with_bvh ? GenerateBVHTree(in.surface, in.numberOfSurfaces) : nullptr;
@@ -249,7 +265,10 @@ void driver_photon(double rsp, InputInfo* input, OutputInfo* output, MCG59* rng,
             int idx = GetAreaIndex(photon.position, input->area);
             if (idx >= 0) {
                 ComputeDroppedWeightOfPhoton(&photon, input, output, idx);
-                photon.direction = GetNewPhotonDirection(&photon, input->layerInfo[photon.layerId].anisotropy, rng);
+                if (input->useBiasing)
+                    ComputePhotonDirectionWithBiasing(&photon, input, rng);
+                else
+                    photon.direction = GetNewPhotonDirection(&photon, input->layerInfo[photon.layerId].anisotropy, rng);
                 ev = XMC_EV_SCATTER;
                 aux = idx;
             } else {
@@ -445,6 +464,11 @@ int xref_detector_id(const xmc_input* x, const double* pos, const double* dir, c
     p.direction = {{dir[0], dir[1], dir[2]}};
     for (int i = 0; i < MAX_LAYERS; i++) p.visitedLayers[i] = visited[i] != 0;
     return GetDetectorId(&p, &r.in);
+}
+
+// ComputeWeightIntegral (mcml_integration.cpp:7-82) of the reference build.
+double ref_weight_integral(double dot, double anisotropy, double eps, unsigned n) {
+    return ComputeWeightIntegral(dot, anisotropy, eps, n);
 }
 
 }  // extern "C"

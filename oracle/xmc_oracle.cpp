@@ -34,6 +34,7 @@
 
 #include <omp.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -109,6 +110,28 @@ inline float f_logf(float x) {
     return std::fmaf(kf, 0.693145751953125f, std::fmaf(kf, 1.42860677e-06f, l1p));
 }
 
+// exp(x) of the 32-bit mode (the kernel's xexpf): k = rint(x log2 e), r = x - k ln2
+// with ln2 split as in f_logf (Cody-Waite), exp(r) by its Taylor series to r^7,
+// scaled by 2^k through the exponent bits; 0 below -87.
+inline float f_expf(float x) {
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) return std::numeric_limits<float>::infinity();
+    float kf = std::rint(x * 1.44269504f);
+    float r = std::fmaf(kf, -0.693145751953125f, x);
+    r = std::fmaf(kf, -1.42860677e-06f, r);
+    float p = std::fmaf(r, 1.0f / 5040.0f, 1.0f / 720.0f);
+    p = std::fmaf(r, p, 1.0f / 120.0f);
+    p = std::fmaf(r, p, 1.0f / 24.0f);
+    p = std::fmaf(r, p, 1.0f / 6.0f);
+    p = std::fmaf(r, p, 0.5f);
+    p = std::fmaf(r, p, 1.0f);
+    p = std::fmaf(r, p, 1.0f);
+    uint32_t bits = (uint32_t)((int)kf + 127) << 23;
+    float s;
+    std::memcpy(&s, &bits, 4);
+    return p * s;
+}
+
 inline void f_sincos2pi(float u, float* sp, float* cp) {
     float u4 = 4.0f * u;
     float q = std::rint(u4);
@@ -133,6 +156,42 @@ inline void f_sincos2pi(float u, float* sp, float* cp) {
     }
 }
 
+// ComputeWeightIntegral (mcml_integration.cpp:7-82): 2-point Gauss-Legendre on
+// n x n cells of (theta, phi) in [0, pi] x [0, 2 pi], statement for statement.
+double weight_integral(double dot, double anisotropy, double eps, unsigned n) {
+    const double coeff = 2.0 * sqrt(3.0);
+    const double pi = 3.14159265358979323846;
+    double anisotropy2 = anisotropy * anisotropy;
+    double sqrt_dot = sqrt(1 - dot * dot);
+    double xh = pi / n, yh = 2 * pi / n;
+    double sum = 0.0;
+    for (unsigned i = 0; i < n; ++i) {
+        double a = xh * i, b = a + xh;
+        double xp = 0.5 * (a + b) + (b - a) / coeff;
+        double xm = 0.5 * (a + b) - (b - a) / coeff;
+        for (unsigned j = 0; j < n; ++j) {
+            double c = yh * j, d = c + yh;
+            double yp = 0.5 * (c + d) + (d - c) / coeff;
+            double ym = 0.5 * (c + d) - (d - c) / coeff;
+            double stxp = sin(xp), stxm = sin(xm), spyp = sin(yp), spym = sin(ym), cpyp = cos(yp), cpym = cos(ym);
+            auto term = [&](double st, double cp, double sp) {
+                double px = st * cp;
+                double fx = dot * px + sqrt_dot * st * sp;
+                double tmp = 1 + anisotropy2 - 2 * anisotropy * fx;
+                double f = (1 - anisotropy2) / (2 * tmp * sqrt(tmp));
+                double p = pow(0.1 + 0.9 * 0.5 * (1 + px), eps);
+                return f * p * st;
+            };
+            double f_xp_yp = term(stxp, cpyp, spyp);
+            double f_xm_yp = term(stxm, cpyp, spyp);
+            double f_xp_ym = term(stxp, cpym, spym);
+            double f_xm_ym = term(stxm, cpym, spym);
+            sum += 0.25 * (b - a) * (d - c) * (f_xp_yp + f_xp_ym + f_xm_yp + f_xm_ym);
+        }
+    }
+    return sum / (2.0 * pi);
+}
+
 // ----------------------------------------------------------------------------
 // precision policies
 // ----------------------------------------------------------------------------
@@ -155,6 +214,7 @@ template <> struct Prec<double> {
         return g_uniform24 ? (double)(x >> 35) * 0x1p-24 : (double)x / (double)kMcgM;
     }
     static double log_(double u) { return log(u); }
+    static double pow_(double p, double e) { return pow(p, e); }
     static void sincos_2pi(double u, double* s, double* c) {
         double psi = 2.0 * kPi * u;  // mcml_kernel.cpp:453-455
         *c = cos(psi);
@@ -194,6 +254,8 @@ template <> struct Prec<float> {
     static constexpr float kEps = 1.0e-6f;
     static float uniform(uint64_t x) { return (float)(x >> 35) * 0x1p-24f; }
     static float log_(float u) { return f_logf(u); }
+    // the acceptance probability's power (0.1 <= p <= 1): exact for exponent 1
+    static float pow_(float p, float e) { return e == 1.0f ? p : f_expf(e * f_logf(p)); }
     static void sincos_2pi(float u, float* s, float* c) { f_sincos2pi(u, s, c); }
     static float one_minus_sq(float x) { return (1.0f - x) * (1.0f + x); }
 };
@@ -500,6 +562,53 @@ template <class R> struct Scene {
     V3<R> start_pos, start_dir;
     // time bins as InitOutput builds them (launcher_omp.cpp:38-55)
     std::vector<double> t_start, t_finish;
+    // attractive-vector biasing (mcml_kernel.cpp:325-440): the tables and each
+    // layer's (GetWeightTableIndex, :436-446: the last one of equal anisotropy)
+    bool biasing = false;
+    R factor = (R)1;
+    V3<R> target;
+    std::vector<std::vector<double>> tables;
+    std::vector<std::vector<float>> tables_f;  // the 32-bit mode's float copies
+    std::vector<int> layer_table;
+
+    // GetLocalAttractiveVector (:325-362), statement for statement
+    V3<R> attractive(const V3<R>& pos, int l) const {
+        const Layer& L = layers[l];
+        R D = (R)1 / ((R)3 * (((R)1 - L.g) * L.mus + L.mua));
+        R ltr = (R)3 * D;
+        R delta = (R)0.74 * ltr;
+        V3<R> r = sub(pos, start_pos);
+        V3<R> rd = sub(target, start_pos);
+        V3<R> r1 = rd;
+        r1.z += ltr;
+        V3<R> r2 = rd;
+        r2.z -= ltr + (R)2 * delta;
+        V3<R> t1 = sub(r, r1);
+        R l1 = length(t1);
+        t1 = divv(t1, l1 * l1 * l1);
+        V3<R> t2 = sub(r, r2);
+        R l2 = length(t2);
+        t2 = divv(t2, l2 * l2 * l2);
+        return normalize(sub(t2, t1));
+    }
+
+    // GetNewPhotonWeight (:420-434): the weight times the table entry of the old
+    // direction's cosine to the attractive vector, over the acceptance probability
+    R biased_weight(R w, const V3<R>& old_dir, const V3<R>& a, R p, int l) const {
+        int t = layer_table[l];
+        int n = (int)tables[t].size() - 1;
+        R d = dot(old_dir, a);
+        int k;
+        if constexpr (sizeof(R) == 4) {
+            k = (int)std::floor((d + 1.0f) * (0.5f * (float)n) + 0.5f);
+        } else {
+            double lo = -1.0, hi = 1.0, step = (hi - lo) / n;
+            k = (int)floor((d - lo) / step + 0.5);
+        }
+        k = k < 0 ? 0 : (k > n ? n : k);
+        if constexpr (sizeof(R) == 4) return w * tables_f[t][k] / p;
+        return w * (R)tables[t][k] / p;
+    }
 
     void init(const xmc_input* input) {
         in = input;
@@ -547,6 +656,7 @@ template <class R> struct Scene {
             q.cos_target = (R)cos(c.targetAngle * kPi / 180.0);
             rings.push_back(q);
         }
+        if (input->useBiasing) init_bias(input);
         rsp = xmco_specular(input->layerInfo);
         w0 = (R)in_val(1.0 - rsp);
         min_weight = (R)in_val(input->minWeight);
@@ -566,6 +676,46 @@ template <class R> struct Scene {
             t_start[T - 1] = input->timeStart + (T - 1) * dt;
             t_finish[T - 1] = input->timeFinish;
         }
+    }
+
+    void init_bias(const xmc_input* input) {
+        biasing = true;
+        factor = (R)input->attractiveFactor;
+        target = cvt<R>(input->targetPoint);
+        std::vector<double> gs;  // GetDifferentAnisotropies (weight_integral.cpp:67-90)
+        for (int l = 1; l < input->numberOfLayers; l++) {
+            double g = input->layerInfo[l].anisotropy;
+            if (std::find(gs.begin(), gs.end(), g) == gs.end()) gs.push_back(g);
+        }
+        std::vector<double> tg;
+        if (input->weightTable && input->numberOfWeightTables > 0) {
+            for (int t = 0; t < input->numberOfWeightTables; t++) {
+                const xmc_weight_table& w = input->weightTable[t];
+                tables.emplace_back(w.elements, w.elements + w.numberOfElements);
+                tg.push_back(w.anisotropy);
+            }
+        } else {  // ComputeWeightIntegral (weight_integral.cpp:33-64)
+            const int bp = input->weightIntegralPrecision > 0 ? input->weightIntegralPrecision : 128;
+            const int tp = input->weightTablePrecision > 0 ? input->weightTablePrecision : 262144;
+            for (double g : gs) {
+                unsigned n = bp;
+                if (g > 0.98) n *= 16;
+                else if (g > 0.97) n *= 8;
+                else if (g > 0.95) n *= 4;
+                else if (g > 0.90) n *= 2;
+                std::vector<double> e(tp + 1);
+                const double lo = -1.0, hi = 1.0, step = (hi - lo) / tp;
+#pragma omp parallel for schedule(dynamic, 64)
+                for (int j = 0; j <= tp; j++) e[j] = weight_integral(lo + j * step, g, input->attractiveFactor, n);
+                tables.push_back(e);
+                tg.push_back(g);
+            }
+        }
+        for (const auto& e : tables) tables_f.emplace_back(e.begin(), e.end());
+        layer_table.assign(input->numberOfLayers, -1);
+        for (int l = 0; l < input->numberOfLayers; l++)
+            for (size_t t = 0; t < tg.size(); t++)
+                if (tg[t] == input->layerInfo[l].anisotropy) layer_table[l] = (int)t;
     }
 
     V3<R> tri_normal(int ti) const {
@@ -1257,7 +1407,21 @@ template <class R> struct Walker {
                 p.weight -= dw;
                 if (!tl.absorption.empty()) tl.absorption[idx] += (double)dw;
                 tl.w_abs += (double)dw;
-                p.dir = new_direction(p.dir, L.g, rng);
+                if (sc.biasing) {
+                    // ComputePhotonDirectionWithBiasing (:398-418)
+                    V3<R> a = sc.attractive(p.pos, p.layer);
+                    V3<R> nd;
+                    R pr;
+                    do {
+                        nd = new_direction(p.dir, L.g, rng);
+                        pr = (R)0.1 + (R)0.9 * (((R)1 + dot(nd, a)) / (R)2);
+                        pr = Prec<R>::pow_(pr, sc.factor);
+                    } while (pr <= Prec<R>::uniform(rng.next()) && rng.draws <= kDrawLimit);
+                    p.weight = sc.biased_weight(p.weight, p.dir, a, pr, p.layer);
+                    p.dir = nd;
+                } else {
+                    p.dir = new_direction(p.dir, L.g, rng);
+                }
                 tl.scatter++;
                 record(XMC_EV_SCATTER, p, idx, rng);
             } else {
@@ -1645,6 +1809,13 @@ double xmco_triangle_distance(const double* o, const double* d, const double* a,
 }
 
 float xmco_logf(float x) { return f_logf(x); }
+void xmco_expf_many(const float* x, float* y, int64_t n) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) y[i] = f_expf(x[i]);
+}
+double xmco_weight_integral(double dot, double anisotropy, double eps, unsigned n) {
+    return weight_integral(dot, anisotropy, eps, n);
+}
 void xmco_sincos2pi(float u, float* s, float* c) { f_sincos2pi(u, s, c); }
 void xmco_hg_float_many(float g, const float* u, float* c, float* s, int64_t n) {
     for (int64_t i = 0; i < n; i++) hg_float(g, u[i], c + i, s + i);

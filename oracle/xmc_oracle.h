@@ -107,6 +107,10 @@ void xmco_logf_many(const float* x, float* y, int64_t n);
  * computes them (1 - cos and 1 + cos without cancellation). */
 void xmco_hg_float_many(float g, const float* u, float* c, float* s, int64_t n);
 void xmco_sincos2pi_many(const float* u, float* s, float* c, int64_t n);
+/* exp of the 32-bit mode (the acceptance power of the biasing, p^f = exp(f log p)) */
+void xmco_expf_many(const float* x, float* y, int64_t n);
+/* ComputeWeightIntegral (mcml_integration.cpp:7-82), double, restated */
+double xmco_weight_integral(double dot, double anisotropy, double eps, unsigned n);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,8 @@ def oracle():
         _sig(lib, "xmco_shadow_trace", I, [P, U64, U64, I, P, I, P, PI])
         _sig(lib, "xmco_query", I, [P, PD, PD, PD, I, I, I, PI, PD, PI, PI, PD])
         _sig(lib, "xmco_mcg59_init", U64, [U64, C.c_uint32, C.c_uint32, PU64])
+        _sig(lib, "xmco_weight_integral", D, [D, D, D, C.c_uint])
+        _sig(lib, "xmco_expf_many", None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64])
         _sig(lib, "xmco_mcg59_next", D, [PU64, U64])
         _sig(lib, "xmco_specular", D, [P])
         _sig(lib, "xmco_cosine_theta", D, [D, PU64, U64])
@@ -106,6 +108,7 @@ def ref():
         _sig(lib, "xref_step", D, [D, D, PU64, U64])
         _sig(lib, "xref_triangle_distance", D, [PD, PD, PD, PD, PD])
         _sig(lib, "xref_detector_id", I, [P, PD, PD, C.POINTER(C.c_uint8)])
+        _sig(lib, "ref_weight_integral", D, [D, D, D, C.c_uint])
         _ref = lib
     return _ref
 

@@ -1,0 +1,132 @@
+"""Attractive-vector biasing on the CPU: the weight integral
+(ComputeWeightIntegral, mcml_integration.cpp:7-82) against the reference's
+own unit-test values (McmlIntegrationTestCase.cpp:7-38), its closed forms and
+the reference build; the oracle's biased walk against the reference's
+ComputePhoton with useBiasing = 1, photon for photon on the same streams; the
+32-bit mode's exp against libm.  No GPU."""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from xmcml_amd import scenes
+from xmcml_amd.abi import TERMINATE_KILL, InputHandle, OutputHandle
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "reference_unit_cases.json").read_text())["weight_integral"]
+
+
+def W(dot, g, eps, n):
+    return ol.oracle().xmco_weight_integral(dot, g, eps, n)
+
+
+@pytest.mark.parametrize("case", GOLD, ids=[c["source"] for c in GOLD])
+def test_weight_integral_reference_unit_values(case):
+    want = case.get("expected", case.get("expected_times_2pi", 0.0) / (2.0 * np.pi))
+    got = W(case["dot"], case["anisotropy"], case["eps"], case["n"])
+    assert abs(got - want) <= case["delta"], (got, want)
+
+
+def test_weight_integral_closed_forms():
+    """eps = 0: the Henyey-Greenstein density integrates to 1 whatever dot; eps = 1:
+    E[0.1 + 0.45 (1 + a.d')] = 0.55 + 0.45 g dot (the mean cosine of HG is g)."""
+    for g in (0.0, 0.5, 0.8):
+        for dot in (-0.9, -0.2, 0.3, 1.0):
+            assert abs(W(dot, g, 0, 256) - 1.0) < 1e-9
+            assert abs(W(dot, g, 1, 256) - (0.55 + 0.45 * g * dot)) < 1e-9
+
+
+@pytest.mark.skipif(not ol.ref_available(), reason="reference build oracle/_ref is not present")
+def test_weight_integral_equals_reference_build():
+    for dot, g, eps, n in [(0.4, 0.9, 0.0, 64), (-0.875, 0.9, 2.0, 128), (0.11, 0.95, 1.5, 96), (1.0, 0.7, 3.0, 32)]:
+        assert W(dot, g, eps, n) == ol.ref().ref_weight_integral(dot, g, eps, n)
+
+
+def small_tables(spec, factor, n=32, points=512):
+    """Weight tables from the oracle's restatement at low precision (the
+    parity tests need equal tables on both sides, not accurate ones)."""
+    x = -1.0 + np.arange(points + 1) * (2.0 / points)
+    return [(g, np.array([W(v, g, factor, n) for v in x])) for g in scenes.distinct_anisotropies(spec)]
+
+
+def biased_slab(n_photons, factor=1.0, target=(3.0, 0.0, 0.0)):
+    s = scenes.slab_scene(n_photons)
+    s.termination = TERMINATE_KILL  # ComputePhoton kills below minWeight (mcml_kernel.cpp:121-124)
+    return scenes.biased(s, target=target, factor=factor, tables=small_tables(s, factor))
+
+
+@pytest.mark.skipif(not ol.ref_available(), reason="reference build oracle/_ref is not present")
+@pytest.mark.parametrize("factor", [1.0, 2.0])
+def test_biased_walk_equals_reference(factor):
+    """The 64-bit oracle and the reference's ComputePhoton with biasing, on
+    the same per-photon streams: bit-identical tallies."""
+    spec = biased_slab(300, factor)
+    inp = InputHandle(spec)
+    a = OutputHandle(spec)
+    b = OutputHandle(spec)
+    ol.run_oracle(spec, inp, a, 11, 0, 300, 64, 1)  # one thread: the same summation order
+    assert ol.ref().xref_run(inp.ptr, b.ptr, 11, 0, 300, 0) == 0
+    assert np.array_equal(a.absorption, b.absorption)
+    assert np.array_equal(a.det_weight(), b.det_weight()) and np.array_equal(a.det_count(), b.det_count())
+    assert np.array_equal(a.bin_weight(), b.bin_weight())
+    assert a.det_count().sum() > 0
+
+
+def test_biasing_changes_the_walk_but_not_the_mean():
+    """Importance sampling: biased and plain walks of the 64-bit oracle agree on
+    the total detected weight and absorbed weight within 3 combined standard
+    errors (independent batches), although the biased walk takes other paths
+    (other detected packet counts per ring).  Attractive factor 0.2: the
+    per-scattering weight factor W/p has mean 1 but multiplies over hundreds of
+    scatterings, and at factor 1 its product's tail defeats finite samples."""
+    plain = scenes.slab_scene(0)
+    bias = scenes.biased(plain, target=(2.0, 0.0, 0.0), factor=0.2, tables=small_tables(plain, 0.2, n=64))
+    res = {}
+    for name, spec in (("plain", plain), ("biased", bias)):
+        inp = InputHandle(spec)
+        R, A, counts = [], [], np.zeros(spec.num_detectors)
+        nb, n = 16, 5000
+        for k in range(nb):
+            o = OutputHandle(spec, absorption=False)
+            c = ol.run_oracle(spec, inp, o, 100 + k + (50 if spec.use_biasing else 0), 0, n, 64, 8)
+            R.append(o.det_weight().sum() / n)
+            A.append(c.wAbsorbed / n)
+            counts += o.det_count()
+        res[name] = (np.mean(R), np.std(R, ddof=1) / np.sqrt(nb), np.mean(A), np.std(A, ddof=1) / np.sqrt(nb), counts)
+    (r0, e0, a0, f0, c0), (r1, e1, a1, f1, c1) = res["plain"], res["biased"]
+    assert abs(r0 - r1) < 3 * np.hypot(e0, e1), (r0, r1, e0, e1)
+    assert abs(a0 - a1) < 3 * np.hypot(f0, f1), (a0, a1, f0, f1)
+    assert not np.array_equal(c0, c1)
+
+
+def test_weight_table_is_the_mean_acceptance():
+    """The table entry W(d) is E[p] over Henyey-Greenstein deflections of a
+    direction at cosine d to the attractive vector (the normalisation that
+    makes E[W/p | accepted] = 1): 4e5 sampled deflections per d, 4 sigma."""
+    rng = np.random.default_rng(3)
+    g, f = 0.9, 1.7
+    a = np.array([0.0, 0.0, -1.0])
+    for d in (-0.8, 0.0, 0.6):
+        o = np.array([np.sqrt(1 - d * d), 0.0, -d])
+        u = rng.random(400_000)
+        t = (1 - g * g) / (1 - g + 2 * g * u)
+        c = (1 + g * g - t * t) / (2 * g)
+        s = np.sqrt(1 - c * c)
+        ph = 2 * np.pi * rng.random(u.size)
+        e1 = np.cross(o, [0.0, 1.0, 0.0] if abs(o[0]) > 0.9 else [1.0, 0.0, 0.0])
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(o, e1)
+        n = c[:, None] * o + (s * np.cos(ph))[:, None] * e1 + (s * np.sin(ph))[:, None] * e2
+        p = (0.1 + 0.45 * (1 + n @ a)) ** f
+        assert abs(p.mean() - W(d, g, f, 256)) < 4 * p.std() / np.sqrt(u.size)
+
+
+def test_float_exp_against_libm():
+    x = np.concatenate([np.linspace(-86.9, 0.0, 200001), np.linspace(0.0, 87.9, 2001)]).astype(np.float32)
+    y = np.zeros_like(x)
+    ol.oracle().xmco_expf_many(x.ctypes.data_as(ol.C.POINTER(ol.C.c_float)),
+                               y.ctypes.data_as(ol.C.POINTER(ol.C.c_float)), x.size)
+    ref = np.exp(x.astype(np.float64))
+    rel = np.abs(y.astype(np.float64) - ref) / ref
+    assert rel.max() < 4 * 2.0 ** -23, rel.max()

@@ -72,6 +72,10 @@ class RingDetector(C.Structure):
     ]
 
 
+class WeightTable(C.Structure):
+    _fields_ = [("anisotropy", C.c_double), ("numberOfElements", C.c_int), ("elements", C.POINTER(C.c_double))]
+
+
 class Input(C.Structure):
     _fields_ = [
         ("numberOfPhotons", C.c_uint64),
@@ -93,6 +97,13 @@ class Input(C.Structure):
         ("targetRangeLayers", C.c_uint8 * MAX_LAYERS),
         ("terminationMode", C.c_int),
         ("rouletteChance", C.c_double),
+        ("useBiasing", C.c_int),
+        ("attractiveFactor", C.c_double),
+        ("targetPoint", Double3),
+        ("weightIntegralPrecision", C.c_int),
+        ("weightTablePrecision", C.c_int),
+        ("weightTable", C.POINTER(WeightTable)),
+        ("numberOfWeightTables", C.c_int),
     ]
 
 
@@ -255,6 +266,14 @@ class SceneSpec:
     target_range_layers: Sequence[int] = ()
     photons: int = 0
     name: str = "scene"
+    # attractive-vector biasing (InputInfo.useBiasing..targetPoint); weight_tables:
+    # [(anisotropy, elements)], empty = computed by the library
+    use_biasing: bool = False
+    attractive_factor: float = 1.0
+    target_point: Sequence[float] = (0.0, 0.0, 0.0)
+    weight_integral_precision: int = 128
+    weight_table_precision: int = 262144
+    weight_tables: List[tuple] = field(default_factory=list)
 
     @property
     def grid_size(self) -> int:
@@ -332,6 +351,22 @@ class InputHandle:
             inp.targetRangeLayers[j] = 1
         inp.terminationMode = spec.termination
         inp.rouletteChance = spec.roulette_chance
+        inp.useBiasing = int(bool(spec.use_biasing))
+        inp.attractiveFactor = float(spec.attractive_factor)
+        inp.targetPoint = _d3(spec.target_point)
+        inp.weightIntegralPrecision = int(spec.weight_integral_precision)
+        inp.weightTablePrecision = int(spec.weight_table_precision)
+        if spec.weight_tables:
+            tabs = (WeightTable * len(spec.weight_tables))()
+            for i, (g, el) in enumerate(spec.weight_tables):
+                el = np.ascontiguousarray(el, dtype=np.float64)
+                keep.append(el)
+                tabs[i].anisotropy = float(g)
+                tabs[i].numberOfElements = el.size
+                tabs[i].elements = el.ctypes.data_as(C.POINTER(C.c_double))
+            inp.weightTable = tabs
+            inp.numberOfWeightTables = len(spec.weight_tables)
+            self._tabs = tabs
         self._keep = keep
         self._surfs, self._layers, self._rings, self._cubes = surfs, layers, rings, cubes
         self.c = inp

@@ -112,6 +112,7 @@ struct xmc_scene {
     std::vector<int> perm;  // (surface, index) order -> leaf-order position
     DevBuf<xmc::GpuNode> nodes;
     DevBuf<xmc::GpuTri> tris;
+    DevBuf<float> bias_table;  // weight-integral tables of the biasing, float
     DevBuf<float> normals;  // 4 per triangle
     DevBuf<int8_t> next_layer;
     DevBuf<float> cube, ring;
@@ -170,6 +171,75 @@ int validate(const xmc_input* in) {
         return fail(XMC_ERR_ARG, "unknown terminationMode");
     if (in->terminationMode == XMC_TERMINATE_ROULETTE && !(in->rouletteChance > 0 && in->rouletteChance <= 1))
         return fail(XMC_ERR_ARG, "rouletteChance must be in (0, 1]");
+    return XMC_OK;
+}
+
+// Attractive-vector biasing: the weight-integral tables (the caller's, or computed on
+// the GPU for every distinct anisotropy of layers 1.., GetDifferentAnisotropies,
+// weight_integral.cpp:67-90), each layer's table (the last one whose anisotropy equals
+// the layer's, GetWeightTableIndex, mcml_kernel.cpp:436-446) and the per-layer float
+// constants of GetLocalAttractiveVector (:325-362).
+int setup_bias(const xmc_input* in, xmc_scene* sc, xmc::Params& P) {
+    if (!(in->attractiveFactor == in->attractiveFactor)) return fail(XMC_ERR_ARG, "attractiveFactor is NaN");
+    std::vector<double> gs;
+    for (int l = 1; l < in->numberOfLayers; l++) {
+        double g = in->layerInfo[l].anisotropy;
+        if (std::find(gs.begin(), gs.end(), g) == gs.end()) gs.push_back(g);
+    }
+    std::vector<xmc_weight_table> tables;
+    std::vector<std::vector<double>> own;
+    if (in->weightTable && in->numberOfWeightTables > 0) {
+        for (int t = 0; t < in->numberOfWeightTables; t++) {
+            const xmc_weight_table& w = in->weightTable[t];
+            if (!w.elements || w.numberOfElements < 2) return fail(XMC_ERR_ARG, "weight table with < 2 elements");
+            tables.push_back(w);
+        }
+    } else {
+        const int bp = in->weightIntegralPrecision > 0 ? in->weightIntegralPrecision : 128;
+        const int tp = in->weightTablePrecision > 0 ? in->weightTablePrecision : 262144;
+        for (double g : gs) {
+            own.emplace_back((size_t)tp + 1);
+            int rc = xmc_weight_table_compute(g, in->attractiveFactor, bp, tp, own.back().data(), sc->device);
+            if (rc) return rc;
+            tables.push_back({g, tp + 1, own.back().data()});
+        }
+    }
+    std::vector<float> flat;
+    std::vector<int> first(tables.size());
+    for (size_t t = 0; t < tables.size(); t++) {
+        first[t] = (int)flat.size();
+        for (int e = 0; e < tables[t].numberOfElements; e++) flat.push_back((float)tables[t].elements[e]);
+    }
+    xmc::Bias& B = P.bias;
+    std::memset(&B, 0, sizeof(B));
+    for (int l = 1; l < in->numberOfLayers; l++) {
+        const xmc_layer& L = in->layerInfo[l];
+        if (L.absorptionCoefficient == 0.0 && L.scatteringCoefficient == 0.0) continue;  // glass: no scattering
+        int k = -1;
+        for (size_t t = 0; t < tables.size(); t++)
+            if (tables[t].anisotropy == L.anisotropy) k = (int)t;
+        if (k < 0) return fail(XMC_ERR_ARG, "no weight table for the anisotropy of layer " + std::to_string(l));
+        B.off[l] = first[k];
+        B.last[l] = tables[k].numberOfElements - 1;
+        B.half_n[l] = 0.5f * (float)B.last[l];
+    }
+    HIP_TRY(sc->bias_table.alloc(flat.size()));
+    HIP_TRY(hipMemcpy(sc->bias_table.p, flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice));
+    B.table = sc->bias_table.p;
+    B.on = 1;
+    B.factor = (float)in->attractiveFactor;
+    for (int a = 0; a < 3; a++) {
+        const double t = a == 0 ? in->targetPoint.x : (a == 1 ? in->targetPoint.y : in->targetPoint.z);
+        B.rd[a] = (float)t - P.start_pos[a];
+    }
+    for (int l = 0; l < in->numberOfLayers; l++) {  // the float steps of :329-347
+        float g = P.layers.g[l], mus = P.layers.mus[l], mua = P.layers.mua[l];
+        float D = 1.0f / (3.0f * ((1.0f - g) * mus + mua));
+        float ltr = 3.0f * D;
+        float delta = 0.74f * ltr;
+        B.r1z[l] = B.rd[2] + ltr;
+        B.r2z[l] = B.rd[2] - (ltr + 2.0f * delta);
+    }
     return XMC_OK;
 }
 
@@ -565,6 +635,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     P.min_weight = (float)in->minWeight;
     P.chance = (float)in->rouletteChance;
     P.roulette = in->terminationMode == XMC_TERMINATE_ROULETTE;
+    if (in->useBiasing && (rc = setup_bias(in, sc, P))) return rc;
 
     sc->block = o.blockSize > 0 ? o.blockSize : 256;
     if (launchable) {

@@ -652,6 +652,11 @@ int fill_document(const XElem& root, xmc_document& d) {
         return set_error(XMC_ERR_ARG, "fewer <CubeDetector> elements than NumberOfCubeDetectors");
     if (declared_rings > 0 && (!ring_section || (int)ring_map.size() != declared_rings))
         return set_error(XMC_ERR_ARG, "fewer <RingDetector> elements than NumberOfRingDetectors");
+    in.useBiasing = d.bias.useBiasing;
+    in.attractiveFactor = d.bias.attractiveFactor;
+    in.targetPoint = d.bias.targetPoint;
+    in.weightIntegralPrecision = d.bias.weightIntegralPrecision;
+    in.weightTablePrecision = d.bias.weightTablePrecision;
     for (auto& kv : cube_map) d.cubes.push_back(kv.second);
     for (auto& kv : ring_map) d.rings.push_back(kv.second);
     in.numberOfCubeDetectors = (int)d.cubes.size();

@@ -111,6 +111,23 @@ struct AbsTile {
     int count;
 };
 
+// Attractive-vector biasing (mcml_kernel.cpp:325-440) in float: the
+// attractive vector of a photon at p in layer l is
+// normalize(t2 / |t2|^3 - t1 / |t1|^3), t_k = (p - start) - (rd.x, rd.y, rz_k[l]),
+// rd = target - start, r1z = rd.z + ltr, r2z = rd.z - (ltr + 2 delta) with
+// ltr = 3 / (3 ((1 - g) mus + mua)), delta = 0.74 ltr; the weight factor of a
+// deflection from direction d is table[off[l] + round((d.a + 1) half_n[l])] / p.
+struct Bias {
+    int on;
+    float factor;               // attractiveFactor (the acceptance probability's exponent)
+    float rd[3];
+    float r1z[kMaxLayers], r2z[kMaxLayers];
+    const float* table;         // every layer's weight-integral table, float
+    int off[kMaxLayers];        // first entry of layer l's table
+    int last[kMaxLayers];       // its last index (entries - 1)
+    float half_n[kMaxLayers];   // 0.5 * last
+};
+
 struct Params {
     const GpuNode* nodes;
     const GpuTri* tris;
@@ -140,6 +157,7 @@ struct Params {
     Trace trace;
     Paths paths;
     AbsTile abs_tile;
+    Bias bias;
 };
 
 struct QueryParams {

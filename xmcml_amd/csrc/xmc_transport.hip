@@ -59,6 +59,26 @@ __device__ __forceinline__ float xlogf(float x) {
     return __builtin_fmaf(kf, 0.693145751953125f, __builtin_fmaf(kf, 1.42860677e-06f, l1p));
 }
 
+// exp(x): x = k ln2 + r with k = rint(x log2 e) and the same two-part ln2 as
+// xlogf (Cody-Waite), exp(r) for |r| <= ln2/2 by its Taylor series to r^7
+// (truncation below 6e-9 relative), scaled by 2^k through the exponent bits.
+// Below -87 the result is taken as 0 (the normal range ends near -87.3).
+__device__ __forceinline__ float xexpf(float x) {
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) return kInf;
+    float kf = __builtin_rintf(x * 1.44269504f);
+    float r = __builtin_fmaf(kf, -0.693145751953125f, x);
+    r = __builtin_fmaf(kf, -1.42860677e-06f, r);
+    float p = __builtin_fmaf(r, 1.0f / 5040.0f, 1.0f / 720.0f);
+    p = __builtin_fmaf(r, p, 1.0f / 120.0f);
+    p = __builtin_fmaf(r, p, 1.0f / 24.0f);
+    p = __builtin_fmaf(r, p, 1.0f / 6.0f);
+    p = __builtin_fmaf(r, p, 0.5f);
+    p = __builtin_fmaf(r, p, 1.0f);
+    p = __builtin_fmaf(r, p, 1.0f);
+    return p * __uint_as_float((uint32_t)((int)kf + 127) << 23);
+}
+
 // sin and cos of 2 pi u, u in [0, 1): exact quadrant split q = rint(4u),
 // theta = (4u - q) pi/2 in [-pi/4, pi/4], Taylor series to theta^9 / theta^10.
 __device__ __forceinline__ void xsincos2pi(float u, float& sp, float& cp) {
@@ -539,6 +559,21 @@ __device__ __forceinline__ V3 spin(V3 d, float g, uint64_t& x, uint32_t& draws) 
     return r;
 }
 
+// GetLocalAttractiveVector (:325-362) in float: the unit vector along
+// t2 / |t2|^3 - t1 / |t1|^3 of the photon at p in layer l (constants in Bias).
+__device__ __forceinline__ V3 attractive(const Bias& B, V3 start, V3 p, int l) {
+    float rx = p.x - start.x, ry = p.y - start.y, rz = p.z - start.z;
+    V3 t1 = {rx - B.rd[0], ry - B.rd[1], rz - B.r1z[l]};
+    float l1 = __builtin_sqrtf(dot(t1, t1));
+    float c1 = l1 * l1 * l1;
+    t1 = {t1.x / c1, t1.y / c1, t1.z / c1};
+    V3 t2 = {rx - B.rd[0], ry - B.rd[1], rz - B.r2z[l]};
+    float l2 = __builtin_sqrtf(dot(t2, t2));
+    float c2 = l2 * l2 * l2;
+    t2 = {t2.x / c2, t2.y / c2, t2.z / c2};
+    return normalize(sub(t2, t1));
+}
+
 // GetAreaIndex (:190-227) in float, with partitions per unit length (scale)
 // in place of the division; -1 outside.  Partitions are <= 255.
 __device__ __forceinline__ int area_index(const Area& A, V3 p, int& bx, int& by, int& bz) {
@@ -902,18 +937,35 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                 n_culled++;
                 phase = kResolve;
             } else {
-                Geo geo{C.nodes, C.tris, C.root_ref, {C.root_lo[0], C.root_lo[1], C.root_lo[2]},
-                        {C.root_hi[0], C.root_hi[1], C.root_hi[2]}};
-                if (e >= 0) {  // start below the root: same leaves, same order, same hit
-                    geo.root_ref = C.dist.entry_ref[e];
-                    const float* b = C.dist.entry_box + 6 * e;
-                    geo.rlo[0] = b[0];
-                    geo.rlo[1] = b[1];
-                    geo.rlo[2] = b[2];
-                    geo.rhi[0] = b[3];
-                    geo.rhi[1] = b[4];
-                    geo.rhi[2] = b[5];
-                    n_entry++;
+                // the root's box as values.  A single scene's are kernel arguments, uniform over
+                // the wave: readfirstlane stops the compiler from turning the choice between them
+                // and an entry's box into a load through a chosen pointer (which would copy the
+                // kernel arguments into private memory).  A batch lane reads its own configuration.
+                auto u = [](float v) {
+                    if constexpr (BATCHED) return v;
+                    else return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+                };
+                int root = C.root_ref;
+                if constexpr (!BATCHED) root = __builtin_amdgcn_readfirstlane(root);
+                Geo geo{C.nodes, C.tris, root, {u(C.root_lo[0]), u(C.root_lo[1]), u(C.root_lo[2])},
+                        {u(C.root_hi[0]), u(C.root_hi[1]), u(C.root_hi[2])}};
+                if constexpr (INTERLEAVE) {
+                    // start below the root (same leaves, same order, same hit); the entry table
+                    // always has an entry 0 when this variant runs
+                    const int ei = e >= 0 ? e : 0;
+                    const int ref = C.dist.entry_ref[ei];
+                    const float* b = C.dist.entry_box + 6 * ei;
+                    const float b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3], b4 = b[4], b5 = b[5];
+                    if (e >= 0) {
+                        geo.root_ref = ref;
+                        geo.rlo[0] = b0;
+                        geo.rlo[1] = b1;
+                        geo.rlo[2] = b2;
+                        geo.rhi[0] = b3;
+                        geo.rhi[1] = b4;
+                        geo.rhi[2] = b5;
+                        n_entry++;
+                    }
                 }
                 Ray ray = make_ray(pos, dir);
                 if constexpr (INTERLEAVE) {
@@ -1073,7 +1125,28 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                     if (!in_tile) unsafeAtomicAdd(C.tally.absorption + idx, (double)dw);
                 }
                 w_abs += (double)dw;
-                dir = spin(dir, Lg, x, draws);
+                if (C.bias.on) {
+                    // ComputePhotonDirectionWithBiasing (:398-418): draw deflections until one is
+                    // accepted with probability (0.1 + 0.45 (1 + d.a))^factor, then weight it by
+                    // the weight integral of the old direction (GetNewPhotonWeight, :420-434)
+                    V3 A = attractive(C.bias, V3{C.start_pos[0], C.start_pos[1], C.start_pos[2]}, pos, layer);
+                    float d0 = dot(dir, A);
+                    V3 nd;
+                    float p, u;
+                    do {
+                        nd = spin(dir, Lg, x, draws);
+                        p = 0.1f + 0.9f * ((1.0f + dot(nd, A)) / 2.0f);
+                        if (C.bias.factor != 1.0f) p = xexpf(C.bias.factor * xlogf(p));
+                        u = next_u(x);
+                        draws++;
+                    } while (p <= u && draws <= kDrawLimit);
+                    int k = (int)__builtin_floorf((d0 + 1.0f) * C.bias.half_n[layer] + 0.5f);
+                    k = k < 0 ? 0 : (k > C.bias.last[layer] ? C.bias.last[layer] : k);
+                    w = w * C.bias.table[C.bias.off[layer] + k] / p;
+                    dir = nd;
+                } else {
+                    dir = spin(dir, Lg, x, draws);
+                }
                 n_scatter++;
                 rec.put(XMC_EV_SCATTER, layer, idx, draws, pos, dir, w, tim);
             } else {

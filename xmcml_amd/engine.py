@@ -36,6 +36,7 @@ EXPORTS = [
     "xmc_mlp_create", "xmc_mlp_destroy", "xmc_mlp_param_count", "xmc_mlp_get_params", "xmc_mlp_set_params",
     "xmc_mlp_forward", "xmc_mlp_backward", "xmc_mlp_get_grads", "xmc_mlp_adam", "xmc_mlp_train_step",
     "xmc_mlp_gemm_check", "xmc_mlp_train_sampled", "xmc_dev_alloc", "xmc_dev_free", "xmc_dev_copy", "xmc_dev_sync",
+    "xmc_weight_integrals", "xmc_weight_table_compute",
     # file formats and backups (include/xmcml_io.h)
     "xmc_read_input_file", "xmc_read_surface_file", "xmc_document_input", "xmc_document_biasing",
     "xmc_document_free", "xmc_write_surface_file", "xmc_write_output_file", "xmc_write_backup_file",
@@ -106,6 +107,8 @@ def lib():
     sig("xmc_dev_free", _I, [_P])
     sig("xmc_dev_copy", _I, [_P, _P, _U64, _I])
     sig("xmc_dev_sync", _I, [_I])
+    sig("xmc_weight_integrals", _I, [_PD, _I, C.c_double, C.c_double, _I, _PD, _I])
+    sig("xmc_weight_table_compute", _I, [C.c_double, C.c_double, _I, _I, _PD, _I])
     _S = C.c_char_p
     sig("xmc_read_input_file", _I, [_S, C.POINTER(_P)])
     sig("xmc_read_surface_file", _I, [_P, _S])
@@ -289,6 +292,24 @@ def distance_grid(spec: SceneSpec, target_cells: int = 0) -> dict:
            "xmc_distance_grid")
     return dict(lo=np.array(lo[:], np.float32), h=h.value, step_q=q.value, dims=tuple(dims[:]),
                 q=grid.reshape(dims[2], dims[1], dims[0]))
+
+
+def weight_integrals(dots, anisotropy: float, attractive_factor: float, precision: int, device: int = 0):
+    """ComputeWeightIntegral at each dot, on the GPU (FP64)."""
+    d = np.ascontiguousarray(dots, dtype=np.float64)
+    out = np.zeros(d.size)
+    _check(lib().xmc_weight_integrals(_dp(d), d.size, anisotropy, attractive_factor, precision, _dp(out), device),
+           "xmc_weight_integrals")
+    return out
+
+
+def weight_table(anisotropy: float, attractive_factor: float = 1.0, integral_precision: int = 128,
+                 table_precision: int = 262144, device: int = 0):
+    """The biasing's weight-integral table of one anisotropy (tablePrecision + 1 entries)."""
+    out = np.zeros(table_precision + 1)
+    _check(lib().xmc_weight_table_compute(anisotropy, attractive_factor, integral_precision, table_precision,
+                                          _dp(out), device), "xmc_weight_table_compute")
+    return out
 
 
 def launch(spec: SceneSpec, out: OutputHandle, seed: int, first: int, n: int, absorption: bool = True) -> int:

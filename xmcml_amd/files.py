@@ -109,7 +109,10 @@ def input_to_spec(x: Input, termination: int = TERMINATE_KILL, roulette_chance: 
         time_bins=x.timeScaleSize, start_position=(x.startPosition.x, x.startPosition.y, x.startPosition.z),
         start_direction=(x.startDirection.x, x.startDirection.y, x.startDirection.z),
         target_range_layers=[j for j in range(MAX_LAYERS) if x.targetRangeLayers[j]],
-        photons=int(x.numberOfPhotons), name="file")
+        photons=int(x.numberOfPhotons), name="file",
+        use_biasing=bool(x.useBiasing), attractive_factor=x.attractiveFactor,
+        target_point=(x.targetPoint.x, x.targetPoint.y, x.targetPoint.z),
+        weight_integral_precision=x.weightIntegralPrecision, weight_table_precision=x.weightTablePrecision)
 
 
 def _xyz(tag: str, v, ind: str) -> str:

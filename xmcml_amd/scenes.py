@@ -327,3 +327,29 @@ def batched_configs(count: int = 1024, seed: int = 3, photon_scale: float = 1.0,
         budget = math.exp(rng.uniform(math.log(min_photons), math.log(max_photons)))
         out.append(layered_scene(layers, max(1, int(budget * photon_scale)), name=f"batch-{seed}-{c}"))
     return out
+
+
+def biased(spec: SceneSpec, target=(3.0, 0.0, 0.0), factor: float = 1.0, tables=None,
+           integral_precision: int = 128, table_precision: int = 262144) -> SceneSpec:
+    """The same input with attractive-vector biasing towards `target`
+    (InputInfo.useBiasing..targetPoint); tables = [(anisotropy, elements)] or
+    None for tables the library computes at the given precisions."""
+    import copy
+    s = copy.copy(spec)
+    s.use_biasing = True
+    s.attractive_factor = factor
+    s.target_point = tuple(float(v) for v in target)
+    s.weight_integral_precision = integral_precision
+    s.weight_table_precision = table_precision
+    s.weight_tables = list(tables or [])
+    s.name = spec.name + "-biased"
+    return s
+
+
+def distinct_anisotropies(spec: SceneSpec):
+    """GetDifferentAnisotropies (weight_integral.cpp:67-90): layers 1.., first seen first."""
+    out = []
+    for l in spec.layers[1:]:
+        if l.g not in out:
+            out.append(l.g)
+    return out
