@@ -20,6 +20,9 @@
  *       -> xmc_read_backup_file()
  *   LaunchOMPWithBackups + DoBackup (base/xmcml/xmcmlLauncher/launcher_mpi.cpp:311-380)
  *       -> xmc_launch_with_backups()
+ *   WriteWeightIntegralTableToFile / ParseWeightIntegralFile
+ *                     (base/xmcml/xmcmlLauncher/weight_integral.cpp:105-247)
+ *       -> xmc_write_weight_table_file() / xmc_read_weight_table_file()
  *
  * Differences from the reference, all deliberate:
  *   - Every function returns an int status (XMC_OK or a negative code, the
@@ -60,11 +63,12 @@ extern "C" {
 #define XMC_SECTION_DETECTOR_RANGES 0x0B
 #define XMC_SECTION_SCATTERING 0x0C
 #define XMC_SECTION_SURFACES 0x0100
+#define XMC_SECTION_WEIGHT_INTEGRAL 0x0200
 
 /* The <Biasing> block of an input file (InputInfo.useBiasing..targetPoint,
- * mcml_kernel_types.h:74-80; defaults parser.cpp:10-12,32-41).  The engine
- * does not run attractive-vector biasing; the values are parsed so that a
- * caller can see that the file asks for it. */
+ * mcml_kernel_types.h:74-80; defaults parser.cpp:10-12,32-41).  The same
+ * values are set in the document's xmc_input, so a launch of it runs
+ * attractive-vector biasing when the file asks for it. */
 typedef struct xmc_biasing {
     int useBiasing;
     int weightIntegralPrecision; /* default 128 */
@@ -117,6 +121,18 @@ int xmc_backup_file_name(uint64_t numberOfPhotons, char* buffer, int bufferSize)
  * count and time-bin edges are checked); specularReflectance is not read
  * (reader.cpp:375-378 skips it).  rng receives (seed, next photon). */
 int xmc_read_backup_file(const char* fileName, xmc_output* output, xmc_rng* rng);
+
+/* The biasing's weight tables (the launcher's -t file): section 0x0200 with
+ * the attractive factor, the integral precision, the number of tables and per
+ * table (anisotropy, number of elements, elements).  The writer takes
+ * input->weightTable (fileName NULL = the reference's
+ * "xmcml_weight_integral_table.bin" in the working directory); the reader
+ * checks the file against the document's input as ParseWeightIntegralFile
+ * does (same attractive factor and precisions, a table within 1e-6 of every
+ * distinct anisotropy of layers 1..) and makes the tables the input's
+ * weightTable. */
+int xmc_write_weight_table_file(const xmc_input* input, const char* fileName);
+int xmc_read_weight_table_file(xmc_document* doc, const char* fileName);
 
 /* LaunchOMPWithBackups + DoBackup for one process: runs input->numberOfPhotons
  * photons from rng->nextPhoton in portions of portionSize on one scene; after

@@ -1,7 +1,8 @@
 /*
  * ref_io_driver.cpp -- drives the reference's own file readers and writers
  * (ParseInputFile, ParseSurfaceFile, WriteOutputToFile, WriteBackupToFile,
- * ParseBackupFile; base/xmcml/xmcmlLauncher/{parser,writer,reader}.cpp, with
+ * ParseBackupFile, WriteWeightIntegralTableToFile, ParseWeightIntegralFile;
+ * base/xmcml/xmcmlLauncher/{parser,writer,reader,weight_integral}.cpp, with
  * the tinyxml sources of the same repository) compiled unmodified by
  * oracle/Makefile into oracle/_ref/libxmcml_ref_io.so.
  *
@@ -15,6 +16,7 @@
 #include "parser.h"
 #include "reader.h"
 #include "writer.h"
+#include "weight_integral.h"
 
 #include <cstdint>
 #include <cstring>
@@ -22,6 +24,10 @@
 #include <vector>
 
 #include "../include/xmcml_io.h"
+
+// Defined in weight_integral.cpp without a header declaration (:9-14).
+bool WriteWeightIntegralTableToFile(InputInfo* input);
+bool ParseWeightIntegralFile(InputInfo* input, CmdArguments* args);
 
 namespace {
 
@@ -253,6 +259,59 @@ int refio_read_output(const char* path, xmc_output* o) {
     std::string p(path);
     if (ReadOutputFormBackupFile(&p[0], &ro.out) != 0) return 0;
     ro.copy_back(o);
+    return 1;
+}
+
+// WriteWeightIntegralTableToFile for the given tables (elements concatenated);
+// the file lands in the working directory under the reference's own name.
+int refio_write_weight_table(double factor, int integralPrecision, int numberOfTables, const double* anisotropy,
+                             const int* numberOfElements, const double* elements) {
+    InputInfo in;
+    std::memset(&in, 0, sizeof(in));
+    in.attractiveFactor = factor;
+    in.weightIntegralPrecision = integralPrecision;
+    in.numberOfWeightTables = numberOfTables;
+    std::vector<WeightIntegralTable> t(numberOfTables);
+    for (int i = 0; i < numberOfTables; i++) {
+        t[i].anisotropy = anisotropy[i];
+        t[i].numberOfElements = numberOfElements[i];
+        t[i].elements = const_cast<double*>(elements);
+        elements += numberOfElements[i];
+    }
+    in.weightTable = t.data();
+    return WriteWeightIntegralTableToFile(&in) ? 1 : 0;
+}
+
+// ParseWeightIntegralFile against an input of numberOfLayers layers with the
+// given anisotropies and biasing settings.  On success the tables are copied
+// out (at most capTables tables and capElements elements in all).
+int refio_parse_weight_table(const char* path, int numberOfLayers, const double* layerAnisotropy, double factor,
+                             int integralPrecision, int tablePrecision, int capTables, int capElements,
+                             int* numberOfTables, double* anisotropy, int* numberOfElements, double* elements) {
+    InputInfo in;
+    std::memset(&in, 0, sizeof(in));
+    std::vector<LayerInfo> layers(numberOfLayers);
+    for (int i = 0; i < numberOfLayers; i++) layers[i].anisotropy = layerAnisotropy[i];
+    in.layerInfo = layers.data();
+    in.numberOfLayers = numberOfLayers;
+    in.attractiveFactor = factor;
+    in.weightIntegralPrecision = integralPrecision;
+    in.weightTablePrecision = tablePrecision;
+    CmdArguments args;
+    std::memset(&args, 0, sizeof(args));
+    std::string p(path);
+    args.weightTableFileName = &p[0];
+    if (!ParseWeightIntegralFile(&in, &args)) return 0;
+    *numberOfTables = in.numberOfWeightTables;
+    int used = 0;
+    for (int i = 0; i < in.numberOfWeightTables && i < capTables; i++) {
+        const WeightIntegralTable& w = in.weightTable[i];
+        anisotropy[i] = w.anisotropy;
+        numberOfElements[i] = w.numberOfElements;
+        for (int k = 0; k < w.numberOfElements && used < capElements; k++) elements[used++] = w.elements[k];
+        delete[] w.elements;
+    }
+    delete[] in.weightTable;
     return 1;
 }
 

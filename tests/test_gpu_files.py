@@ -83,3 +83,30 @@ def test_run_files_writes_the_result(engine, slab_files):
     assert out.scattering.sum() > 0  # the result carries the scattering grid, as the reference's does
     whole = _one_launch(doc, spec)
     _close(out, whole)
+
+
+def test_run_files_with_biasing_takes_or_writes_the_table_file(engine, tmp_path, monkeypatch):
+    """GetWeightIntegralTable (weight_integral.cpp:16-31): without a table file
+    the tables are computed on the GPU and written under the reference's name
+    in the working directory; a run given that file (the launcher's -t) reads
+    it back and adds the same photons."""
+    monkeypatch.chdir(tmp_path)
+    spec = scenes.biased(scenes.slab_scene(N), target=(2.0, 0.0, 0.0), factor=0.5, integral_precision=16,
+                         table_precision=1024)
+    files.write_input_xml(spec, tmp_path / "b.xml", photons=N)
+    files.write_surface_file(spec, tmp_path / "b.surface")
+    print("files: biased slab, tables computed and written", flush=True)
+    a = files.run_files(tmp_path / "b.xml", tmp_path / "b.surface", tmp_path / "a.mcml.out", seed=11, portion=N)
+    table = tmp_path / files.WEIGHT_TABLE_FILE
+    assert table.exists()
+    doc = files.Document(tmp_path / "b.xml", tmp_path / "b.surface")
+    doc.read_weight_table_file(table)
+    for g, el in doc.spec().weight_tables:
+        assert np.array_equal(el, engine.weight_table(g, 0.5, 16, 1024))
+    table.rename(tmp_path / "given.bin")
+    print("files: biased slab, tables from the file", flush=True)
+    b = files.run_files(tmp_path / "b.xml", tmp_path / "b.surface", tmp_path / "b.mcml.out", seed=11, portion=N,
+                        weight_table_path=tmp_path / "given.bin")
+    assert not table.exists()
+    _close(a, b)
+    assert a.det_weight().sum() > 0

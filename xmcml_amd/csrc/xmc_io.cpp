@@ -8,7 +8,9 @@
 // are decoded, and comments, declarations and CDATA are understood.
 #include "../../include/xmcml_io.h"
 
+#include <algorithm>
 #include <cerrno>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -253,6 +255,14 @@ struct File {
     bool get1(T& v) {
         return get(&v, 1);
     }
+    // Bytes between the read position and the end of the file (-1 if unknown).
+    long long remaining() {
+        long here = std::ftell(f);
+        if (here < 0 || std::fseek(f, 0, SEEK_END) != 0) return -1;
+        long end = std::ftell(f);
+        if (std::fseek(f, here, SEEK_SET) != 0 || end < here) return -1;
+        return (long long)(end - here);
+    }
 };
 
 bool put_zeros_f64(File& F, size_t n) {
@@ -407,6 +417,8 @@ struct xmc_document {
     std::vector<xmc_cube_detector> cubes;
     std::vector<xmc_ring_detector> rings;
     xmc_biasing bias{};
+    std::vector<std::vector<double>> table_elements;  // weight tables read from a table file
+    std::vector<xmc_weight_table> tables;
 
     void relink() {
         in.area = has_area ? &area : nullptr;
@@ -419,6 +431,9 @@ struct xmc_document {
         in.surface = surfaces.empty() ? nullptr : surfaces.data();
         in.cubeDetector = cubes.empty() ? nullptr : cubes.data();
         in.ringDetector = rings.empty() ? nullptr : rings.data();
+        for (size_t t = 0; t < tables.size(); t++) tables[t].elements = table_elements[t].data();
+        in.weightTable = tables.empty() ? nullptr : tables.data();
+        in.numberOfWeightTables = (int)tables.size();
     }
 };
 
@@ -873,6 +888,67 @@ int xmc_read_backup_file(const char* fileName, xmc_output* out, xmc_rng* rng) {
         d.targetWeight = target[i];
     }
     if (rng) *rng = r;
+    return XMC_OK;
+}
+
+int xmc_write_weight_table_file(const xmc_input* in, const char* fileName) {
+    // WriteWeightIntegralTableToFile, weight_integral.cpp:105-157
+    if (!in || (in->numberOfWeightTables > 0 && !in->weightTable)) return set_error(XMC_ERR_ARG, "input / tables");
+    if (!fileName) fileName = "xmcml_weight_integral_table.bin";
+    File F(fileName, "wb");
+    if (!F.f) return set_error(XMC_ERR_ARG, std::string("cannot create ") + fileName);
+    uint32_t len = 8 + 4 + 4;  // GetWeightIntegralSectionLength, :159-171
+    for (int t = 0; t < in->numberOfWeightTables; t++) len += 8 + 4 + (uint32_t)in->weightTable[t].numberOfElements * 8u;
+    bool ok = header(F, XMC_SECTION_WEIGHT_INTEGRAL, len) && F.put1(in->attractiveFactor) &&
+              F.put1(in->weightIntegralPrecision) && F.put1(in->numberOfWeightTables);
+    for (int t = 0; ok && t < in->numberOfWeightTables; t++) {
+        const xmc_weight_table& w = in->weightTable[t];
+        ok = F.put1(w.anisotropy) && F.put1(w.numberOfElements) && F.put(w.elements, (size_t)w.numberOfElements);
+    }
+    if (!ok || std::fflush(F.f) != 0) return set_error(XMC_ERR_ARG, std::string("write failed: ") + fileName);
+    return XMC_OK;
+}
+
+int xmc_read_weight_table_file(xmc_document* doc, const char* fileName) {
+    // ParseWeightIntegralFile, weight_integral.cpp:173-247: the file must have been made for this
+    // input's attractive factor, integral and table precisions, and hold every distinct anisotropy
+    if (!doc || !fileName) return set_error(XMC_ERR_ARG, "document / file name is NULL");
+    File F(fileName, "rb");
+    if (!F.f) return set_error(XMC_ERR_ARG, std::string("cannot open ") + fileName);
+    auto bad = [&](const std::string& what) { return set_error(XMC_ERR_ARG, std::string(fileName) + ": " + what); };
+    const xmc_input& in = doc->in;
+    if (!expect(F, XMC_SECTION_WEIGHT_INTEGRAL)) return bad("not a weight table file (section id)");
+    double factor = 0.0;
+    int precision = 0, n = 0;
+    if (!F.get1(factor) || factor != in.attractiveFactor) return bad("attractive factor differs from the input's");
+    if (!F.get1(precision) || precision != in.weightIntegralPrecision)
+        return bad("integral precision differs from the input's");
+    std::vector<double> gs;  // GetDifferentAnisotropies, :67-90
+    for (int l = 1; l < in.numberOfLayers; l++) {
+        double g = in.layerInfo[l].anisotropy;
+        if (std::find(gs.begin(), gs.end(), g) == gs.end()) gs.push_back(g);
+    }
+    if (!F.get1(n) || n < (int)gs.size()) return bad("fewer tables than distinct anisotropies");
+    if ((long long)n * 12 > F.remaining()) return bad("truncated table list");  // before sizing from a count
+    std::vector<std::vector<double>> elems((size_t)n);
+    std::vector<xmc_weight_table> tabs((size_t)n);
+    for (int t = 0; t < n; t++) {
+        int m = 0;
+        if (!F.get1(tabs[t].anisotropy) || !F.get1(m)) return bad("truncated table header");
+        if (m - 1 != in.weightTablePrecision) return bad("table size differs from the input's table precision");
+        if (m < 1 || (long long)m * 8 > F.remaining()) return bad("truncated table");
+        elems[t].resize((size_t)m);
+        if (!F.get(elems[t].data(), (size_t)m)) return bad("truncated table");
+        tabs[t].numberOfElements = m;
+    }
+    for (double g : gs) {
+        bool found = false;
+        for (const xmc_weight_table& w : tabs) found = found || std::fabs(w.anisotropy - g) < 1e-6;
+        if (!found) return bad("no table for anisotropy " + std::to_string(g));
+    }
+    doc->table_elements.swap(elems);
+    doc->tables.swap(tabs);
+    doc->relink();
     return XMC_OK;
 }
 

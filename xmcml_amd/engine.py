@@ -40,7 +40,8 @@ EXPORTS = [
     # file formats and backups (include/xmcml_io.h)
     "xmc_read_input_file", "xmc_read_surface_file", "xmc_document_input", "xmc_document_biasing",
     "xmc_document_free", "xmc_write_surface_file", "xmc_write_output_file", "xmc_write_backup_file",
-    "xmc_backup_file_name", "xmc_read_backup_file", "xmc_launch_with_backups",
+    "xmc_backup_file_name", "xmc_read_backup_file", "xmc_launch_with_backups", "xmc_write_weight_table_file",
+    "xmc_read_weight_table_file",
 ]
 
 
@@ -121,6 +122,8 @@ def lib():
     sig("xmc_backup_file_name", _I, [_U64, C.c_char_p, _I])
     sig("xmc_read_backup_file", _I, [_S, _P, _P])
     sig("xmc_launch_with_backups", _I, [_P, _P, _P, _P, _U64, _S])
+    sig("xmc_write_weight_table_file", _I, [_P, _S])
+    sig("xmc_read_weight_table_file", _I, [_P, _S])
     _lib = L
     return L
 

@@ -54,6 +54,13 @@ class Document:
     def ptr(self):
         return engine.lib().xmc_document_input(self._h)
 
+    def read_weight_table_file(self, path) -> None:
+        """ParseWeightIntegralFile: the file's tables become the input's
+        weightTable (the file must match the input's attractive factor and
+        precisions and cover every distinct anisotropy of layers 1..)."""
+        engine._check(engine.lib().xmc_read_weight_table_file(self._h, _b(path)),
+                      f"xmc_read_weight_table_file({path})")
+
     def biasing(self) -> Biasing:
         b = Biasing()
         engine._check(engine.lib().xmc_document_biasing(self._h, C.byref(b)), "xmc_document_biasing")
@@ -112,7 +119,10 @@ def input_to_spec(x: Input, termination: int = TERMINATE_KILL, roulette_chance: 
         photons=int(x.numberOfPhotons), name="file",
         use_biasing=bool(x.useBiasing), attractive_factor=x.attractiveFactor,
         target_point=(x.targetPoint.x, x.targetPoint.y, x.targetPoint.z),
-        weight_integral_precision=x.weightIntegralPrecision, weight_table_precision=x.weightTablePrecision)
+        weight_integral_precision=x.weightIntegralPrecision, weight_table_precision=x.weightTablePrecision,
+        weight_tables=[(x.weightTable[i].anisotropy,
+                        np.ctypeslib.as_array(x.weightTable[i].elements, (x.weightTable[i].numberOfElements,)).copy())
+                       for i in range(x.numberOfWeightTables if x.weightTable else 0)])
 
 
 def _xyz(tag: str, v, ind: str) -> str:
@@ -133,6 +143,17 @@ def write_input_xml(spec: SceneSpec, path, photons: Optional[int] = None) -> Non
     if spec.target_range_layers:
         o.append("    <TargetRange>\n" + "".join(f"        <Layer>{int(j)}</Layer>\n"
                                                 for j in spec.target_range_layers) + "    </TargetRange>\n")
+    if spec.use_biasing:
+        t = [float(v) for v in spec.target_point]
+        if any(v != int(v) for v in t):
+            raise ValueError("the reader takes TargetPoint coordinates as integers (atoi, parser.cpp:233-241)")
+        o.append("    <Biasing>\n        <WeightIntegral>\n"
+                 f"            <IntegralPrecision>{int(spec.weight_integral_precision)}</IntegralPrecision>\n"
+                 f"            <TablePrecision>{int(spec.weight_table_precision)}</TablePrecision>\n"
+                 f"            <AttractiveFactor>{float(spec.attractive_factor)!r}</AttractiveFactor>\n"
+                 "        </WeightIntegral>\n        <TargetPoint>\n"
+                 + "".join(f"            <{a}>{int(v)}</{a}>\n" for a, v in zip("XYZ", t))
+                 + "        </TargetPoint>\n    </Biasing>\n")
     o.append(_xyz("StartPosition", [float(v) for v in spec.start_position], "    "))
     o.append(_xyz("StartDirection", [float(v) for v in spec.start_direction], "    "))
     o.append("    <Area>\n" + _xyz("Corner", [float(v) for v in spec.area_corner], "        ")
@@ -195,6 +216,13 @@ def write_backup_file(inp, out: OutputHandle, rng: Rng, path=None) -> None:
                                                      None if path is None else _b(path)), "xmc_write_backup_file")
 
 
+def write_weight_table_file(inp, path=None) -> None:
+    """WriteWeightIntegralTableToFile of the input's weightTable (path None =
+    "xmcml_weight_integral_table.bin" in the working directory)."""
+    engine._check(engine.lib().xmc_write_weight_table_file(_ptr(inp), None if path is None else _b(path)),
+                  "xmc_write_weight_table_file")
+
+
 def backup_file_name(photons: int) -> str:
     buf = C.create_string_buffer(64)
     engine._check(engine.lib().xmc_backup_file_name(int(photons), buf, 64), "xmc_backup_file_name")
@@ -222,12 +250,39 @@ def _ptr(inp):
     return inp.ptr
 
 
+WEIGHT_TABLE_FILE = "xmcml_weight_integral_table.bin"
+
+
+def get_weight_tables(doc: Document, weight_table_path=None, device: int = 0) -> None:
+    """GetWeightIntegralTable (weight_integral.cpp:16-31) for a document that
+    asks for biasing: read the tables from a table file, or compute them on
+    the GPU for every distinct anisotropy at the document's precisions, write
+    them to "xmcml_weight_integral_table.bin" in the working directory and
+    take them from there."""
+    if weight_table_path is not None:
+        doc.read_weight_table_file(weight_table_path)
+        return
+    from .scenes import distinct_anisotropies
+    x = doc.input
+    spec = doc.spec()
+    spec.weight_tables = [(g, engine.weight_table(g, x.attractiveFactor, x.weightIntegralPrecision,
+                                                  x.weightTablePrecision, device))
+                          for g in distinct_anisotropies(spec)]
+    write_weight_table_file(InputHandle(spec), None)
+    doc.read_weight_table_file(WEIGHT_TABLE_FILE)
+
+
 def run_files(xml_path, surface_path, output_path, seed: int = 777, portion: int = 10_000_000,
-              backup_path=None, resume_from=None, options: Optional[Options] = None) -> OutputHandle:
-    """main.cpp:63-177 on the GPU: parse, optionally resume from a backup,
-    run in backup portions, write the result file.  Returns the output."""
+              backup_path=None, resume_from=None, options: Optional[Options] = None,
+              weight_table_path=None) -> OutputHandle:
+    """main.cpp:63-177 on the GPU: parse, take the biasing's weight tables
+    (from weight_table_path, the launcher's -t file, or computed and written
+    to the working directory), optionally resume from a backup, run in backup
+    portions, write the result file.  Returns the output."""
     doc = Document(xml_path, surface_path)
     x = doc.input
+    if x.useBiasing:
+        get_weight_tables(doc, weight_table_path)
     spec = doc.spec()
     out = OutputHandle(spec, absorption=True, scattering=True)
     rng = Rng(seed, 0)
