@@ -284,6 +284,33 @@ def test_walk_skipping_grids_change_nothing(engine, grids):
     np.testing.assert_allclose(a.det_weight(), b.det_weight(), rtol=1e-11, atol=1e-12)
 
 
+def test_entry_trees_change_nothing_on_the_full_heightfield(engine):
+    """The 98,306-triangle heightfield, where a cell's entry is a restricted
+    copy of the BVH (chains of one-child nodes skipped, chain-ending leaves
+    behind gate nodes): traces of 64 photons and the tallies of 20,000 equal
+    those of walks from the root, event for event."""
+    n = 20_000
+    spec = scenes.heightfield_scene(n)
+    res = []
+    for opts in (dict(distance_grid=False, entry_grid=False), dict()):
+        with engine.Scene(spec, **opts) as sc:
+            traces, counts = sc.trace(8, 0, 64, max_events=20000)
+            assert counts.max() < 20000
+            sc.run(9, 0, n)
+            res.append((traces, sc.collect(), sc.stats()))
+    (ta, a, sa), (tb, b, sb) = res
+    assert sb["entryWalks"] > 0 and sb["culledWalks"] > 0
+    for p in range(64):
+        assert len(ta[p]) == len(tb[p]), p
+        for k in ta[p].dtype.names:
+            assert np.array_equal(ta[p][k], tb[p][k]), (p, k)
+    for k in ("scatterEvents", "boundaryEvents", "rouletteEvents"):
+        assert sa[k] == sb[k], k
+    assert np.array_equal(a.det_count(), b.det_count())
+    assert np.array_equal(a.bin_count(), b.bin_count())
+    np.testing.assert_allclose(a.det_weight(), b.det_weight(), rtol=1e-11, atol=1e-12)
+
+
 def test_heightfield_statistics_at_1e8_photons(engine):
     """The 3-D surface configuration at its full 1e8 photons on the GPU (seed
     1, twenty launches of 5e6) against the double oracle (seed 2, forty

@@ -404,7 +404,8 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
         size_t target = std::min<size_t>(std::max<size_t>((size_t)nt * 128, (size_t)1 << 18), (size_t)16 << 20);
         xmc::build_distance_grid(in->surface, in->numberOfSurfaces, lo, hi, target, dg);
         xmc::EntryGrid eg;
-        if (want_entry) xmc::build_entry_grid(gb, dg.lo, dg.h, dg.dims, eg);
+        if (want_entry)
+            xmc::build_entry_grid(gb, dg.lo, dg.h, dg.dims, want_dist ? dg.q.data() : nullptr, dg.step_q, eg);
         std::vector<uint32_t> cells(dg.q.size());
         for (size_t c = 0; c < cells.size(); c++) {
             uint32_t q = want_dist ? dg.q[c] : 0u;
@@ -412,6 +413,15 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
             cells[c] = q | (e << 8);
         }
         if (eg.ref.size() >= xmc::kEntryRoot - 1) return fail(XMC_ERR_LIMIT, "too many BVH entry nodes");
+        if (gb.nodes.size() + eg.extra.size() > (size_t)INT32_MAX) return fail(XMC_ERR_LIMIT, "too many BVH nodes");
+        if (!eg.extra.empty()) {
+            // the entries' restricted trees follow the BVH's own nodes in one array
+            std::vector<xmc::GpuNode> all(gb.nodes);
+            all.insert(all.end(), eg.extra.begin(), eg.extra.end());
+            HIP_TRY(sc->nodes.alloc(all.size()));
+            HIP_TRY(hipMemcpy(sc->nodes.p, all.data(), all.size() * sizeof(xmc::GpuNode), hipMemcpyHostToDevice));
+            P.nodes = sc->nodes.p;
+        }
         HIP_TRY(sc->dist.alloc(cells.size()));
         HIP_TRY(hipMemcpy(sc->dist.p, cells.data(), cells.size() * 4, hipMemcpyHostToDevice));
         HIP_TRY(sc->entry_ref.alloc(std::max<size_t>(eg.ref.size(), 1)));
