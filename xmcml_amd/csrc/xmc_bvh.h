@@ -30,6 +30,12 @@ bool build_reference_bvh(const xmc_surface* surfaces, int n_surfaces, RefBvh& ou
 // GPU node: the boxes of both children (float), then the child references.
 // A reference r >= 0 is an internal node; r < 0 is a leaf whose triangles
 // start at ~r in the triangle array and run until the one flagged last.
+// A child reference in an entry's restricted tree (xmc_entry.h) for a child
+// none of whose leaves is in reach: the walk neither visits nor pushes it (a
+// walk from the root would visit only nodes inside it, push and pop only its
+// own entries, and change neither the hit nor the stack below).
+constexpr int kDeadRef = 0x7FFFFFFF;
+
 struct alignas(16) GpuNode {
     float c0lo[3];
     float c0hi[3];

@@ -125,6 +125,7 @@ struct xmc_scene {
     DevBuf<int2> spill;
     DevBuf<int> perm_dev;
     DevBuf<uint32_t> dist;  // walk-skipping grid cells
+    DevBuf<uint32_t> dist2;  // second-level entries (steps of up to 3 cells)
     DevBuf<int> entry_ref;
     DevBuf<float> entry_box;
     // voxel-plane crossings: scattering grid, detector trajectory counts and
@@ -413,6 +414,12 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
             cells[c] = q | (e << 8);
         }
         if (eg.ref.size() >= xmc::kEntryRoot - 1) return fail(XMC_ERR_LIMIT, "too many BVH entry nodes");
+        if (want_entry) {
+            std::vector<uint32_t> cells2(eg.cell2.size());
+            for (size_t c = 0; c < cells2.size(); c++) cells2[c] = (uint32_t)(eg.cell2[c] + 1);
+            HIP_TRY(sc->dist2.alloc(cells2.size()));
+            HIP_TRY(hipMemcpy(sc->dist2.p, cells2.data(), cells2.size() * 4, hipMemcpyHostToDevice));
+        }
         if (gb.nodes.size() + eg.extra.size() > (size_t)INT32_MAX) return fail(XMC_ERR_LIMIT, "too many BVH nodes");
         if (!eg.extra.empty()) {
             // the entries' restricted trees follow the BVH's own nodes in one array
@@ -441,6 +448,8 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
         P.dist.margin = want_entry ? eg.margin : 0.0f;
         P.dist.entry_ref = sc->entry_ref.p;
         P.dist.entry_box = sc->entry_box.p;
+        P.dist.cells2 = want_entry ? sc->dist2.p : nullptr;
+        P.dist.margin2 = want_entry ? eg.margin2 : 0.0f;
     }
 
     // layer boundary table: FindIntersectionLayer (mcml_kernel.cpp:566-580)

@@ -87,7 +87,8 @@ struct Trace {
 // bits 8-31 hold the cell's entry for walks that move at most `margin`
 // along every axis: 0 = no triangle within reach, kEntryRoot = start at the
 // root, else 1 + an index into entry_ref / entry_box (node reference and its
-// box, 6 floats).
+// box, 6 floats).  cells2 holds, per cell, the entry (same encoding, all 32
+// bits) for walks that move more than `margin` but at most `margin2`.
 constexpr uint32_t kEntryRoot = 0xFFFFFFu;
 
 struct DistGrid {
@@ -100,6 +101,8 @@ struct DistGrid {
     int dims[3];
     const int* entry_ref;
     const float* entry_box;
+    const uint32_t* cells2;  // null without entries
+    float margin2;
 };
 
 // A box of absorption voxels (around the beam entry, where most weight is

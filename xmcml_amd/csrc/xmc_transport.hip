@@ -311,14 +311,14 @@ __device__ __forceinline__ bool walk_visit(const GpuNode* nodes, const GpuTri* t
                 if (tn1 <= tn2) {
                     if (tn1 <= step) {
                         cur = left;
-                        if (tn2 <= step) st.push(right, tn2);
+                        if (tn2 <= step && right != kDeadRef) st.push(right, tn2);
                     } else {
                         use_stack = true;
                     }
                 } else {
                     if (tn2 <= step) {
                         cur = right;
-                        if (tn1 <= step) st.push(left, tn1);
+                        if (tn1 <= step && left != kDeadRef) st.push(left, tn1);
                     } else {
                         use_stack = true;
                     }
@@ -330,6 +330,8 @@ __device__ __forceinline__ bool walk_visit(const GpuNode* nodes, const GpuTri* t
                 if (f2 && tn2 <= step) cur = right;
                 else use_stack = true;
             }
+            // a child with nothing in reach (entry trees only): go on as its walk would end
+            if (cur == kDeadRef) use_stack = true;
         } else {
             if (f1 && f2) {
                 if (tn1 <= tn2) {
@@ -697,7 +699,8 @@ __device__ __forceinline__ int time_bin(const Detectors& D, double t) {
 // under).  Skipping needs that to exceed the step by a margin that covers
 // float rounding of the lookup and of the triangle test, or a step that moves
 // at most the grid's margin along every axis from a cell with no triangle
-// within reach; a short step from a cell with an entry starts there.
+// within reach; a short step from a cell with an entry starts there.  Steps
+// that move at most margin2 use the cell's second-level entry likewise.
 __device__ __forceinline__ int walk_start(const DistGrid& G, V3 p, V3 d, float step) {
     float fx = (p.x - G.lo[0]) * G.inv_h;
     float fy = (p.y - G.lo[1]) * G.inv_h;
@@ -715,7 +718,11 @@ __device__ __forceinline__ int walk_start(const DistGrid& G, V3 p, V3 d, float s
     if (room > 0.0f && step * step * dot(d, d) < room * room) return -1;
     float reach = step * fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z)));
     uint32_t e = word >> 8;
-    if (!(reach <= G.margin) || e == kEntryRoot) return -2;
+    if (!(reach <= G.margin)) {
+        if (G.cells2 == nullptr || !(reach <= G.margin2)) return -2;
+        e = G.cells2[((size_t)iz * G.dims[1] + iy) * G.dims[0] + ix];
+    }
+    if (e == kEntryRoot) return -2;
     return (int)e - 1;
 }
 
