@@ -232,6 +232,8 @@ template <> struct Prec<double> {
 //  - the scattering angle comes from hg_float (1 - cos and 1 + cos without
 //    cancellation) instead of cos first, which loses sin(theta) for the
 //    strongly forward scatters of g ~ 0.9;
+//  - dot and cross products use fused multiply-adds (dot<float>, cross<float>)
+//    and the slab distance (b - o) / d is fma(b, 1/d, -(o * (1/d)));
 //  - 1 - x^2 is evaluated as (1 - x)(1 + x); in the spin, sin(theta_u) is
 //    sqrt(ux^2 + uy^2) and the rotation is used when that square is above
 //    2e-12 (the reference's |uz| <= 1 - 1e-12, which rounds to 1.0f in
@@ -275,6 +277,15 @@ template <class R> inline R length(V3<R> a) { return std::sqrt(dot(a, a)); }
 template <class R> inline V3<R> normalize(V3<R> a) { return divv(a, length(a)); }
 template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// float rule: dot and cross products with fused multiply-adds (the GPU kernel's
+// dot and cross), dot = fma(a.z, b.z, fma(a.y, b.y, a.x b.x)),
+// cross_i = fma(a_j, b_k, -(a_k b_j)); double mode keeps the reference's sums
+template <> inline float dot<float>(V3<float> a, V3<float> b) {
+    return std::fmaf(a.z, b.z, std::fmaf(a.y, b.y, a.x * b.x));
+}
+template <> inline V3<float> cross<float>(V3<float> a, V3<float> b) {
+    return {std::fmaf(a.y, b.z, -(a.z * b.y)), std::fmaf(a.z, b.x, -(a.x * b.z)), std::fmaf(a.x, b.y, -(a.y * b.x))};
 }
 template <class R> inline V3<R> plane_normal(V3<R> a, V3<R> b, V3<R> c) {
     return cross(sub(a, b), sub(a, c));
@@ -354,8 +365,16 @@ bool box_hit(const Box<R>& b, const V3<R>& o, const V3<R>& d, R& tnear, R& tfar)
     for (int i = 0; i < 3; i++) {
         if (std::fabs(d[i]) > Prec<R>::kEps) {
             R t = (R)1.0 / d[i];
-            R t1 = (b.lo[i] - o[i]) * t;
-            R t2 = (b.hi[i] - o[i]) * t;
+            R t1, t2;
+            if constexpr (sizeof(R) == 4) {
+                // float rule: (b - o) / d as fma(b, 1/d, -(o * (1/d)))
+                const R oi = o[i] * t;
+                t1 = std::fmaf(b.lo[i], t, -oi);
+                t2 = std::fmaf(b.hi[i], t, -oi);
+            } else {
+                t1 = (b.lo[i] - o[i]) * t;
+                t2 = (b.hi[i] - o[i]) * t;
+            }
             if (t2 < t1) { R tt = t1; t1 = t2; t2 = tt; }
             if (t1 > tmin) tmin = t1;
             if (t2 < tmax) tmax = t2;

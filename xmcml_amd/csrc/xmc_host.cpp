@@ -363,15 +363,15 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     HIP_TRY(sc->perm_dev.alloc(nt));
     HIP_TRY(hipMemcpy(sc->perm_dev.p, sc->perm.data(), nt * sizeof(int), hipMemcpyHostToDevice));
     {
-        // unit normals with the kernel's float operations (cross, sqrt, three divisions; no
-        // FMA contraction): the same bits the kernel would compute at each boundary hit
+        // unit normals with the kernel's float operations (cross and dot with fused
+        // multiply-adds, sqrt, three divisions): the bits the kernel would compute at each hit
         std::vector<float> nrm(4 * nt, 0.0f);
         for (size_t k = 0; k < nt; k++) {
             const xmc::GpuTri& t = gb.tris[k];
-            float cx = t.e1[1] * t.e2[2] - t.e1[2] * t.e2[1];
-            float cy = t.e1[2] * t.e2[0] - t.e1[0] * t.e2[2];
-            float cz = t.e1[0] * t.e2[1] - t.e1[1] * t.e2[0];
-            float l = std::sqrt(cx * cx + cy * cy + cz * cz);
+            float cx = std::fmaf(t.e1[1], t.e2[2], -(t.e1[2] * t.e2[1]));
+            float cy = std::fmaf(t.e1[2], t.e2[0], -(t.e1[0] * t.e2[2]));
+            float cz = std::fmaf(t.e1[0], t.e2[1], -(t.e1[1] * t.e2[0]));
+            float l = std::sqrt(std::fmaf(cz, cz, std::fmaf(cy, cy, cx * cx)));
             nrm[4 * k] = cx / l;
             nrm[4 * k + 1] = cy / l;
             nrm[4 * k + 2] = cz / l;

@@ -115,9 +115,14 @@ struct V3 {
     float x, y, z;
 };
 __device__ __forceinline__ V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// float rule: dot and cross products are evaluated with fused multiply-adds,
+// dot = fma(a.z, b.z, fma(a.y, b.y, a.x b.x)), cross_i = fma(a_j, b_k, -(a_k b_j))
+__device__ __forceinline__ float dot(V3 a, V3 b) {
+    return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, a.x * b.x));
+}
 __device__ __forceinline__ V3 cross(V3 a, V3 b) {
-    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+    return {__builtin_fmaf(a.y, b.z, -(a.z * b.y)), __builtin_fmaf(a.z, b.x, -(a.x * b.z)),
+            __builtin_fmaf(a.x, b.y, -(a.y * b.x))};
 }
 __device__ __forceinline__ V3 normalize(V3 a) {
     float l = __builtin_sqrtf(dot(a, a));
@@ -134,7 +139,12 @@ __device__ __forceinline__ float offset_scale(V3 p) {
 struct Ray {
     V3 o, d, inv;
     bool ax, ay, az;  // |d| > EPSILON per axis
+    V3 oi;            // o * inv per axis, for the slab distances fma(bound, inv, -o inv)
 };
+
+__device__ __forceinline__ void set_oi(Ray& r) {
+    r.oi = {r.o.x * r.inv.x, r.o.y * r.inv.y, r.o.z * r.inv.z};
+}
 
 __device__ __forceinline__ Ray make_ray(V3 o, V3 d) {
     Ray r;
@@ -146,32 +156,34 @@ __device__ __forceinline__ Ray make_ray(V3 o, V3 d) {
     r.inv.x = r.ax ? 1.0f / d.x : 0.0f;
     r.inv.y = r.ay ? 1.0f / d.y : 0.0f;
     r.inv.z = r.az ? 1.0f / d.z : 0.0f;
+    set_oi(r);
     return r;
 }
 
 // IntersectAABB (mcml_intersection.cpp:11-42).  Slab tests of the three axes
 // without the early exit: tmin only grows and tmax only shrinks, so the
-// result and t_near are the same.
+// result and t_near are the same.  Float rule: the slab distance (b - o) / d
+// is fma(b, 1/d, -(o * (1/d))).
 __device__ __forceinline__ bool box_hit(const Ray& r, float lx, float ly, float lz, float hx, float hy, float hz,
                                         float& tnear) {
     float tmin = -kInf, tmax = kInf;
     bool ok = true;
     if (r.ax) {
-        float t1 = (lx - r.o.x) * r.inv.x, t2 = (hx - r.o.x) * r.inv.x;
+        float t1 = __builtin_fmaf(lx, r.inv.x, -r.oi.x), t2 = __builtin_fmaf(hx, r.inv.x, -r.oi.x);
         tmin = fmaxf(tmin, fminf(t1, t2));
         tmax = fminf(tmax, fmaxf(t1, t2));
     } else {
         ok = ok && !((lx - r.o.x > 0.0f) || (hx - r.o.x < 0.0f));
     }
     if (r.ay) {
-        float t1 = (ly - r.o.y) * r.inv.y, t2 = (hy - r.o.y) * r.inv.y;
+        float t1 = __builtin_fmaf(ly, r.inv.y, -r.oi.y), t2 = __builtin_fmaf(hy, r.inv.y, -r.oi.y);
         tmin = fmaxf(tmin, fminf(t1, t2));
         tmax = fminf(tmax, fmaxf(t1, t2));
     } else {
         ok = ok && !((ly - r.o.y > 0.0f) || (hy - r.o.y < 0.0f));
     }
     if (r.az) {
-        float t1 = (lz - r.o.z) * r.inv.z, t2 = (hz - r.o.z) * r.inv.z;
+        float t1 = __builtin_fmaf(lz, r.inv.z, -r.oi.z), t2 = __builtin_fmaf(hz, r.inv.z, -r.oi.z);
         tmin = fmaxf(tmin, fminf(t1, t2));
         tmax = fminf(tmax, fmaxf(t1, t2));
     } else {
@@ -991,7 +1003,8 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             for (int k = 0; k < kVisitsPerRound; ++k) {
                 if (phase == kWalk) {
                     const Params& C = CFG();
-                    Ray ray{pos, dir, rinv, (raxes & 1u) != 0, (raxes & 2u) != 0, (raxes & 4u) != 0};
+                    Ray ray{pos, dir, rinv, (raxes & 1u) != 0, (raxes & 2u) != 0, (raxes & 4u) != 0, {}};
+                    set_oi(ray);
                     if (!walk_visit(C.nodes, C.tris, ray, step, true, st, h, cur, best)) phase = kResolve;
                 }
                 if (__ballot(phase == kWalk) == 0) break;
