@@ -1,6 +1,7 @@
 // Conservative distance grid for skipping BVH walks that cannot hit.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
