@@ -250,7 +250,8 @@ def main(argv=None) -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--photons", type=int, default=None,
-                    help="photons per step, one launch (default: 1e8, the configuration's count; 1e7 for head)")
+                    help="photons per step, one launch (default: 1e8 -- the heightfield configuration's count, "
+                         "and a tenth of the head configuration's 1e9)")
     ap.add_argument("--workload", choices=["heightfield", "head", "slab", "batch"], default="heightfield",
                     help="heightfield: the 3-D surface configuration (the metric's workload); head: five nested "
                          "noisy ellipsoids of 327,680 triangles each; slab: the planar 3-layer configuration with "
@@ -276,7 +277,9 @@ def main(argv=None) -> int:
     from xmcml_amd import engine, scenes
 
     if args.photons is None:
-        args.photons = 10_000_000 if args.workload == "head" else 100_000_000
+        # a launch ends with its slowest last photons: a tail that is a smaller share of a
+        # longer launch (1e8 photons = 381 per lane of the persistent grid)
+        args.photons = 100_000_000
     if args.workload == "batch":
         return bench_batch(args)
     make = {"heightfield": scenes.heightfield_scene, "head": scenes.head_scene, "slab": scenes.slab_scene}[args.workload]
