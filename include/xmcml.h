@@ -254,7 +254,9 @@ typedef struct xmc_options {
     int trajectoryCapacity; /* voxels kept per photon, 0 = 20480 (MAX_TRAJECTORY_SIZE, mcml_kernel_types.h:8);
                              * a detected photon with more crossings counts only the first ones and is
                              * reported in xmc_stats.trajectoryOverflows */
-    int noEntryGrid;     /* 1 = short step-limited walks start at the BVH root, not at a cell's entry node */
+    int noEntryGrid;     /* 1 = step-limited walks start at the BVH root, not on a cell's entry tree (the entry
+                          * trees of a 1.6M-triangle mesh take ~25 s of host time and ~3.6 GB of device
+                          * memory to build; results are the same either way) */
 } xmc_options;
 
 typedef struct xmc_scene xmc_scene;
