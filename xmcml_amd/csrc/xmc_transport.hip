@@ -31,10 +31,10 @@ constexpr float kInf = __builtin_inff();
 #define XMC_PARK_HITS 1
 #endif
 #ifndef XMC_PARK_MIN
-#define XMC_PARK_MIN 12
+#define XMC_PARK_MIN 16
 #endif
 #ifndef XMC_PARK_WAIT
-#define XMC_PARK_WAIT 8
+#define XMC_PARK_WAIT 16
 #endif
 #ifndef XMC_START_MIN
 #define XMC_START_MIN 4
@@ -43,7 +43,7 @@ constexpr float kInf = __builtin_inff();
 #define XMC_START_WAIT 4
 #endif
 #ifndef XMC_VISITS_PER_ROUND
-#define XMC_VISITS_PER_ROUND 2
+#define XMC_VISITS_PER_ROUND 1
 #endif
 constexpr float kEpsF = 1.0e-6f;                 // EPSILON
 constexpr float kCosZeroF = 1.0f - 1.0e-6f;      // float "cosine of zero"
@@ -803,9 +803,10 @@ constexpr int kVisitsPerRound = XMC_VISITS_PER_ROUND;
 // a round) wait, up to kParkWait rounds, until kParkMin lanes of the wave
 // have one: a round pays for every path any lane takes, and about half of
 // all rounds would otherwise carry one.  Each photon's own operations are
-// unchanged.  Measured (heightfield / slab photons/s): none 2.73e7 / 4.46e7;
-// 2 lanes or 1 round 2.78e7 / 4.45e7; 6 or 5 2.90e7 / 4.58e7; 12 or 8
-// 2.92e7 / 4.68e7; 16 or 16 2.91e7 / 4.79e7.
+// unchanged.  Measured at four waves per SIMD (heightfield / slab photons/s):
+// none 2.73e7 / 4.46e7; 2 lanes or 1 round 2.78e7 / 4.45e7; 6 or 5 2.90e7 /
+// 4.58e7; 12 or 8 2.92e7 / 4.68e7; 16 or 16 2.91e7 / 4.79e7.  At five waves
+// with one visit per round, 16 or 16 is 0.7 % ahead of 12 or 8.
 constexpr bool kParkHits = XMC_PARK_HITS != 0;
 constexpr int kParkMin = XMC_PARK_MIN;    // lanes that make a boundary round worth running
 constexpr uint32_t kParkWait = XMC_PARK_WAIT;  // rounds a boundary interaction may wait
