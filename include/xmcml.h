@@ -255,7 +255,7 @@ typedef struct xmc_options {
                              * a detected photon with more crossings counts only the first ones and is
                              * reported in xmc_stats.trajectoryOverflows */
     int noEntryGrid;     /* 1 = step-limited walks start at the BVH root, not on a cell's entry tree (the entry
-                          * trees of a 1.6M-triangle mesh take ~25 s of host time and ~3.6 GB of device
+                          * trees of a 1.6M-triangle mesh take ~30 s of host time and ~5 GB of device
                           * memory to build; results are the same either way) */
 } xmc_options;
 
