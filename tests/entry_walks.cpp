@@ -230,7 +230,7 @@ int main(int argc, char** argv) {
         lo[a] = gb.root_lo[a] - pad;
         hi[a] = gb.root_hi[a] + pad;
     }
-    size_t target = std::min<size_t>(std::max<size_t>(nt * 128, (size_t)1 << 18), (size_t)16 << 20);
+    size_t target = std::min<size_t>(std::max<size_t>(nt * 256, (size_t)1 << 18), (size_t)32 << 20);
     build_distance_grid(S.data(), (int)S.size(), lo, hi, target, dg);
     EntryGrid eg;
     build_entry_grid(gb, dg.lo, dg.h, dg.dims, dg.q.data(), dg.step_q, eg);
