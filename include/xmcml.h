@@ -30,7 +30,9 @@
  *   - Random streams are per photon, not per thread: photon p of a run with
  *     seed s draws the MCG59 sequence that starts at (2s+1) * a^(p * 2^24)
  *     mod 2^59 (a = 13^13).  The result therefore does not depend on how
- *     photons are scheduled.  xmc_rng carries (seed, next photon) where
+ *     photons are scheduled.  A seed has 2^33 such blocks (the generator's
+ *     period is 2^57), so photon indices of one seed stay below 2^33
+ *     (8.6e9); runs that would pass it fail with XMC_ERR_LIMIT.  xmc_rng carries (seed, next photon) where
  *     LaunchOMP carries MCG59[numThreads]; both are advanced in place.
  *   - Termination can follow xmcml (die when weight < minWeight,
  *     mcml_kernel.cpp:121-124) or MCML Russian roulette
@@ -170,7 +172,7 @@ typedef struct xmc_time_info {
 /* DetectorInfo, mcml_kernel_types.h:36-45. */
 typedef struct xmc_detector_info {
     uint64_t numberOfPhotons;
-    uint64_t* trajectory; /* may be NULL; not written by the GPU path */
+    uint64_t* trajectory; /* trajectorySize counts per voxel, or NULL (map not tallied) */
     int trajectorySize;
     xmc_time_info* timeScale;
     int timeScaleSize;
@@ -184,7 +186,7 @@ typedef struct xmc_output {
     uint64_t numberOfPhotons;
     double specularReflectance;
     double* absorption; /* gridSize doubles; may be NULL (tally skipped) */
-    double* scattering; /* gridSize doubles; not written by the GPU path */
+    double* scattering; /* gridSize doubles, or NULL (grid not tallied) */
     int gridSize;
     xmc_detector_info* detectorInfo;
     int numberOfDetectors;
@@ -358,7 +360,13 @@ int xmc_weight_table_compute(double anisotropy, double attractiveFactor, int int
 int xmc_launch(const xmc_input* input, xmc_output* output, xmc_rng* rng,
                const xmc_options* options);
 
-/* InitOutput / FreeOutput equivalents (malloc-based). */
+/* InitOutput / FreeOutput equivalents (malloc-based).  Like InitOutput
+ * (launcher_omp.cpp:10-58), xmc_init_output allocates every tally: the
+ * absorption and scattering grids and each detector's trajectory map, so
+ * that xmc_launch on it produces all of them, as LaunchOMP does.  The
+ * scattering grid and trajectory maps need the path-tally kernel variant
+ * (about 2.5x slower on the 3-D surface configuration); a caller that does
+ * not want them frees and NULLs those arrays first. */
 int xmc_init_output(const xmc_input* input, xmc_output* output);
 void xmc_free_output(xmc_output* output);
 

@@ -1051,6 +1051,27 @@ template <class R> V3<R> new_direction(V3<R> d, R g, Mcg59& rng) {  // :442-473
     return r;
 }
 
+// MCML roulette (stable/plane_parallel/1.0/McmlGpu/kernel/kernel.cu:214-222),
+// applied only to a live photon below the threshold (kernel.cu:449-452): a
+// zero weight dies without a draw; otherwise one uniform u, survive when
+// u < chance with the weight divided by chance, else die (the weight is kept
+// so that the caller can book it as lost).  Expected weight afterwards equals
+// the weight before.
+enum { kRouletteSkip = 0, kRouletteZero = 1, kRouletteLive = 2, kRouletteDie = 3 };
+template <class R> int roulette_step(R& weight, bool& dead, R min_weight, R chance, Mcg59& rng) {
+    if (!(weight < min_weight) || dead) return kRouletteSkip;
+    if (weight == (R)0.0) {
+        dead = true;
+        return kRouletteZero;
+    }
+    if (Prec<R>::uniform(rng.next()) < chance) {
+        weight /= chance;
+        return kRouletteLive;
+    }
+    dead = true;
+    return kRouletteDie;
+}
+
 template <class R> R step_size(R mua, R mus, Mcg59& rng) {  // :618-631
     R u;
     do {
@@ -1450,21 +1471,17 @@ template <class R> struct Walker {
             }
         }
         if (in->terminationMode == XMC_TERMINATE_ROULETTE) {
-            if (p.weight < sc.min_weight && !p.dead) {
-                tl.roulette++;
-                if (p.weight == (R)0.0) {
-                    p.dead = true;
-                    record(XMC_EV_ROULETTE_DIE, p, 0, rng);
-                } else if (Prec<R>::uniform(rng.next()) < sc.chance) {
-                    R before = p.weight;
-                    p.weight /= sc.chance;
-                    tl.w_rgain += (double)p.weight - (double)before;
-                    record(XMC_EV_ROULETTE_LIVE, p, 0, rng);
-                } else {
-                    p.dead = true;
-                    tl.w_rlost += (double)p.weight;
-                    record(XMC_EV_ROULETTE_DIE, p, 0, rng);
-                }
+            const R before = p.weight;
+            const int r = roulette_step(p.weight, p.dead, sc.min_weight, sc.chance, rng);
+            if (r != kRouletteSkip) tl.roulette++;
+            if (r == kRouletteLive) {
+                tl.w_rgain += (double)p.weight - (double)before;
+                record(XMC_EV_ROULETTE_LIVE, p, 0, rng);
+            } else if (r == kRouletteDie) {
+                tl.w_rlost += (double)p.weight;
+                record(XMC_EV_ROULETTE_DIE, p, 0, rng);
+            } else if (r == kRouletteZero) {
+                record(XMC_EV_ROULETTE_DIE, p, 0, rng);
             }
         } else if (p.weight < sc.min_weight) {
             if (!p.dead) {
@@ -1673,6 +1690,33 @@ int xmco_shadow_trace(const xmc_input* input, uint64_t seed, uint64_t photon, in
     g_uniform24 = precision == 65;
     if (precision == 32) return shadow_impl<float>(input, seed, photon, trace, n, out, counts);
     return shadow_impl<double>(input, seed, photon, trace, n, out, counts);
+}
+
+// The walk's roulette step on n photons: photon i has weight weights[i] and is
+// dead when dead[i]; it draws from stream i of `seed` (precision 32: float
+// weights and 24-bit uniforms; 64: double and 59-bit).  Returns the weight
+// after the step in out_weight (0 for a photon that died), the outcome
+// (kRoulette*) and the uniforms drawn.
+int xmco_roulette_many(int precision, const double* weights, const uint8_t* dead, double min_weight, double chance,
+                       uint64_t seed, int64_t n, double* out_weight, int* outcome, int* draws) {
+    g_uniform24 = false;
+    for (int64_t i = 0; i < n; ++i) {
+        Mcg59 g = stream_for(seed, (uint64_t)i);
+        bool d = dead[i] != 0;
+        int r;
+        if (precision == 32) {
+            float w = (float)weights[i];
+            r = roulette_step<float>(w, d, (float)min_weight, (float)chance, g);
+            out_weight[i] = d ? 0.0 : (double)w;
+        } else {
+            double w = weights[i];
+            r = roulette_step<double>(w, d, min_weight, chance, g);
+            out_weight[i] = d ? 0.0 : w;
+        }
+        outcome[i] = r;
+        draws[i] = (int)g.draws;
+    }
+    return XMC_OK;
 }
 
 int xmco_bvh(const xmc_input* input, int* nn, int* root, int* nt, double* boxes, int* links,

@@ -111,6 +111,11 @@ void xmco_sincos2pi_many(const float* u, float* s, float* c, int64_t n);
 void xmco_expf_many(const float* x, float* y, int64_t n);
 /* ComputeWeightIntegral (mcml_integration.cpp:7-82), double, restated */
 double xmco_weight_integral(double dot, double anisotropy, double eps, unsigned n);
+/* The walk's MCML roulette step on n photons (stream i of seed each); out_weight
+ * is the weight after the step (0 when dead), outcome 0 skipped / 1 zero weight
+ * died / 2 survived / 3 died, draws the uniforms used. */
+int xmco_roulette_many(int precision, const double* weights, const uint8_t* dead, double min_weight,
+                       double chance, uint64_t seed, int64_t n, double* out_weight, int* outcome, int* draws);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parents[1]
 ORACLE_SO = ROOT / "oracle" / "liboracle.so"
 REF_SO = ROOT / "oracle" / "_ref" / "libxmcml_ref.so"
+REF_TAPE_SO = ROOT / "oracle" / "_ref" / "libxmcml_ref_tape.so"
 
 P = C.c_void_p
 D = C.c_double
@@ -43,6 +44,11 @@ def _sig(lib, name, res, args):
 
 _oracle = None
 _ref = None
+_ref_tape = None
+
+# one nearest-surface query of the reference's photon loop (oracle/ref_tape.cpp)
+QUERY_DTYPE = np.dtype([("origin", "<f8", 3), ("direction", "<f8", 3), ("step", "<f8"), ("distance", "<f8"),
+                        ("found", "<i4"), ("surface", "<i4"), ("draws", "<u4"), ("pad", "<i4")])
 
 
 def oracle():
@@ -60,6 +66,7 @@ def oracle():
         _sig(lib, "xmco_mcg59_init", U64, [U64, C.c_uint32, C.c_uint32, PU64])
         _sig(lib, "xmco_weight_integral", D, [D, D, D, C.c_uint])
         _sig(lib, "xmco_expf_many", None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64])
+        _sig(lib, "xmco_roulette_many", I, [I, P, P, D, D, U64, C.c_int64, P, P, P])
         _sig(lib, "xmco_mcg59_next", D, [PU64, U64])
         _sig(lib, "xmco_specular", D, [P])
         _sig(lib, "xmco_cosine_theta", D, [D, PU64, U64])
@@ -111,6 +118,48 @@ def ref():
         _sig(lib, "ref_weight_integral", D, [D, D, D, C.c_uint])
         _ref = lib
     return _ref
+
+
+def ref_tape_available() -> bool:
+    return REF_TAPE_SO.exists()
+
+
+def _bind_ref(lib):
+    _sig(lib, "xref_run", I, [P, P, U64, U64, U64, I])
+    _sig(lib, "xref_launch_omp", D, [P, P, U64, I])
+    _sig(lib, "xref_trace", I, [P, U64, U64, P, I])
+    return lib
+
+
+def ref_tape():
+    """The reference sources linked so that they draw the GPU engine's 24-bit
+    uniforms (the top 24 bits of the same MCG59 states) and log every BVH
+    query their photon loop makes."""
+    global _ref_tape
+    if _ref_tape is None:
+        if not REF_TAPE_SO.exists():
+            raise RuntimeError(f"{REF_TAPE_SO} is not built")
+        lib = _bind_ref(C.CDLL(str(REF_TAPE_SO)))
+        _sig(lib, "xref_tape_log_begin", None, [P, I])
+        _sig(lib, "xref_tape_log_end", I, [C.POINTER(C.c_uint32)])
+        _ref_tape = lib
+    return _ref_tape
+
+
+def tape_queries(fn, capacity=200000):
+    """Runs fn() (a call into the tape library) with the query log open and
+    returns the logged queries and the uniforms drawn."""
+    lib = ref_tape()
+    buf = np.zeros(capacity, dtype=QUERY_DTYPE)
+    lib.xref_tape_log_begin(buf.ctypes.data, capacity)
+    try:
+        fn()
+    finally:
+        draws = C.c_uint32()
+        n = lib.xref_tape_log_end(C.byref(draws))
+    if n > capacity:
+        raise RuntimeError(f"more than {capacity} queries ({n})")
+    return buf[:n], draws.value
 
 
 def dptr(a: np.ndarray):
@@ -182,3 +231,29 @@ def shadow_compare(inp, seed, photon, trace, precision=65, detail=False):
             break
         i += k
     return (ok, worst, where) if detail else (ok, worst)
+
+
+BOUNDARY_EVENTS = (2, 3, 4)  # XMC_EV_REFLECT, XMC_EV_TRANSMIT, XMC_EV_ESCAPE
+
+
+def compare_walks(a, b):
+    """Compares two traces of the same photon on the same uniforms (a: the
+    reference's double walk, b: the float walk).  Returns the common prefix of
+    integer decisions (event type, layer, uniforms drawn), whether it is the
+    whole life of both, and the worst relative errors over that prefix: of the
+    weight; of position (relative to max(|pos|, 1 mm)) and direction before the
+    photon's second boundary interaction; and of position and direction over
+    the whole prefix."""
+    m = min(len(a), len(b))
+    same = (a["type"][:m] == b["type"][:m]) & (a["layer"][:m] == b["layer"][:m]) & (a["draws"][:m] == b["draws"][:m])
+    k = m if same.all() else int(np.argmin(same))
+    res = dict(prefix=k, full=(k == len(a) == len(b)), weight=0.0, early=0.0, whole=0.0)
+    if k == 0:
+        return res
+    sc = np.maximum(np.abs(a["pos"][:k]).max(axis=1), 1.0)
+    e = np.maximum(np.abs(a["pos"][:k] - b["pos"][:k]).max(axis=1) / sc, np.abs(a["dir"][:k] - b["dir"][:k]).max(axis=1))
+    dw = np.abs(a["weight"][:k] - b["weight"][:k]) / np.maximum(np.abs(a["weight"][:k]), 1e-30)
+    bi = np.nonzero(np.isin(a["type"][:k], BOUNDARY_EVENTS))[0]
+    cut = int(bi[1]) if len(bi) > 1 else k
+    res.update(weight=float(dw.max()), early=float(e[:cut].max()) if cut else 0.0, whole=float(e.max()))
+    return res

@@ -76,9 +76,13 @@ def test_init_output_matches_initoutput_layout():
     assert engine.lib().xmc_init_output(inp.ptr, C.byref(out)) == 0
     ref = OutputHandle(spec)
     assert out.gridSize == spec.grid_size and out.numberOfDetectors == 20
+    assert out.absorption and out.scattering
     for d in range(out.numberOfDetectors):
         di = out.detectorInfo[d]
         assert di.timeScaleSize == 7
+        # every detector's voxel trajectory map, zeroed (launcher_omp.cpp:31-34)
+        assert di.trajectorySize == spec.grid_size and di.trajectory
+        assert not np.ctypeslib.as_array(di.trajectory, shape=(spec.grid_size,)).any()
         for b in range(7):
             assert di.timeScale[b].timeStart == ref.bins[d * 7 + b].timeStart
             assert di.timeScale[b].timeFinish == ref.bins[d * 7 + b].timeFinish

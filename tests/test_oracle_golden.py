@@ -242,3 +242,25 @@ def test_detector_id(case):
     d = _a([0.0, 0.0, -1.0])
     for lib in both():
         assert fn(lib, "detector_id")(inp.ptr, ol.dptr(_a(case["pos"])), ol.dptr(d), visited) == case["expected"]
+
+
+def test_a_seed_has_2_33_disjoint_photon_streams():
+    """Photon p starts at (2s+1) a^(p 2^24) mod 2^59.  MCG59 (a = 13^13 = 5 mod 8)
+    has period 2^57 on odd states, so the jump a^(2^24 2^i) is the identity
+    exactly from i = 33 on: photon p + 2^33 repeats photon p, photons below
+    2^33 never share a block.  The engine caps photon indices at 2^33."""
+    a, mask = 302875106592253, (1 << 59) - 1
+    assert a % 8 == 5
+    j = pow(a, 1 << 24, 1 << 59)
+    table = []
+    for _ in range(40):
+        table.append(j)
+        j = (j * j) & mask
+    assert all(t != 1 for t in table[:33])
+    assert all(t == 1 for t in table[33:])
+    o = ol.oracle()
+    for p in (0, 5, 123456789):
+        assert o.xmco_photon_stream(3, p) != o.xmco_photon_stream(3, p + (1 << 32))
+        assert o.xmco_photon_stream(3, p) == o.xmco_photon_stream(3, p + (1 << 33))
+    src = (ol.ROOT / "xmcml_amd" / "csrc" / "xmc_params.h").read_text()
+    assert "constexpr int kJumpBits = 57 - kStreamLog2;" in src and "constexpr int kStreamLog2 = 24;" in src

@@ -706,7 +706,7 @@ int xmc_scene_destroy(xmc_scene* sc) {
 static int run_impl(xmc_scene* sc, uint64_t seed, uint64_t first, uint64_t n, hipStream_t s, const xmc::Trace* tr) {
     if (n == 0) return XMC_OK;
     if (first + n < first || first + n > (1ull << xmc::kJumpBits))
-        return fail(XMC_ERR_LIMIT, "photon index beyond 2^40");
+        return fail(XMC_ERR_LIMIT, "photon index beyond 2^33 (a seed has 2^33 disjoint streams; use another seed)");
     HIP_TRY(hipSetDevice(sc->device));
     xmc::Params P = sc->base;
     P.first_photon = first;
@@ -1028,10 +1028,10 @@ int xmc_init_output(const xmc_input* in, xmc_output* out) {  // InitOutput, laun
     for (int d = 0; d < nd; d++) {
         xmc_detector_info& di = out->detectorInfo[d];
         di.trajectorySize = G;
-        di.trajectory = nullptr;  // not produced by the GPU path
+        di.trajectory = (uint64_t*)calloc(G, sizeof(uint64_t));  // launcher_omp.cpp:33-34
         di.timeScaleSize = T;
         di.timeScale = (xmc_time_info*)calloc(T, sizeof(xmc_time_info));
-        if (!di.timeScale) return fail(XMC_ERR_ARG, "out of host memory");
+        if (!di.timeScale || !di.trajectory) return fail(XMC_ERR_ARG, "out of host memory");
         for (int j = 0; j < T - 1; j++) {
             di.timeScale[j].timeStart = in->timeStart + j * dt;
             di.timeScale[j].timeFinish = di.timeScale[j].timeStart + dt;
@@ -1145,7 +1145,7 @@ int xmc_batch_create(const xmc_input* configs, int n, const xmc_options* opt, xm
         b->prefix[c + 1] = b->prefix[c] + configs[c].numberOfPhotons;
         if (m->ref.depth > b->depth) b->depth = m->ref.depth;
     }
-    if (b->prefix[n] > (1ull << xmc::kJumpBits)) return fail(XMC_ERR_LIMIT, "batch photon total beyond 2^40");
+    if (b->prefix[n] > (1ull << xmc::kJumpBits)) return fail(XMC_ERR_LIMIT, "batch photon total beyond 2^33 (a seed has 2^33 disjoint streams)");
     HIP_TRY(hipSetDevice(b->device));
     HIP_TRY(b->cfgs.alloc(n));
     HIP_TRY(hipMemcpy(b->cfgs.p, host.data(), n * sizeof(xmc::Params), hipMemcpyHostToDevice));

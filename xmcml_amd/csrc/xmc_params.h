@@ -10,7 +10,10 @@ namespace xmc {
 
 constexpr int kMaxLayers = XMC_MAX_LAYERS;
 constexpr int kStreamLog2 = 24;        // MCG59 draws reserved per photon
-constexpr int kJumpBits = 40;          // photons < 2^40 per seed
+// photons < 2^33 per seed: MCG59 with an odd state has period 2^57
+// (a = 13^13 = 5 mod 8), so 2^57 / 2^24 = 2^33 disjoint blocks; photon
+// p + 2^33 would repeat photon p's numbers
+constexpr int kJumpBits = 57 - kStreamLog2;
 constexpr uint64_t kMcgA = 302875106592253ULL;     // 13^13, mcml_mcg59.cpp:3
 constexpr uint64_t kMcgMask = 576460752303423487ULL;  // 2^59 - 1
 constexpr int kStackLds = 8;           // LDS entries of the per-lane short stack

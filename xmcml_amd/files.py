@@ -284,7 +284,8 @@ def run_files(xml_path, surface_path, output_path, seed: int = 777, portion: int
     if x.useBiasing:
         get_weight_tables(doc, weight_table_path)
     spec = doc.spec()
-    out = OutputHandle(spec, absorption=True, scattering=True)
+    # every tally InitOutput allocates (launcher_omp.cpp:10-58), trajectory maps included
+    out = OutputHandle(spec, absorption=True, scattering=True, trajectory=True)
     rng = Rng(seed, 0)
     total = int(x.numberOfPhotons)
     if resume_from is not None:
