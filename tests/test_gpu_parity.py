@@ -301,14 +301,17 @@ def test_entry_trees_change_nothing_on_the_full_heightfield(engine):
 def test_heightfield_statistics_at_1e8_photons(engine):
     """The 3-D surface configuration at its full 1e8 photons on the GPU (seed
     1, twenty launches of 5e6) against the double oracle (seed 2, forty
-    samples of 40,000 photons: 1.6e6), independent streams.  Ring bins, total
-    reflectance and absorption: every one within three combined standard
-    errors, R and A within 0.5 %.  The 80-odd populated TOF bins (summed over
-    rings) are tested as a family, because with that many bins a few |z| above
-    3 are expected by chance (P(all 80 within 3) = 0.9973^80 = 0.81 for a
-    normal z, less for the t-like tail of 20- and 40-sample variances): at
-    least 97 % within 3, none beyond 4.5 (a family-wise level near 3 sigma's
-    0.27 % for 80 tests), mean z^2 chi-square-like (< 1.5)."""
+    samples of 40,000 photons: 1.6e6), independent streams.  R and A within
+    three combined standard errors and within 0.5 %.  The 20 rings and the
+    80-odd populated TOF bins (summed over rings) are tested as families,
+    because a 3-sigma test per bin raises false alarms in proportion to the
+    bins: P(all 20 rings within 3) = 0.9973^20 = 0.95 for a normal z, less for
+    the t-like tail of 20- and 40-sample variances (the first run at this
+    power had one ring of 20 at 3.26 while the paired test below bounds that
+    ring's float-vs-double bias at -1.3e-4 +- 3.5e-4 relative).  Per family:
+    at least 90 % (rings) / 97 % (TOF) within 3, none beyond 3.5 (rings) /
+    4.5 (TOF) -- a family-wise level near one 3-sigma test's 0.27 % -- and a
+    chi-square-like mean z^2 (< 1.6 / < 1.5)."""
     spec = scenes.heightfield_scene(0)
     inp = InputHandle(spec)
     nb_g, n_gpu, nb_c, n_cpu = 20, 5_000_000, 40, 40_000
@@ -340,7 +343,7 @@ def test_heightfield_statistics_at_1e8_photons(engine):
     print(f"max |z|: rings {zr.max():.2f}, TOF {zt.max():.2f} over {zt.size} bins "
           f"({np.mean(zt < 3):.3f} within 3, mean z^2 {np.mean(zt ** 2):.2f}), R {z[-2]:.2f}, A {z[-1]:.2f}; "
           f"R {mg[-2]:.5f} vs {mc[-2]:.5f}, A {mg[-1]:.5f} vs {mc[-1]:.5f}")
-    assert zr.size == 20 and np.all(zr < 3.0)
+    assert zr.size == 20 and np.mean(zr < 3.0) >= 0.9 and zr.max() < 3.5 and np.mean(zr ** 2) < 1.6
     assert z[-2] < 3.0 and z[-1] < 3.0
     assert zt.size > 40
     assert np.mean(zt < 3.0) >= 0.97 and zt.max() < 4.5 and np.mean(zt ** 2) < 1.5
@@ -357,3 +360,38 @@ def test_photon_indices_are_capped_at_2_33(engine):
         sc.collect()
         with pytest.raises(engine.EngineError, match="2\\^33"):
             sc.run(1, (1 << 33) - 100, 101)
+
+
+def test_heightfield_paired_with_the_double_walk(engine):
+    """Float-transport bias, with the power that independent streams lack:
+    the GPU's photons 0..4e5 of seed 7 on the 98k-triangle heightfield against
+    the same photons walked in double arithmetic on the same 24-bit uniforms
+    (the 64-bit oracle in its recorded-sequence mode, whose arithmetic is the
+    reference's).  Most photons take identical decisions, so per-batch
+    differences are small and their mean has a standard error far below that
+    of an independent comparison.  Every ring, R and A: |mean difference| <
+    3 paired standard errors, and < 2e-3 relative (measured at most 1.2e-3 on
+    the outer rings, 1e-5 on R and A)."""
+    spec = scenes.heightfield_scene(0)
+    inp = InputHandle(spec)
+    nb, n = 16, 25_000
+    G, D = [], []
+    with engine.Scene(spec, absorption=False) as sc:
+        for b in range(nb):
+            sc.run(7, b * n, n)
+            g = sc.collect()
+            st = sc.stats()
+            G.append(np.concatenate([g.det_weight() / n, [g.det_weight().sum() / n, st["wAbsorbed"] / n]]))
+    print(f"paired heightfield: GPU {nb} x {n} photons of seed 7 done; double walk of the same photons", flush=True)
+    for b in range(nb):
+        o = OutputHandle(spec, absorption=False)
+        c = ol.run_oracle(spec, inp, o, 7, b * n, n, 65, 0)
+        D.append(np.concatenate([o.det_weight() / n, [o.det_weight().sum() / n, c.wAbsorbed / n]]))
+    d = np.array(G) - np.array(D)
+    se = d.std(0, ddof=1) / math.sqrt(nb)
+    z = np.abs(d.mean(0)) / np.where(se > 0, se, 1.0)
+    rel = np.abs(d.mean(0)) / np.array(D).mean(0)
+    print(f"paired: max |z| {z.max():.2f}, max relative difference {rel.max():.2e} (R {rel[-2]:.2e}, "
+          f"A {rel[-1]:.2e})")
+    assert np.all(z < 3.0), z
+    assert np.all(rel < 2e-3), rel
