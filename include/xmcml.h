@@ -300,6 +300,21 @@ int xmc_scene_query(xmc_scene* scene, const double* origins, const double* direc
                     int* found, double* distance, int* surfaceId, int* triangleId,
                     double* normal);
 
+/* xmc_scene_query on device memory, allocating nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory --
+ * code under HIP graph capture, tight serving loops.  The rays (float
+ * origin[3], direction[3], step; step <= 0 = unlimited) and the outputs are
+ * the caller's device buffers, laid out as xmc_scene_query's; the walk and
+ * its results are the same.  The BVH stack spill lives in a static 32 MiB
+ * device array per GPU (held until the process ends) shared by at most
+ * 131072 lanes; a tree too deep for 256 lanes' spill to fit is rejected with
+ * XMC_ERR_LIMIT.  Queued on hipStream (NULL = the scene's stream) and not
+ * waited for.  Every call on one GPU shares that GPU's array: calls that
+ * overlap in time on one GPU race. */
+int xmc_scene_query_device(xmc_scene* scene, const float* origins, const float* directions, const float* steps,
+                           int numberOfRays, int bruteForce, int* found, float* distance, int* surfaceId,
+                           int* triangleId, float* normal, void* hipStream);
+
 /* BVH built by the scene, in the reference's layout (for parity tests):
  * node arrays in the order GenerateBVHTree emits them (post-order, root
  * last).  Pass NULL arrays to query the sizes. */

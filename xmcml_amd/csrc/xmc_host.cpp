@@ -23,6 +23,8 @@
 extern "C" {
 hipError_t xmc_launch_transport(const xmc::Params* p, int grid, int block, size_t lds, hipStream_t s, int trace);
 hipError_t xmc_launch_query(const xmc::QueryParams* q, const int* perm, int block, size_t lds, hipStream_t s);
+hipError_t xmc_launch_query_stride(const xmc::QueryParams* q, const int* perm, int lanes, size_t lds, hipStream_t s);
+hipError_t xmc_query_scratch(int device, int2** ptr, size_t* entries);
 hipError_t xmc_launch_zero(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t xmc_transport_occupancy(int block, size_t lds, int* blocks_per_cu);
 hipError_t xmc_transport_occupancy_plain(int block, size_t lds, int* blocks_per_cu);
@@ -934,6 +936,52 @@ int xmc_scene_query(xmc_scene* sc, const double* o, const double* d, const doubl
     HIP_TRY(hipMemcpy(hn.data(), fn.p, hn.size() * 4, hipMemcpyDeviceToHost));
     for (int i = 0; i < nr; i++) dist[i] = hdist[i];
     for (size_t i = 0; i < hn.size(); i++) normal[i] = hn[i];
+    return XMC_OK;
+}
+
+int xmc_scene_query_device(xmc_scene* sc, const float* o, const float* d, const float* steps, int nr, int brute,
+                           int* found, float* dist, int* surf, int* tri, float* normal, void* stream) {
+    if (!sc || nr < 0) return fail(XMC_ERR_ARG, "bad query arguments");
+    if (nr == 0) return XMC_OK;
+    if (!o || !d || !steps || !found || !dist || !surf || !tri || !normal)
+        return fail(XMC_ERR_ARG, "NULL device buffer");
+    HIP_TRY(hipSetDevice(sc->device));
+    int2* scratch = nullptr;
+    size_t entries = 0;
+    HIP_TRY(xmc_query_scratch(sc->device, &scratch, &entries));
+    // the stack entries beyond the LDS ones spill to the scratch, [entry][lane]
+    const int need = sc->ref.depth - xmc::kStackLds;
+    const int sdepth = need > 0 ? need : 0;
+    int lanes = xmc::kQueryLanes;
+    if (sdepth > 0 && entries / (size_t)sdepth < (size_t)lanes) lanes = (int)(entries / (size_t)sdepth / 256 * 256);
+    if (lanes < 256)
+        return fail(XMC_ERR_LIMIT, "BVH of depth " + std::to_string(sc->ref.depth) +
+                                       ": 256 lanes' stack spill exceeds the query scratch of " +
+                                       std::to_string(xmc::kQueryScratchBytes >> 20) + " MiB");
+    xmc::QueryParams Q;
+    std::memset(&Q, 0, sizeof(Q));
+    Q.nodes = sc->base.nodes;
+    Q.tris = sc->base.tris;
+    Q.root_ref = sc->base.root_ref;
+    for (int a = 0; a < 3; a++) {
+        Q.root_lo[a] = sc->base.root_lo[a];
+        Q.root_hi[a] = sc->base.root_hi[a];
+    }
+    Q.origins = o;
+    Q.directions = d;
+    Q.steps = steps;
+    Q.n = nr;
+    Q.brute = brute;
+    Q.n_tris = (int)sc->perm.size();
+    Q.found = found;
+    Q.distance = dist;
+    Q.surface = surf;
+    Q.triangle = tri;
+    Q.normal = normal;
+    Q.spill = scratch;
+    Q.spill_depth = sdepth;
+    hipStream_t s = stream ? (hipStream_t)stream : sc->stream;
+    HIP_TRY(xmc_launch_query_stride(&Q, sc->perm_dev.p, lanes, (size_t)xmc::kStackLds * 256 * sizeof(int2), s));
     return XMC_OK;
 }
 

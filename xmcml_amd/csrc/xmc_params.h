@@ -17,6 +17,11 @@ constexpr int kJumpBits = 57 - kStreamLog2;
 constexpr uint64_t kMcgA = 302875106592253ULL;     // 13^13, mcml_mcg59.cpp:3
 constexpr uint64_t kMcgMask = 576460752303423487ULL;  // 2^59 - 1
 constexpr int kStackLds = 8;           // LDS entries of the per-lane short stack
+// Allocation-free ray queries: the per-lane stack spill lives in a static
+// device array of this size; at most kQueryLanes lanes share it.
+constexpr size_t kQueryScratchBytes = 32u << 20;
+constexpr int kQueryLanes = 131072;
+constexpr int kMaxDevices = 64;
 // A photon whose stream block is nearly used up is retired (its weight goes
 // to the "other" ledger entry): drawing past 2^24 would reuse the next
 // photon's numbers.  One interaction draws at most 4 numbers plus redraws.

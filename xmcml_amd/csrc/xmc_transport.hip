@@ -22,6 +22,8 @@
 // deeper entries to a per-lane global area.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "xmc_params.h"
 
 namespace xmc {
@@ -1314,13 +1316,9 @@ template __global__ void transport_kernel<true, false, false>(Params);
 template __global__ void transport_kernel<false, true, false>(Params);
 template __global__ void transport_kernel<true, true, false>(Params);
 
-// One ray per thread: nearest hit through the BVH or over every triangle.
-__global__ void __launch_bounds__(256) query_kernel(QueryParams Q, const int* perm) {
-    extern __shared__ int2 lds_stack[];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st{lds_stack, (int)blockDim.x, (int)threadIdx.x, Q.spill, (int)(gridDim.x * blockDim.x), i,
-             Q.spill_depth, 0, 0};
-    if (i >= Q.n) return;
+// The nearest hit of ray i, through the BVH or over every triangle, written to
+// the query's outputs.
+__device__ __forceinline__ void query_one(const QueryParams& Q, const int* perm, int i, Stack& st) {
     V3 o = {Q.origins[3 * i], Q.origins[3 * i + 1], Q.origins[3 * i + 2]};
     V3 d = {Q.directions[3 * i], Q.directions[3 * i + 1], Q.directions[3 * i + 2]};
     float step = Q.steps[i];
@@ -1362,6 +1360,26 @@ __global__ void __launch_bounds__(256) query_kernel(QueryParams Q, const int* pe
     }
 }
 
+// One ray per thread.
+__global__ void __launch_bounds__(256) query_kernel(QueryParams Q, const int* perm) {
+    extern __shared__ int2 lds_stack[];
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st{lds_stack, (int)blockDim.x, (int)threadIdx.x, Q.spill, (int)(gridDim.x * blockDim.x), i,
+             Q.spill_depth, 0, 0};
+    if (i >= Q.n) return;
+    query_one(Q, perm, i, st);
+}
+
+// A fixed grid of lanes, each taking rays i, i + lanes, ...: the stack spill
+// area is sized by the lanes, not by the rays.
+__global__ void __launch_bounds__(256) query_stride_kernel(QueryParams Q, const int* perm) {
+    extern __shared__ int2 lds_stack[];
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lanes = gridDim.x * blockDim.x;
+    Stack st{lds_stack, (int)blockDim.x, (int)threadIdx.x, Q.spill, lanes, lane, Q.spill_depth, 0, 0};
+    for (int i = lane; i < Q.n; i += lanes) query_one(Q, perm, i, st);
+}
+
 // Fills n 64-bit words with zero (capture-safe replacement for memset).
 __global__ void zero_kernel(unsigned long long* p, size_t n) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -1369,7 +1387,18 @@ __global__ void zero_kernel(unsigned long long* p, size_t n) {
     for (; i < n; i += s) p[i] = 0ull;
 }
 
+// The BVH-stack spill of the allocation-free ray query (xmc_scene_query_device):
+// 32 MiB per GPU, reserved when the library's code object loads and held until
+// the process ends.  Host code never names it: it takes the per-device address
+// from query_scratch() below.
+static __device__ __align__(256) int2 scratch[kQueryScratchBytes / sizeof(int2)];
+
 }  // namespace xmc
+
+namespace {
+std::mutex g_scratch_mutex;
+int2* g_scratch_ptr[xmc::kMaxDevices] = {};
+}  // namespace
 
 template <bool TRACE, bool PATHS>
 static void launch_transport(const xmc::Params* p, bool interleave, int grid, int block, size_t lds, hipStream_t s) {
@@ -1392,6 +1421,37 @@ hipError_t xmc_launch_transport(const xmc::Params* p, int grid, int block, size_
     else if (trace) launch_transport<true, false>(p, interleave, grid, block, lds, s);
     else if (paths) launch_transport<false, true>(p, interleave, grid, block, lds, s);
     else launch_transport<false, false>(p, interleave, grid, block, lds, s);
+    return hipGetLastError();
+}
+
+// The address of `scratch` on `device`: looked up once per device, with that
+// device current, and kept (each GPU holds its own copy of the array).
+hipError_t xmc_query_scratch(int device, int2** ptr, size_t* entries) {
+    if (device < 0 || device >= xmc::kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device] == nullptr) {
+        hipError_t e = hipSetDevice(device);
+        if (e != hipSuccess) return e;
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(xmc::scratch));
+        if (e != hipSuccess) return e;
+        if (p == nullptr) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = static_cast<int2*>(p);
+    }
+    *ptr = g_scratch_ptr[device];
+    *entries = xmc::kQueryScratchBytes / sizeof(int2);
+    return hipSuccess;
+}
+
+// Lanes of query_stride_kernel: at most kQueryLanes, fewer when the tree is so
+// deep that their spill would not fit in the scratch array.
+hipError_t xmc_launch_query_stride(const xmc::QueryParams* q, const int* perm, int lanes, size_t lds, hipStream_t s) {
+    const int block = 256;
+    int grid = lanes / block;
+    const int need = (q->n + block - 1) / block;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(xmc::query_stride_kernel, dim3(grid), dim3(block), lds, s, *q, perm);
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ _lib = None
 
 EXPORTS = [
     "xmc_scene_create", "xmc_scene_destroy", "xmc_scene_run", "xmc_scene_sync", "xmc_scene_collect", "xmc_scene_stats",
-    "xmc_scene_trace", "xmc_scene_query", "xmc_scene_bvh", "xmc_scene_set_launch", "xmc_scene_get_launch",
+    "xmc_scene_trace", "xmc_scene_query", "xmc_scene_query_device", "xmc_scene_bvh", "xmc_scene_set_launch", "xmc_scene_get_launch",
     "xmc_batch_create", "xmc_batch_destroy", "xmc_batch_run", "xmc_batch_collect", "xmc_batch_stats",
     "xmc_batch_set_launch", "xmc_batch_get_launch", "xmc_distance_grid", "xmc_launch", "xmc_init_output",
     "xmc_free_output", "xmc_specular_reflectance", "xmc_last_error", "xmc_version",
@@ -71,6 +71,7 @@ def lib():
     sig("xmc_scene_stats", _I, [_P, _P])
     sig("xmc_scene_trace", _I, [_P, _U64, _U64, _U64, _P, _I, _PI])
     sig("xmc_scene_query", _I, [_P, _PD, _PD, _PD, _I, _I, _PI, _PD, _PI, _PI, _PD])
+    sig("xmc_scene_query_device", _I, [_P, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P])
     sig("xmc_scene_bvh", _I, [_P, _PI, _PI, _PI, _PD, _PI, _PI, _PI])
     sig("xmc_scene_set_launch", _I, [_P, _I, _I])
     sig("xmc_scene_get_launch", _I, [_P, _PI, _PI])
