@@ -201,3 +201,31 @@ def test_surrogate_beats_plain_mc_at_equal_cost(dev):
           f"{t_train_mc:.1f} s")
     assert err_s < 0.25
     assert err_s < err_mc
+
+
+def test_answer_uses_the_network_or_monte_carlo(dev):
+    """Surrogate.answer on in-range, out-of-range and uncertainty-flagged
+    configurations: the flagged ones (outside the box, or above sigma_max)
+    equal plain batched MC of the same configurations at the same seed, bit
+    for bit; the others equal the network's prediction."""
+    from pathlib import Path
+    from xmcml_amd import surrogate as S
+    model = Path(__file__).resolve().parents[1] / "xmcml_amd" / "data" / "surrogate_layered.npz"
+    sur = S.Surrogate.load(model, max_batch=4096)
+    cfg = S.draw_configs(40, 17)
+    cfg[3] = [(1.4, 2.0, 10.0, 0.9, 1.0)] + list(cfg[3][1:])  # mua outside the box
+    cfg[7] = cfg[7][:1]                                        # a single layer
+    mu, sig = sur.predict(S.encode_many([c for i, c in enumerate(cfg) if i not in (3, 7)]))
+    unc = sur.uncertainty(sig)
+    sur.sigma_max = float(np.median(unc))  # flag about half of the in-range ones
+    print(f"answer: 40 configurations, sigma_max {sur.sigma_max:.4f}", flush=True)
+    rings, tof, mc = sur.answer(cfg, mc_photons=20_000, seed=5)
+    assert mc[3] and mc[7]
+    flagged_in_range = [i for i in range(40) if mc[i] and i not in (3, 7)]
+    assert 10 <= len(flagged_in_range) <= 30
+    fb = [cfg[i] for i in range(40) if mc[i]]
+    _, _, _, W = S.simulate(fb, 20_000, 5)
+    assert np.array_equal(np.concatenate([rings[mc], tof[mc]], axis=1), W)
+    ok = ~mc
+    mu_ok, _ = sur.predict(S.encode_many([cfg[i] for i in range(40) if ok[i]]))
+    assert np.array_equal(np.concatenate([rings[ok], tof[ok]], axis=1), np.exp(mu_ok))

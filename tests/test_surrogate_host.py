@@ -142,3 +142,34 @@ def test_standardisation_masked():
     mean, std = S.Surrogate.standardisation(Y, M)
     assert mean[0] == pytest.approx(2.0) and mean[1] == pytest.approx(5.0)
     assert std[0] == pytest.approx(1.0)
+
+
+def test_vectorised_encoding_and_sampling():
+    """encode_arrays / in_range_arrays on (n, 5, 5) layer arrays give what the
+    per-configuration calls give; draw_config_arrays samples the distribution
+    of scenes.draw_layers (every configuration inside the box, marginals of
+    20,000 draws within 4 standard errors of draw_configs' at the deciles)."""
+    import math as m
+    from xmcml_amd import surrogate as S
+    L, cnt = S.draw_config_arrays(20000, 5)
+    assert np.all(S.in_range_arrays(L, cnt))
+    cfg = S.arrays_to_configs(L[:500], cnt[:500])
+    X = S.encode_arrays(L[:500], cnt[:500])
+    assert np.array_equal(X, np.stack([S.encode(c) for c in cfg]))
+    assert [S.in_range(c) for c in cfg] == list(S.in_range_arrays(L[:500], cnt[:500]))
+    bad = [list(c) for c in cfg[:3]]
+    bad[0][0] = (1.4, 2.0, 10.0, 0.9, 1.0)  # mua 2 /mm is outside the box
+    bad[1] = bad[1][:1]                     # one layer
+    Lb, cb = S.configs_to_arrays(bad)
+    assert list(S.in_range_arrays(Lb, cb)) == [False, False, True]
+    ref = S.configs_to_arrays(S.draw_configs(20000, 6))
+    qs = np.linspace(0.1, 0.9, 9)
+    for k, name in enumerate(["n", "mua", "mus", "g", "d"]):
+        a = L[:, :, k][np.arange(5)[None, :] < cnt[:, None]]
+        b = ref[0][:, :, k][np.arange(5)[None, :] < ref[1][:, None]]
+        qa, qb = np.quantile(np.log(a), qs), np.quantile(np.log(b), qs)
+        # quantile standard error sqrt(p(1-p)/n) / density, density >= 0.35 / sd for these shapes;
+        # two independent samples
+        se = np.std(np.log(b)) * np.sqrt(qs * (1 - qs)) / m.sqrt(len(b)) / 0.35 * m.sqrt(2.0)
+        assert np.all(np.abs(qa - qb) < 4 * se + 1e-12), name
+    assert abs(cnt.mean() - ref[1].mean()) < 4 * ref[1].std() / m.sqrt(len(cnt))

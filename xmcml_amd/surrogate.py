@@ -47,39 +47,99 @@ LAYER_BOX = [(1e-3, 1.0, True), (0.5, 5.0, True), (0.7, 0.95, False), (1.33, 1.5
 Layer = Tuple[float, float, float, float, float]  # (n, mua, mus, g, thickness), as scenes.layered_scene
 
 
-def _unit(v, lo, hi, log):
-    if log:
-        return 2.0 * (math.log(v) - math.log(lo)) / (math.log(hi) - math.log(lo)) - 1.0
-    return 2.0 * (v - lo) / (hi - lo) - 1.0
+def configs_to_arrays(configs: Sequence[Sequence[Layer]]) -> Tuple[np.ndarray, np.ndarray]:
+    """(n, 5, 5) layer tuples (n, mua, mus, g, d), zero-padded, and the layer
+    counts (n,) of a list of configurations."""
+    n = len(configs)
+    L = np.zeros((n, MAX_LAYERS, 5))
+    cnt = np.zeros(n, np.int64)
+    for i, c in enumerate(configs):
+        if len(c) > MAX_LAYERS:
+            raise ValueError(f"{len(c)} layers: the surrogate takes 1..{MAX_LAYERS}")
+        cnt[i] = len(c)
+        if c:
+            L[i, :len(c)] = np.asarray(c, dtype=np.float64)
+    return L, cnt
+
+
+def _layer_values(L: np.ndarray) -> np.ndarray:
+    """(..., 5) of (mua, mus', g, n, d) from layer tuples (n, mua, mus, g, d)."""
+    return np.stack([L[..., 1], L[..., 2] * (1.0 - L[..., 3]), L[..., 3], L[..., 0], L[..., 4]], axis=-1)
+
+
+def encode_arrays(L: np.ndarray, count: np.ndarray) -> np.ndarray:
+    """Feature rows (n, 30) of configurations given as (n, 5, 5) layer tuples and
+    layer counts: each used slot's (mua, mus', g, n, d) mapped from LAYER_BOX to
+    [-1, 1] (log scale where the box says), then the 5 slot-occupied flags."""
+    count = np.asarray(count, np.int64)
+    if count.size and (count.min() < 1 or count.max() > MAX_LAYERS):
+        raise ValueError(f"layer counts must be 1..{MAX_LAYERS}")
+    used = np.arange(MAX_LAYERS)[None, :] < count[:, None]
+    V = _layer_values(np.asarray(L, np.float64))
+    F = np.zeros((len(count), MAX_LAYERS, 5))
+    for k, (lo, hi, lg) in enumerate(LAYER_BOX):
+        v = V[..., k]
+        if lg:
+            with np.errstate(divide="ignore", invalid="ignore"):
+                u = 2.0 * (np.log(np.where(used, v, lo)) - math.log(lo)) / (math.log(hi) - math.log(lo)) - 1.0
+        else:
+            u = 2.0 * (v - lo) / (hi - lo) - 1.0
+        F[..., k] = np.where(used, u, 0.0)
+    X = np.zeros((len(count), N_INPUTS), np.float32)
+    X[:, :5 * MAX_LAYERS] = F.reshape(len(count), -1)
+    X[:, 5 * MAX_LAYERS:] = used
+    return X
 
 
 def encode(layers: Sequence[Layer]) -> np.ndarray:
     """Feature vector (30,) of one configuration."""
     if not 1 <= len(layers) <= MAX_LAYERS:
         raise ValueError(f"{len(layers)} layers: the surrogate takes 1..{MAX_LAYERS}")
-    x = np.zeros(N_INPUTS, np.float32)
-    for i, (n, mua, mus, g, d) in enumerate(layers):
-        vals = (mua, mus * (1.0 - g), g, n, d)
-        for k, (v, (lo, hi, lg)) in enumerate(zip(vals, LAYER_BOX)):
-            x[5 * i + k] = _unit(v, lo, hi, lg)
-        x[5 * MAX_LAYERS + i] = 1.0
-    return x
+    return encode_arrays(*configs_to_arrays([layers]))[0]
 
 
 def encode_many(configs: Sequence[Sequence[Layer]]) -> np.ndarray:
-    return np.stack([encode(c) for c in configs]) if configs else np.zeros((0, N_INPUTS), np.float32)
+    return encode_arrays(*configs_to_arrays(configs)) if configs else np.zeros((0, N_INPUTS), np.float32)
+
+
+def in_range_arrays(L: np.ndarray, count: np.ndarray, tol: float = 1e-9) -> np.ndarray:
+    """Per configuration: 2..5 layers, every used layer inside the sampling box."""
+    count = np.asarray(count, np.int64)
+    used = np.arange(MAX_LAYERS)[None, :] < count[:, None]
+    V = _layer_values(np.asarray(L, np.float64))
+    ok = (count >= 2) & (count <= MAX_LAYERS)
+    for k, (lo, hi, _) in enumerate(LAYER_BOX):
+        inside = (V[..., k] >= lo * (1 - tol)) & (V[..., k] <= hi * (1 + tol))
+        ok &= np.all(inside | ~used, axis=1)
+    return ok
 
 
 def in_range(layers: Sequence[Layer], tol: float = 1e-9) -> bool:
     """True when every layer lies in the sampling box (and 2..5 layers)."""
     if not 2 <= len(layers) <= MAX_LAYERS:
         return False
-    for (n, mua, mus, g, d) in layers:
-        vals = (mua, mus * (1.0 - g), g, n, d)
-        for v, (lo, hi, _) in zip(vals, LAYER_BOX):
-            if not (lo * (1 - tol) <= v <= hi * (1 + tol)):
-                return False
-    return True
+    return bool(in_range_arrays(*configs_to_arrays([layers]), tol=tol)[0])
+
+
+def draw_config_arrays(count: int, seed: int) -> Tuple[np.ndarray, np.ndarray]:
+    """`count` configurations of the batched workload's distribution (the one
+    scenes.draw_layers samples), drawn vectorised: (n, 5, 5) layer tuples and
+    layer counts uniform in 2..5."""
+    rng = np.random.default_rng(seed)
+    cnt = rng.integers(2, MAX_LAYERS + 1, size=count)
+    shape = (count, MAX_LAYERS)
+    mua = np.exp(rng.uniform(math.log(1e-3), math.log(1.0), shape))
+    musp = np.exp(rng.uniform(math.log(0.5), math.log(5.0), shape))
+    g = rng.uniform(0.7, 0.95, shape)
+    n = rng.uniform(1.33, 1.5, shape)
+    d = np.exp(rng.uniform(math.log(0.05), math.log(10.0), shape))
+    L = np.stack([n, mua, musp / (1.0 - g), g, d], axis=-1)
+    L[np.arange(MAX_LAYERS)[None, :] >= cnt[:, None]] = 0.0
+    return L, cnt
+
+
+def arrays_to_configs(L: np.ndarray, count: np.ndarray) -> List[List[Layer]]:
+    return [[tuple(float(v) for v in L[i, j]) for j in range(int(count[i]))] for i in range(len(count))]
 
 
 def draw_configs(count: int, seed: int) -> List[List[Layer]]:
@@ -301,15 +361,38 @@ class Surrogate:
         """Raw head output for feature rows already on the device."""
         return self.mlp.forward(x, out)
 
+    def predict_raw(self, X: np.ndarray) -> np.ndarray:
+        """Head output (n, 192) of host feature rows: copied to the device and
+        run in chunks of the model's batch size on two reused device buffers."""
+        n = len(X)
+        O2 = 2 * self.y_mean.size
+        out = np.empty((n, O2), np.float32)
+        if n == 0:
+            return out
+        rows = min(self.mlp.max_batch, n)
+        x = engine.DeviceArray((rows, self.mlp.inputs), device=self.mlp.device)
+        o = engine.DeviceArray((rows, O2), device=self.mlp.device)
+        try:
+            for r0 in range(0, n, rows):
+                m = min(rows, n - r0)
+                if m != rows:  # the last, shorter chunk
+                    x.close()
+                    o.close()
+                    x = engine.DeviceArray((m, self.mlp.inputs), device=self.mlp.device)
+                    o = engine.DeviceArray((m, O2), device=self.mlp.device)
+                x.upload(X[r0:r0 + m])
+                self.mlp.forward(x, o)
+                out[r0:r0 + m] = o.numpy()
+        finally:
+            x.close()
+            o.close()
+        return out
+
     def predict(self, X: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
         """(ln-value means, ln-value standard deviations), each (n, 96)."""
         if len(X) == 0:
             return np.zeros((0, self.y_mean.size)), np.zeros((0, self.y_mean.size))
-        x = engine.DeviceArray(X.shape, X, self.mlp.device)
-        o = self.mlp.forward(x)
-        out = o.numpy()
-        x.close()
-        o.close()
+        out = self.predict_raw(X)
         O = self.y_mean.size
         mu = out[:, :O] * self.y_std + self.y_mean
         sig = np.exp(0.5 * np.clip(out[:, O:], -10, 10)) * self.y_std
@@ -319,24 +402,42 @@ class Surrogate:
         """Per-configuration score: mean predicted sd of ln R over the inner 16 rings."""
         return sig[:, :16].mean(axis=1)
 
-    def answer(self, configs: Sequence[Sequence[Layer]], mc_photons: int = 1_000_000, seed: int = 1):
-        """Ring reflectance (n, 32) and TOF (n, 64) weight fractions for every
-        configuration, with a boolean array of those answered by Monte Carlo."""
-        n = len(configs)
-        ok = np.array([in_range(c) for c in configs], bool)
+    def answer_arrays(self, L: np.ndarray, count: np.ndarray, mc_photons: int = 1_000_000, seed: int = 1):
+        """answer() on configurations given as (n, 5, 5) layer tuples and layer
+        counts.  Returns (rings (n, 32), TOF (n, 64), answered-by-MC flags (n,),
+        seconds spent in each stage)."""
+        t = {}
+        t0 = time.perf_counter()
+        n = len(count)
+        ok = in_range_arrays(L, count)
+        idx = np.nonzero(ok)[0]
+        X = encode_arrays(L[idx], count[idx])
+        t["encode_s"] = time.perf_counter() - t0
         R = np.zeros((n, N_OUTPUTS))
-        if ok.any():
-            X = encode_many([configs[i] for i in np.nonzero(ok)[0]])
+        t0 = time.perf_counter()
+        if idx.size:
             mu, sig = self.predict(X)
             good = self.uncertainty(sig) <= self.sigma_max
-            idx = np.nonzero(ok)[0]
             R[idx] = np.exp(mu)
             ok[idx[~good]] = False
+        t["infer_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
         fb = np.nonzero(~ok)[0]
         if fb.size:
-            _, _, _, W = simulate([configs[i] for i in fb], mc_photons, seed)
+            _, _, _, W = simulate(arrays_to_configs(L[fb], count[fb]), mc_photons, seed)
             R[fb] = W
-        return R[:, :RINGS], R[:, RINGS:], ~ok
+        t["fallback_s"] = time.perf_counter() - t0
+        return R[:, :RINGS], R[:, RINGS:], ~ok, t
+
+    def answer(self, configs: Sequence[Sequence[Layer]], mc_photons: int = 1_000_000, seed: int = 1):
+        """Ring reflectance (n, 32) and TOF (n, 64) weight fractions for every
+        configuration, with a boolean array of those answered by Monte Carlo:
+        configurations outside the sampling box, or whose predicted uncertainty
+        exceeds sigma_max, are simulated (plain batched MC, mc_photons each, in
+        their order, from `seed`); the others come from the network."""
+        L, cnt = configs_to_arrays(configs)
+        rings, tof, mc, _ = self.answer_arrays(L, cnt, mc_photons, seed)
+        return rings, tof, mc
 
     # ---- persistence ----------------------------------------------------
     def save(self, path: Path, meta: Optional[dict] = None):
