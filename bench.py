@@ -66,7 +66,11 @@ def bench_batch(args) -> int:
     specs = scenes.batched_configs(1024, photon_scale=args.batch_scale)
     total = sum(s.photons for s in specs)
     print(f"bench: batch of {len(specs)} configurations, {total:.3e} photons", file=sys.stderr, flush=True)
-    b = engine.Batch(specs)
+    # launch shape from the autotuner: candidates timed on the same configurations at 5 % of their
+    # budgets (cached per device and workload), the winner applied to the full graph
+    b = engine.Batch(specs, tune=True, tune_scale=0.05)
+    print(f"bench: launch shape {b.tuning['block']} threads x {b.tuning['blocks_per_cu']} blocks/CU "
+          f"({'cached' if b.tuning['cached'] else 'tuned'})", file=sys.stderr, flush=True)
     b.run(args.seed)  # warm-up launch (and graph capture); the timed launch replays the same graph
     b.collect()
     b.stats()
@@ -75,6 +79,7 @@ def bench_batch(args) -> int:
     outs = b.collect()
     elapsed = time.perf_counter() - t0
     st = b.stats()
+    launched = b.get_launch()
     b.close()
     value = total / elapsed
     line = {
@@ -83,6 +88,9 @@ def bench_batch(args) -> int:
         "vs_baseline": None, "dtype": "fp32 transport, fp64 tallies", "data": "synthetic",
         "config": {"workload": WORKLOADS["batch"], "configurations": len(specs), "photons": total},
         "configs_per_s": len(specs) / elapsed, "kernel_ms": st["kernelMs"],
+        "launch_shape": {"block": b.tuning["block"], "blocks_per_cu": b.tuning["blocks_per_cu"],
+                         "launched": list(launched), "cached": b.tuning["cached"],
+                         "candidates_ms_at_5pct_budgets": b.tuning.get("timings")},
         "scatter_events_per_photon": st["scatterEvents"] / total,
     }
     if args.cpu_seconds > 0:

@@ -386,6 +386,10 @@ int xmc_init_output(const xmc_input* input, xmc_output* output);
 void xmc_free_output(xmc_output* output);
 
 double xmc_specular_reflectance(const xmc_layer* layers);
+/* The HIP device's name and architecture ("AMD Instinct MI355X gfx950:sramecc+:xnack-"),
+ * NUL-terminated in name[capacity], and its compute-unit count; for caches of
+ * tuned launch shapes.  No counterpart in the reference. */
+int xmc_device_name(int device, char* name, int capacity, int* computeUnits);
 const char* xmc_last_error(void);
 const char* xmc_version(void);
 

@@ -73,3 +73,26 @@ def test_autotuner_picks_a_measured_shape(engine, tmp_path, monkeypatch):
         assert block == res["block"]
         res2 = autotune.tune_batch(b)
         assert res2["cached"] and res2["block"] == res["block"]
+
+
+def test_tuned_batch_gives_the_same_answers(engine, tmp_path, monkeypatch):
+    """Batch(tune=True) times the candidate shapes on a copy at 1 % of the
+    budgets, keyed on the HIP device, and applies the winner; photon p of
+    configuration c keeps its stream, so the tallies equal the default
+    shape's."""
+    monkeypatch.setenv("XMC_AUTOTUNE_CACHE", str(tmp_path / "cache.json"))
+    specs = small_batch(24, 1e-2, seed=4)
+    with engine.Batch(specs) as b:
+        b.run(3)
+        ref = b.collect()
+    with engine.Batch(specs, tune=True) as b:
+        t = b.tuning
+        assert not t["cached"] and len(t["timings"]) == len(autotune.CANDIDATES)
+        assert b.get_launch()[0] == t["block"]
+        b.run(3)
+        outs = b.collect()
+    key = next(iter(__import__("json").loads((tmp_path / "cache.json").read_text())))
+    assert engine.device_name() in key
+    for c in range(len(specs)):
+        assert np.array_equal(outs[c].det_count(), ref[c].det_count()), c
+        np.testing.assert_allclose(outs[c].det_weight(), ref[c].det_weight(), rtol=1e-11, atol=1e-14)

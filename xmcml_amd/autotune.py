@@ -3,9 +3,9 @@
 The tunable knobs are the workgroup size and the number of resident
 workgroups per CU (which, for a persistent kernel, sets the number of lanes
 and therefore photons per lane).  A candidate is timed with the kernel's own
-HIP-event time on a sample of the workload; the winner is cached per device
-and workload signature in a JSON file (XMC_AUTOTUNE_CACHE, default
-~/.cache/xmcml_amd/autotune.json).
+HIP-event time on a sample of the workload; the winner is cached per HIP
+device (its name, architecture and compute units) and workload signature in a
+JSON file (XMC_AUTOTUNE_CACHE, default ~/.cache/xmcml_amd/autotune.json).
 """
 from __future__ import annotations
 
@@ -15,7 +15,8 @@ import os
 from pathlib import Path
 from typing import Callable, Iterable, List, Tuple
 
-CANDIDATES: List[Tuple[int, int]] = [(256, 0), (128, 0), (64, 0), (256, 2), (128, 4)]
+# (threads per workgroup, workgroups per CU; 0 = as many as fit)
+CANDIDATES: List[Tuple[int, int]] = [(256, 0), (256, 4), (256, 3), (192, 0), (128, 0), (64, 0)]
 
 
 def cache_path() -> Path:
@@ -70,10 +71,9 @@ def tune(key: str, apply: Callable[[int, int], None], time_sample: Callable[[], 
     return dict(res, cached=False)
 
 
-def tune_scene(scene, seed: int = 1, sample_photons: int = 200_000, use_cache: bool = True) -> dict:
-    from .engine import lib  # noqa: F401  (fails loudly without the library)
-    dev = os.environ.get("XMC_DEVICE_NAME", "gfx950")
-    key = f"scene:{dev}:{signature('scene', [scene.spec])}:{sample_photons}"
+def tune_scene(scene, seed: int = 1, sample_photons: int = 200_000, use_cache: bool = True, device: int = 0) -> dict:
+    from .engine import device_name
+    key = f"scene:{device_name(device)}:{signature('scene', [scene.spec])}:{sample_photons}"
 
     def sample():
         scene.run(seed, 0, sample_photons)
@@ -84,11 +84,11 @@ def tune_scene(scene, seed: int = 1, sample_photons: int = 200_000, use_cache: b
     return tune(key, scene.set_launch, sample, use_cache=use_cache)
 
 
-def tune_batch(sample_batch, apply_to=None, seed: int = 1, use_cache: bool = True) -> dict:
+def tune_batch(sample_batch, apply_to=None, seed: int = 1, use_cache: bool = True, device: int = 0) -> dict:
     """Tunes on `sample_batch` (the workload with scaled-down budgets) and
     applies the winner to `apply_to` as well when given."""
-    dev = os.environ.get("XMC_DEVICE_NAME", "gfx950")
-    key = f"batch:{dev}:{signature('batch', sample_batch.specs)}"
+    from .engine import device_name
+    key = f"batch:{device_name(device)}:{signature('batch', sample_batch.specs)}"
 
     def apply(b, n):
         sample_batch.set_launch(b, n, True)

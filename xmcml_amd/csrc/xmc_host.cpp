@@ -1011,6 +1011,19 @@ int xmc_scene_bvh(xmc_scene* sc, int* nn, int* root, int* nt, double* boxes, int
     return XMC_OK;
 }
 
+int xmc_device_name(int device, char* name, int capacity, int* cus) {
+    if (!name || capacity < 1) return fail(XMC_ERR_ARG, "name buffer is NULL or empty");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(XMC_ERR_NOGPU, "no HIP device");
+    if (device < 0 || device >= ndev) return fail(XMC_ERR_ARG, "device ordinal out of range");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    std::string s = std::string(prop.name) + " " + prop.gcnArchName;
+    std::snprintf(name, (size_t)capacity, "%s", s.c_str());
+    if (cus) *cus = prop.multiProcessorCount;
+    return XMC_OK;
+}
+
 int xmc_launch(const xmc_input* in, xmc_output* out, xmc_rng* rng, const xmc_options* opt) {
     if (!in || !out || !rng) return fail(XMC_ERR_ARG, "input/output/rng is NULL");
     xmc_options o;

@@ -31,7 +31,7 @@ EXPORTS = [
     "xmc_scene_trace", "xmc_scene_query", "xmc_scene_query_device", "xmc_scene_bvh", "xmc_scene_set_launch", "xmc_scene_get_launch",
     "xmc_batch_create", "xmc_batch_destroy", "xmc_batch_run", "xmc_batch_collect", "xmc_batch_stats",
     "xmc_batch_set_launch", "xmc_batch_get_launch", "xmc_distance_grid", "xmc_launch", "xmc_init_output",
-    "xmc_free_output", "xmc_specular_reflectance", "xmc_last_error", "xmc_version",
+    "xmc_free_output", "xmc_specular_reflectance", "xmc_device_name", "xmc_last_error", "xmc_version",
     # surrogate model (include/xmcml_surrogate.h)
     "xmc_mlp_create", "xmc_mlp_destroy", "xmc_mlp_param_count", "xmc_mlp_get_params", "xmc_mlp_set_params",
     "xmc_mlp_forward", "xmc_mlp_backward", "xmc_mlp_get_grads", "xmc_mlp_adam", "xmc_mlp_train_step",
@@ -72,6 +72,7 @@ def lib():
     sig("xmc_scene_trace", _I, [_P, _U64, _U64, _U64, _P, _I, _PI])
     sig("xmc_scene_query", _I, [_P, _PD, _PD, _PD, _I, _I, _PI, _PD, _PI, _PI, _PD])
     sig("xmc_scene_query_device", _I, [_P, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _P])
+    sig("xmc_device_name", _I, [_I, C.c_char_p, _I, _PI])
     sig("xmc_scene_bvh", _I, [_P, _PI, _PI, _PI, _PD, _PI, _PI, _PI])
     sig("xmc_scene_set_launch", _I, [_P, _I, _I])
     sig("xmc_scene_get_launch", _I, [_P, _PI, _PI])
@@ -279,6 +280,14 @@ class DeviceArray:
             pass
 
 
+def device_name(device: int = 0) -> str:
+    """The HIP device's name and architecture (xmc_device_name)."""
+    buf = C.create_string_buffer(256)
+    cus = C.c_int()
+    _check(lib().xmc_device_name(device, buf, 256, C.byref(cus)), "xmc_device_name")
+    return f"{buf.value.decode(errors='replace')} {cus.value}CU"
+
+
 def device_sync(device: int = 0):
     _check(lib().xmc_dev_sync(device), "xmc_dev_sync")
 
@@ -331,9 +340,14 @@ class Batch:
     graph); configuration c's budget is specs[c].photons."""
 
     def __init__(self, specs, block_size: int = 0, blocks_per_cu: int = 0, absorption: bool = False,
-                 use_graph: bool = True, device: int = 0):
+                 use_graph: bool = True, device: int = 0, tune: bool = False, tune_scale: float = 0.01):
+        """tune: choose the launch shape with the autotuner (xmcml_amd/autotune.py) on a copy of
+        the workload whose photon budgets are scaled by tune_scale, and apply it here; the
+        choice and every candidate's kernel time are in self.tuning."""
         from .abi import Input
         self.specs = list(specs)
+        self.device = device
+        self.tuning = None
         self.inputs = [InputHandle(s) for s in self.specs]
         arr = (Input * len(self.inputs))(*[h.c for h in self.inputs])
         self._arr = arr
@@ -344,6 +358,12 @@ class Batch:
         self.absorption = absorption
         if not use_graph or block_size or blocks_per_cu:
             self.set_launch(block_size, blocks_per_cu, use_graph)
+        if tune:
+            from . import autotune
+            import dataclasses
+            scaled = [dataclasses.replace(s, photons=max(1000, int(s.photons * tune_scale))) for s in self.specs]
+            with Batch(scaled, absorption=absorption, use_graph=use_graph, device=device) as sample:
+                self.tuning = autotune.tune_batch(sample, apply_to=self, device=device)
 
     def close(self):
         if getattr(self, "_h", None):
