@@ -217,8 +217,11 @@ template <> struct Prec<double> {
     static double pow_(double p, double e) { return pow(p, e); }
     static void sincos_2pi(double u, double* s, double* c) {
         double psi = 2.0 * kPi * u;  // mcml_kernel.cpp:453-455
-        *c = cos(psi);
-        *s = sin(psi);
+        // glibc's sincos: g++ turns the reference's adjacent cos(psi), sin(psi)
+        // into this call, and it differs from separate sin and cos by an ulp on
+        // some angles; calling it by name keeps every compiler on the reference's
+        // arithmetic (the sanitized build uses clang)
+        sincos(psi, s, c);
     }
     static double one_minus_sq(double x) { return 1.0 - x * x; }
 };

@@ -9,7 +9,8 @@ from pathlib import Path
 import numpy as np
 
 ROOT = Path(__file__).resolve().parents[1]
-ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+# XMC_ORACLE_LIBRARY selects another build of the oracle (the sanitized one of the CPU test runs)
+ORACLE_SO = Path(os.environ.get("XMC_ORACLE_LIBRARY") or ROOT / "oracle" / "liboracle.so")
 REF_SO = ROOT / "oracle" / "_ref" / "libxmcml_ref.so"
 REF_TAPE_SO = ROOT / "oracle" / "_ref" / "libxmcml_ref_tape.so"
 

@@ -1,6 +1,7 @@
 """Builds the engine library (hipcc, gfx950) and the CPU oracle (g++).
 
-    python -m xmcml_amd.build           # library + oracle (+ reference build when present)
+    python -m xmcml_amd.build             # library + oracle (+ reference build when present)
+    python -m xmcml_amd.build --sanitize  # host-side ASan + UBSan builds for CPU test runs
 
 The engine is one shared object, ``xmcml_amd/libxmcml_amd.so``, exporting
 the C-ABI of include/xmcml.h.  Every HIP translation unit is compiled with
@@ -45,6 +46,24 @@ def build_library() -> Path:
     return LIB
 
 
+# Host-code sanitizers (AddressSanitizer + UndefinedBehaviorSanitizer) for CPU test runs: each
+# -fsanitize= goes to the host compilation only; the gfx950 code is built as usual.
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer",
+            "-Xarch_host", "-shared-libsan", "-g"]
+LIB_SAN = PKG / "libxmcml_amd_san.so"
+
+
+def build_sanitized() -> None:
+    """xmcml_amd/libxmcml_amd_san.so and oracle/liboracle_san.so with host-side
+    ASan + UBSan (tests/run_sanitized.sh runs the CPU suite on them)."""
+    tmp = LIB_SAN.with_suffix(".so.tmp")
+    cmd = [HIPCC, *FLAGS, *SAN_HOST, f"-I{ROOT / 'include'}", *[CSRC / s for s in SOURCES], "-o", tmp]
+    _run(cmd)
+    os.replace(tmp, LIB_SAN)
+    _run(["make", "-s", "-C", ROOT / "oracle", "liboracle_san.so"])
+
+
 def build_oracle() -> None:
     _run(["make", "-s", "-C", ROOT / "oracle", "liboracle.so"])
     if (REFERENCE / "base" / "xmcml" / "xmcml" / "mcml_kernel.cpp").exists():
@@ -52,6 +71,10 @@ def build_oracle() -> None:
 
 
 def main(argv=None) -> int:
+    argv = sys.argv[1:] if argv is None else argv
+    if "--sanitize" in argv:
+        build_sanitized()
+        return 0
     build_library()
     build_oracle()
     return 0

@@ -9,6 +9,7 @@ base/xmcml/xmcmlLauncher/launcher_omp.cpp:123).
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 from typing import Optional
 
@@ -16,7 +17,8 @@ import numpy as np
 
 from .abi import (TRACE_DTYPE, InputHandle, Options, OutputHandle, Rng, SceneSpec, Stats)
 
-LIB_PATH = Path(__file__).resolve().parent / "libxmcml_amd.so"
+# XMC_LIBRARY selects another build of the library (the sanitized one of the CPU test runs)
+LIB_PATH = Path(os.environ.get("XMC_LIBRARY") or Path(__file__).resolve().parent / "libxmcml_amd.so")
 
 _P = C.c_void_p
 _I = C.c_int
