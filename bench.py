@@ -236,6 +236,14 @@ def surrogate_line(model_path: Path, queries: int, mc_configs: int, mc_photons: 
         sur.predict_device(xq, out)
     engine.device_sync()
     t_inf = (time.perf_counter() - t0) / reps
+    # the same forward as one GEMM launch per layer (activations through HBM)
+    sur.mlp.forward(xq, out, per_layer=True)
+    engine.device_sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sur.mlp.forward(xq, out, per_layer=True)
+    engine.device_sync()
+    t_layers = (time.perf_counter() - t0) / reps
     # end to end through Surrogate.answer: `answer_queries` configurations of the batched
     # workload's distribution (one in 1000 pushed out of the sampling box), host arrays in,
     # answers out -- vectorised encoding, copies, inference, the uncertainty check, and plain
@@ -270,6 +278,8 @@ def surrogate_line(model_path: Path, queries: int, mc_configs: int, mc_photons: 
         "inference_ms": t_inf * 1e3,
         "inference_tflops": fl * queries / t_inf / 1e12,
         "inference_frac_of_bf16_peak": fl * queries / t_inf / 2.5e15,
+        "inference_kernel": "fused: whole network per 128-row tile, activations in LDS",
+        "inference_ms_per_layer_launches": t_layers * 1e3,
         "plain_mc_configs_per_s": mc_rate,
         "plain_mc_photons_per_config": mc_photons,
         "speedup_vs_plain_mc": inf_rate / mc_rate,

@@ -56,8 +56,15 @@ int xmc_mlp_get_params(xmc_mlp* m, float* params, int64_t count);
 int xmc_mlp_set_params(xmc_mlp* m, const float* params, int64_t count);
 
 /* out_dev[b][0..2*outputs) = model(x_dev[b][0..inputs)) for b < batch; any
- * batch size (processed in chunks of maxBatch). */
+ * batch size.  Models up to 512 wide (hidden and head) run in one launch that
+ * keeps each 128-row tile's activations on chip; wider ones, or an out_dev
+ * that is not 16-byte aligned, take xmc_mlp_forward_layers.  Both give the
+ * same bits. */
 int xmc_mlp_forward(xmc_mlp* m, const float* x_dev, int batch, float* out_dev, void* stream);
+
+/* The same forward as one matrix-core GEMM launch per layer, activations
+ * through device memory, in chunks of maxBatch rows. */
+int xmc_mlp_forward_layers(xmc_mlp* m, const float* x_dev, int batch, float* out_dev, void* stream);
 
 /* Forward, loss and backward of one batch (batch <= maxBatch) into the
  * gradient buffers (overwritten, not accumulated).  y_dev and mask_dev are

@@ -82,6 +82,32 @@ def test_forward_matches_reference(dev):
     assert np.array_equal(out2[400:], out[:100])
 
 
+@pytest.mark.parametrize("hidden,depth,outputs", [(128, 2, 32), (256, 3, 64), (512, 4, 96), (512, 1, 128)])
+def test_fused_forward_equals_per_layer_launches(dev, hidden, depth, outputs):
+    """The whole-network kernel (128-row tiles held in LDS) against one GEMM
+    launch per layer: same k order and epilogue arithmetic, so the same bits,
+    on batches that leave a ragged last tile, span several tiles, or exceed
+    maxBatch (the per-layer path's chunking)."""
+    from xmcml_amd.surrogate import MLP
+    rng = np.random.default_rng(hidden + depth)
+    m = MLP(30, hidden, depth, outputs, 384, seed=hidden)
+    p = m.get_params()
+    layers = R.unpack(p.copy(), R.layer_shapes(30, hidden, depth, outputs))
+    layers = [(W, 0.1 * rng.standard_normal(W.shape[0]).astype(np.float32)) for W, _ in layers]
+    p = R.pack(layers).astype(np.float32)
+    m.set_params(p)
+    for n in (1, 129, 1000):
+        x = rng.uniform(-1, 1, (n, 30)).astype(np.float32)
+        xt = dev(x.shape, x)
+        fused = m.forward(xt).numpy()
+        layered = m.forward(xt, per_layer=True).numpy()
+        print(hidden, depth, outputs, n, "max |fused - per-layer|", float(np.max(np.abs(fused - layered))))
+        assert np.array_equal(fused, layered), (n, float(np.max(np.abs(fused - layered))))
+        if n == 129:
+            ref, _ = R.RefMLP(p, 30, hidden, depth, outputs).forward(x)
+            assert np.linalg.norm(fused - ref) <= 5e-3 * np.linalg.norm(ref)
+
+
 def test_large_batch_forward_and_backward(dev):
     """Batches large enough for the 128 x 128 tile kernel (bias + GELU forward,
     GELU' backward with a transposed weight operand) against the reference."""

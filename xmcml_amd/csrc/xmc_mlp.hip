@@ -33,6 +33,7 @@ namespace xmlp {
 typedef __bf16 bf16x8 __attribute__((__vector_size__(8 * sizeof(__bf16))));
 typedef float f32x16 __attribute__((__vector_size__(16 * sizeof(float))));
 typedef uint32_t u32x4 __attribute__((__vector_size__(16)));
+typedef float f32x4 __attribute__((__vector_size__(16)));
 
 constexpr int BM = 64, BN = 64, BK = 64;
 constexpr int LDS_STRIDE = BK + 8;  // halfwords per LDS tile row (144 bytes: 16-byte aligned rows)
@@ -54,19 +55,35 @@ __host__ __device__ inline float bf2f(uint16_t h) {
     return f;
 }
 
-// tanh(y) = 1 - 2 / (1 + e^{2y}) with the hardware exponential and
-// reciprocal (v_exp_f32, v_rcp_f32, about 1 ulp each): the GELU epilogues
-// run once per output element, and exact libm calls there cost more VALU
-// time than the matrix products they follow
-__device__ inline float tanh_f(float y) {
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * y));
+// bf16 rounding on the device: v_cvt_pk_bf16_f32, round to nearest even,
+// two values per instruction (the same bits as f2bf for finite inputs)
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ inline uint32_t bf16x2_rne(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
 }
 
-__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + tanh_f(kGeluC * (z + kGeluA * z * z * z))); }
+__device__ inline uint16_t bf16_rne(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 
+// GELU, tanh form: 0.5 z (1 + tanh y) with y = sqrt(2/pi) (z + 0.044715 z^3),
+// evaluated as z / (1 + e^{-2y}) with e^{-2y} = 2^{z (g1 + g3 z^2)} on the
+// hardware exponential and reciprocal (v_exp_f32, v_rcp_f32, about 1 ulp
+// each).  The epilogues run once per output element: five VALU operations
+// and two transcendentals here, against a product of hidden-width length.
+constexpr float kGelu1 = -2.0f * kGeluC * 1.4426950408889634f;  // -2 sqrt(2/pi) log2(e)
+constexpr float kGelu3 = kGelu1 * kGeluA;
+
+__device__ inline float gelu_sigmoid(float z) {  // 1 / (1 + e^{-2y}) = 0.5 (1 + tanh y)
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(z * z, kGelu3, kGelu1)));
+}
+
+__device__ inline float gelu(float z) { return z * gelu_sigmoid(z); }
+
+// d/dz of z s(z), s = 0.5 (1 + tanh y): s + z s (1 - s) 2 y'(z)
 __device__ inline float gelu_grad(float z) {
-    float t = tanh_f(kGeluC * (z + kGeluA * z * z * z));
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluC * (1.0f + 3.0f * kGeluA * z * z);
+    float s = gelu_sigmoid(z);
+    return s + z * s * (1.0f - s) * (2.0f * kGeluC) * (1.0f + 3.0f * kGeluA * z * z);
 }
 
 enum Epi { EPI_GELU = 0, EPI_LINEAR = 1, EPI_DGELU = 2, EPI_PARTIAL = 3 };
@@ -135,7 +152,7 @@ __device__ __forceinline__ void epilogue(const GemmArgs& g, const f32x16& acc, i
         if constexpr (EPI == EPI_GELU) {
             float z = x + bias;
             g.zout[o] = f2bf(z);
-            g.out[o] = f2bf(gelu(z));
+            g.out[o] = bf16_rne(gelu(z));
         } else if constexpr (EPI == EPI_LINEAR) {
             g.outf[o] = x + bias;
         } else if constexpr (EPI == EPI_DGELU) {
@@ -298,7 +315,7 @@ __global__ void pack_input_kernel(const float* x, int batch, int nin, uint16_t* 
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)rows * ldx) return;
     int r = (int)(i / ldx), c = (int)(i % ldx);
-    a0[i] = (r < batch && c < nin) ? f2bf(x[(long)r * nin + c]) : (uint16_t)0;
+    a0[i] = (r < batch && c < nin) ? bf16_rne(x[(long)r * nin + c]) : (uint16_t)0;
 }
 
 // masked Gaussian NLL of the head output; writes the bf16 gradient of the
@@ -403,6 +420,176 @@ __global__ void to_bf16_kernel(const float* src, uint16_t* dst, long count) {
     if (i < count) dst[i] = f2bf(src[i]);
 }
 
+// ---------------------------------------------------------------------------
+// Fused inference: the whole network on one row tile per block.
+//
+// The tile's activations stay in LDS as bf16 [rows][HMAX + 8] for the whole
+// pass (input columns 0..63 first, each hidden layer overwriting them after a
+// barrier); nothing but the fp32 head rows goes back to HBM.  Weights are read
+// from L2 in the matrix-core fragment order they are packed into (one
+// coalesced 1 KB read per fragment): fragment (c, s) of a layer holds, for
+// lane l, W[32c + l%32][16s + 8(l/32) .. +7].
+//
+// Products are computed transposed, D[n][m] = sum_k W[n][k] X[m][k], so a
+// lane's accumulator holds four consecutive output columns of one row: the
+// epilogue writes 8-byte bf16 quads to LDS and 16-byte fp32 quads of the head.
+// The k order (ascending, 16 per instruction, from zero) and the epilogue
+// arithmetic are those of the per-layer GEMMs, so both give the same bits.
+//
+// A block holds R = kTileRows/32 row tiles and 2R waves.  A layer with
+// NT = fout/32 column tiles gives each wave NT R / (2R) = NT/2 accumulator
+// tiles: all R row tiles of NT/(2R) column tiles when 2R divides NT, else one
+// row tile (wave % R) of NT/2 column tiles.
+
+constexpr int kTileRows = 64;                 // rows per block: kTileRows / 32 row tiles
+constexpr int kInferWaves = kTileRows / 16;   // waves per block: each owns half a row tile's worth of work
+constexpr int kInferThreads = 64 * kInferWaves;
+constexpr int kRowTiles = kTileRows / 32;
+
+template <int HMAX>
+struct InferShape {
+    static constexpr int HS = HMAX + 8;  // halfwords per LDS activation row (16-byte aligned, rotates banks)
+};
+
+struct InferArgs {
+    const float* x;           // [batch][nin] fp32
+    int batch, nin;
+    const uint16_t* wp;       // packed bf16 weights, layer l at woff[l]
+    const float* w;           // fp32 parameters (biases at boff[l])
+    float* out;               // [batch][head] fp32
+    int layers, hidden, head;
+    long woff[9], boff[9];
+};
+
+// one layer's products into acc[CT][RT]
+template <int HS, int RT, int CT>
+__device__ __forceinline__ void infer_products(f32x16 (&acc)[CT][RT], const uint16_t* wl, int ks_count,
+                                               const uint16_t* act, int rt0, int ct0, int lane) {
+#pragma unroll
+    for (int j = 0; j < CT; j++)
+#pragma unroll
+        for (int i = 0; i < RT; i++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc[j][i][v] = 0.0f;
+    const bf16x8* wf = reinterpret_cast<const bf16x8*>(wl) + (long)ct0 * ks_count * 64 + lane;
+    const uint16_t* ap = act + (32 * rt0 + (lane & 31)) * HS + 8 * (lane >> 5);
+    // four k steps per trip: a weight fragment is requested three steps before
+    // its products (a ring of four register sets), an activation fragment one
+    // step before (a ring of two).  Requests past the last step re-read the
+    // last fragment (unused) rather than branch.  The scheduling barriers keep
+    // each step's requests ahead of its products, which the scheduler would
+    // otherwise sink to just before their use.
+    bf16x8 w[4][CT], a[2][RT];
+    const int last = ks_count - 1;
+#define XMC_WLOAD(S, K)                                                                   \
+    _Pragma("unroll") for (int j = 0; j < CT; j++) w[S][j] = wf[((long)j * ks_count + (K)) * 64];
+#define XMC_ALOAD(S, K)                                                                   \
+    _Pragma("unroll") for (int i = 0; i < RT; i++) a[S][i] =                              \
+        *reinterpret_cast<const bf16x8*>(ap + 32 * i * HS + (K) * 16);
+#define XMC_STEP(WS, AS)                                                                  \
+    __builtin_amdgcn_sched_barrier(0);                                                    \
+    _Pragma("unroll") for (int j = 0; j < CT; j++)                                        \
+    _Pragma("unroll") for (int i = 0; i < RT; i++)                                        \
+        acc[j][i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[WS][j], a[AS][i], acc[j][i], 0, 0, 0); \
+    __builtin_amdgcn_sched_barrier(0);
+    XMC_WLOAD(0, 0)
+    XMC_WLOAD(1, 1)
+    XMC_WLOAD(2, 2)
+    XMC_ALOAD(0, 0)
+    for (int ks = 0; ks < ks_count; ks += 4) {
+        XMC_WLOAD(3, ks + 3)
+        XMC_ALOAD(1, ks + 1)
+        XMC_STEP(0, 0)
+        XMC_WLOAD(0, min(ks + 4, last))
+        XMC_ALOAD(0, ks + 2)
+        XMC_STEP(1, 1)
+        XMC_WLOAD(1, min(ks + 5, last))
+        XMC_ALOAD(1, ks + 3)
+        XMC_STEP(2, 0)
+        XMC_WLOAD(2, min(ks + 6, last))
+        XMC_ALOAD(0, min(ks + 4, last))
+        XMC_STEP(3, 1)
+    }
+#undef XMC_WLOAD
+#undef XMC_ALOAD
+#undef XMC_STEP
+}
+
+// bias + GELU into the LDS activations (hidden layers) or bias into the
+// caller's fp32 rows (head); element 4g + e of a lane's accumulator is
+// column 8g + 4(lane/32) + e of its column tile, row lane % 32 of its row tile
+template <int HS, int RT, int CT, bool HEAD>
+__device__ __forceinline__ void infer_epilogue(const f32x16 (&acc)[CT][RT], const float* bias, uint16_t* act,
+                                               const InferArgs& g, int r0, int rt0, int ct0, int lane) {
+#pragma unroll
+    for (int j = 0; j < CT; j++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int n = 32 * (ct0 + j) + 8 * q + 4 * (lane >> 5);
+            const float4 b = *reinterpret_cast<const float4*>(bias + n);
+#pragma unroll
+            for (int i = 0; i < RT; i++) {
+                const int m = 32 * (rt0 + i) + (lane & 31);
+                float z0 = acc[j][i][4 * q] + b.x, z1 = acc[j][i][4 * q + 1] + b.y;
+                float z2 = acc[j][i][4 * q + 2] + b.z, z3 = acc[j][i][4 * q + 3] + b.w;
+                if constexpr (HEAD) {
+                    if (r0 + m < g.batch) {
+                        f32x4 o = {z0, z1, z2, z3};
+                        __builtin_nontemporal_store(o, reinterpret_cast<f32x4*>(g.out + (long)(r0 + m) * g.head + n));
+                    }
+                } else {
+                    uint2 p;
+                    p.x = bf16x2_rne(gelu(z0), gelu(z1));
+                    p.y = bf16x2_rne(gelu(z2), gelu(z3));
+                    *reinterpret_cast<uint2*>(act + m * HS + n) = p;
+                }
+            }
+        }
+}
+
+template <int HS, int RT, int CT, bool HEAD>
+__device__ __forceinline__ void infer_layer(const InferArgs& g, int l, int ks_count, uint16_t* act, int r0, int w,
+                                            int lane) {
+    // all row tiles of CT column tiles, or one row tile (w % kRowTiles) of CT
+    const int rt0 = RT == kRowTiles ? 0 : w % kRowTiles;
+    const int ct0 = RT == kRowTiles ? w * CT : (w / kRowTiles) * CT;
+    f32x16 acc[CT][RT];
+    infer_products<HS, RT, CT>(acc, g.wp + g.woff[l], ks_count, act, rt0, ct0, lane);
+    if constexpr (!HEAD) __syncthreads();  // every wave has read the layer's input before it is overwritten
+    infer_epilogue<HS, RT, CT, HEAD>(acc, g.w + g.boff[l], act, g, r0, rt0, ct0, lane);
+    if constexpr (!HEAD) __syncthreads();
+}
+
+// RTH x CTH: accumulator tiles per wave of the hidden layers, RTO x CTO of the head
+template <int HMAX, int RTH, int CTH, int RTO, int CTO>
+__global__ __launch_bounds__(kInferThreads) __attribute__((amdgpu_waves_per_eu(2))) void infer_kernel(InferArgs g) {
+    constexpr int HS = InferShape<HMAX>::HS;
+    __shared__ __attribute__((aligned(16))) uint16_t act[kTileRows * HS];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int r0 = blockIdx.x * kTileRows;
+    // input tile: columns 0..63, zero beyond nin and beyond the batch
+    for (int e = t; e < kTileRows * 64; e += kInferThreads) {
+        const int r = e >> 6, c = e & 63;
+        float v = (c < g.nin && r0 + r < g.batch) ? g.x[(long)(r0 + r) * g.nin + c] : 0.0f;
+        act[r * HS + c] = bf16_rne(v);
+    }
+    __syncthreads();
+    for (int l = 0; l < g.layers - 1; l++)
+        infer_layer<HS, RTH, CTH, false>(g, l, (l == 0 ? 64 : g.hidden) / 16, act, r0, w, lane);
+    infer_layer<HS, RTO, CTO, true>(g, g.layers - 1, g.hidden / 16, act, r0, w, lane);
+}
+
+// row-major bf16 weights [fout][fin] -> fragment order (see above)
+__global__ void pack_fragments_kernel(const uint16_t* src, uint16_t* dst, int fout, int fin) {
+    long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long)fout * fin) return;
+    const int ksn = fin / 16;
+    const int e = (int)(p & 7), lane = (int)((p >> 3) & 63);
+    const long f = p >> 9;  // fragment index c * ksn + s
+    const int c = (int)(f / ksn), s = (int)(f % ksn);
+    dst[p] = src[(long)(32 * c + (lane & 31)) * fin + 16 * s + 8 * (lane >> 5) + e];
+}
+
 }  // namespace xmlp
 
 using namespace xmlp;
@@ -458,6 +645,8 @@ struct xmc_mlp {
     float* m1 = nullptr;
     float* m2 = nullptr;
     uint16_t* wb = nullptr;  // bf16 mirror (same offsets as w; only weight blocks used)
+    uint16_t* wp = nullptr;  // the mirror's weight blocks in matrix-core fragment order (fused inference)
+    bool packed = false;     // wp matches wb
     uint16_t* a0 = nullptr;  // packed input [rows][in_pad]
     std::vector<uint16_t*> z, a;  // per hidden layer [rows][hidden]
     float* out = nullptr;         // [rows][head]
@@ -505,6 +694,70 @@ long user_count(const xmc_mlp* m) {
 int refresh_mirror(xmc_mlp* m, hipStream_t s) {
     long n = m->nparam;
     hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->w, m->wb, n);
+    MLP_TRY(hipGetLastError());
+    m->packed = false;
+    return 0;
+}
+
+// The fused kernel for this model's widths, or nullptr: [128][hidden + 8]
+// bf16 activations in LDS, each wave's accumulator tiles fixed at compile time
+// (a layer of fout columns gives each of the 8 waves fout/64 32 x 32 tiles)
+typedef void (*InferKernel)(InferArgs);
+
+// accumulator tiles per wave of a layer with nt column tiles
+constexpr int shape_rt(int nt) { return nt % kInferWaves == 0 ? kRowTiles : 1; }
+constexpr int shape_ct(int nt) { return nt % kInferWaves == 0 ? nt / kInferWaves : nt / 2; }
+
+template <int HM, int NTH, int NTO>
+InferKernel pick() {
+    return infer_kernel<HM, shape_rt(NTH), shape_ct(NTH), shape_rt(NTO), shape_ct(NTO)>;
+}
+
+InferKernel fused_kernel(const xmc_mlp* m) {
+    const int h = m->d.hidden, o = m->head;
+#define XMC_INFER(HM, NTH)                                                    \
+    switch (o) {                                                              \
+        case 64: return pick<HM, NTH, 2>();                                   \
+        case 128: return pick<HM, NTH, 4>();                                  \
+        case 192: return pick<HM, NTH, 6>();                                  \
+        case 256: return pick<HM, NTH, 8>();                                  \
+        default: return nullptr;                                              \
+    }
+    if (h == 128) XMC_INFER(256, 4)
+    if (h == 256) XMC_INFER(256, 8)
+    if (h == 512) XMC_INFER(512, 16)
+#undef XMC_INFER
+    return nullptr;
+}
+
+// whole-network forward of `batch` rows of x into out, one launch
+int forward_fused(xmc_mlp* m, const float* x, int batch, float* out, hipStream_t s) {
+    if (!m->packed) {
+        for (int l = 0; l < m->layers; l++) {
+            long n = (long)m->fout[l] * m->fin[l];
+            hipLaunchKernelGGL(pack_fragments_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                               m->wb + m->woff[l], m->wp + m->woff[l], m->fout[l], m->fin[l]);
+            MLP_TRY(hipGetLastError());
+        }
+        m->packed = true;
+    }
+    if (batch == 0) return 0;
+    InferArgs g{};
+    g.x = x;
+    g.batch = batch;
+    g.nin = m->d.inputs;
+    g.wp = m->wp;
+    g.w = m->w;
+    g.out = out;
+    g.layers = m->layers;
+    g.hidden = m->d.hidden;
+    g.head = m->head;
+    for (int l = 0; l < m->layers; l++) {
+        g.woff[l] = m->woff[l];
+        g.boff[l] = m->boff[l];
+    }
+    dim3 grid((unsigned)((batch + kTileRows - 1) / kTileRows));
+    hipLaunchKernelGGL(fused_kernel(m), grid, dim3(kInferThreads), 0, s, g);
     MLP_TRY(hipGetLastError());
     return 0;
 }
@@ -647,6 +900,7 @@ int adam_impl(xmc_mlp* m, float lr, float b1, float b2, float eps, float decay, 
     float c2 = 1.0f - std::pow(b2, (float)m->step);
     for (int l = 0; l < m->layers; l++) {
         long nw = (long)m->fout[l] * m->fin[l];
+        m->packed = false;
         hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, m->w + m->woff[l],
                            m->g + m->woff[l], m->m1 + m->woff[l], m->m2 + m->woff[l], m->wb + m->woff[l], nw, lr,
                            b1, b2, eps, c1, c2, decay);
@@ -721,6 +975,7 @@ int xmc_mlp_create(const xmc_mlp_desc* desc, uint64_t seed, xmc_mlp** out) {
     MLP_TRY(dalloc(m, (void**)&m->m1, pb));
     MLP_TRY(dalloc(m, (void**)&m->m2, pb));
     MLP_TRY(dalloc(m, (void**)&m->wb, off * sizeof(uint16_t)));
+    MLP_TRY(dalloc(m, (void**)&m->wp, off * sizeof(uint16_t)));
     MLP_TRY(dalloc(m, (void**)&m->a0, (size_t)m->rows * m->in_pad * 2));
     for (int l = 0; l < d.depth; l++) {
         uint16_t *zp, *ap;
@@ -809,7 +1064,7 @@ int xmc_mlp_set_params(xmc_mlp* m, const float* params, int64_t count) {
     return 0;
 }
 
-int xmc_mlp_forward(xmc_mlp* m, const float* x, int batch, float* out, void* stream) {
+int xmc_mlp_forward_layers(xmc_mlp* m, const float* x, int batch, float* out, void* stream) {
     if (!m || !x || !out) return fail(-1, "NULL argument");
     if (batch < 0) return fail(-1, "negative batch");
     MLP_TRY(hipSetDevice(m->d.device));
@@ -825,6 +1080,14 @@ int xmc_mlp_forward(xmc_mlp* m, const float* x, int batch, float* out, void* str
             MLP_TRY(hipMemcpyAsync(dst, m->out, (size_t)nb * m->head * 4, hipMemcpyDeviceToDevice, s));
     }
     return 0;
+}
+
+int xmc_mlp_forward(xmc_mlp* m, const float* x, int batch, float* out, void* stream) {
+    if (!m || !x || !out) return fail(-1, "NULL argument");
+    if (batch < 0) return fail(-1, "negative batch");
+    if (!fused_kernel(m) || ((uintptr_t)out & 15)) return xmc_mlp_forward_layers(m, x, batch, out, stream);
+    MLP_TRY(hipSetDevice(m->d.device));
+    return forward_fused(m, x, batch, out, (hipStream_t)stream);
 }
 
 int xmc_mlp_backward(xmc_mlp* m, const float* x, const float* y, const float* mask, int batch, float scale,

@@ -36,7 +36,7 @@ EXPORTS = [
     "xmc_free_output", "xmc_specular_reflectance", "xmc_device_name", "xmc_last_error", "xmc_version",
     # surrogate model (include/xmcml_surrogate.h)
     "xmc_mlp_create", "xmc_mlp_destroy", "xmc_mlp_param_count", "xmc_mlp_get_params", "xmc_mlp_set_params",
-    "xmc_mlp_forward", "xmc_mlp_backward", "xmc_mlp_get_grads", "xmc_mlp_adam", "xmc_mlp_train_step",
+    "xmc_mlp_forward", "xmc_mlp_forward_layers", "xmc_mlp_backward", "xmc_mlp_get_grads", "xmc_mlp_adam", "xmc_mlp_train_step",
     "xmc_mlp_gemm_check", "xmc_mlp_train_sampled", "xmc_dev_alloc", "xmc_dev_free", "xmc_dev_copy", "xmc_dev_sync",
     "xmc_weight_integrals", "xmc_weight_table_compute",
     # file formats and backups (include/xmcml_io.h)
@@ -101,6 +101,7 @@ def lib():
     sig("xmc_mlp_get_params", _I, [_P, _PF, C.c_int64])
     sig("xmc_mlp_set_params", _I, [_P, _PF, C.c_int64])
     sig("xmc_mlp_forward", _I, [_P, _P, _I, _P, _P])
+    sig("xmc_mlp_forward_layers", _I, [_P, _P, _I, _P, _P])
     sig("xmc_mlp_backward", _I, [_P, _P, _P, _P, _I, _F, _PF, _P])
     sig("xmc_mlp_get_grads", _I, [_P, _PF, C.c_int64])
     sig("xmc_mlp_adam", _I, [_P, _F, _F, _F, _F, _F, _P])

@@ -245,13 +245,15 @@ class MLP:
             raise ValueError(f"{name} must have shape (batch, {cols})")
         return a.ptr
 
-    def forward(self, x, out=None):
-        """Head output (n, 2*outputs) of device rows x; returns the device array."""
+    def forward(self, x, out=None, per_layer: bool = False):
+        """Head output (n, 2*outputs) of device rows x; returns the device array.
+        per_layer: one GEMM launch per layer instead of the fused whole-network
+        kernel (same bits)."""
         if out is None:
             out = engine.DeviceArray((x.shape[0], 2 * self.outputs), device=self.device)
-        engine._check(engine.lib().xmc_mlp_forward(self._h, self._dev(x, self.inputs, "x"), x.shape[0],
-                                                   self._dev(out, 2 * self.outputs, "out"), None),
-                      "xmc_mlp_forward")
+        name = "xmc_mlp_forward_layers" if per_layer else "xmc_mlp_forward"
+        engine._check(getattr(engine.lib(), name)(self._h, self._dev(x, self.inputs, "x"), x.shape[0],
+                                                  self._dev(out, 2 * self.outputs, "out"), None), name)
         return out
 
     def backward(self, x, y, mask, scale: float, want_loss: bool = True) -> Optional[float]:

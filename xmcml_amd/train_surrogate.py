@@ -75,6 +75,9 @@ def main(argv=None) -> int:
                     help="simulated data sets to train on (concatenated); the judgement set is the first one's")
     ap.add_argument("--data-out", type=Path)
     ap.add_argument("--data-only", action="store_true", help="simulate and write --data-out, do not train")
+    ap.add_argument("--eval-from", type=Path,
+                    help="take the held-out configurations and their reference MC (layers, Wref, Kref) from this "
+                         "model or data file instead of simulating them")
     ap.add_argument("--out", type=Path)
     ap.add_argument("--log-every", type=int, default=1000)
     args = ap.parse_args(argv)
@@ -107,6 +110,10 @@ def main(argv=None) -> int:
     else:
         train_cfg = S.draw_configs(args.train_configs, args.train_seed)
         eval_cfg = S.draw_configs(args.eval_configs, args.eval_seed)
+        given = None
+        if args.eval_from:
+            given = np.load(args.eval_from, allow_pickle=False)
+            eval_cfg = configs_from_array(given["eval_layers"], given["eval_count"])
         t0 = time.perf_counter()
         Y, M, _, _ = S.simulate(train_cfg, args.train_photons, seed=args.train_seed * 7919,
                                 progress=progress("training data"))
@@ -114,8 +121,11 @@ def main(argv=None) -> int:
         t0 = time.perf_counter()
         Kref = Wref = W_lo = np.zeros((0, S.N_OUTPUTS))
         if eval_cfg:
-            _, _, Kref, Wref = S.simulate(eval_cfg, args.eval_photons, seed=args.eval_seed * 7919, chunk=128,
-                                          progress=progress("held-out reference"))
+            if given is not None:
+                Kref, Wref = given["Kref"], given["Wref"]
+            else:
+                _, _, Kref, Wref = S.simulate(eval_cfg, args.eval_photons, seed=args.eval_seed * 7919, chunk=128,
+                                              progress=progress("held-out reference"))
             # plain MC at the training budget on the held-out configurations
             _, _, _, W_lo = S.simulate(eval_cfg, args.train_photons, seed=args.eval_seed * 104729)
         timings["eval_mc_s"] = time.perf_counter() - t0
