@@ -139,22 +139,31 @@ __device__ inline void store_pieces(uint16_t* lds, const u32x4 (&v)[2], int t) {
 // The accumulator tile of one wave (32 x 32, v_mfma_f32_32x32x16_bf16 layout:
 // element v of a lane is row 8*(v/4) + 4*(lane/32) + v%4, column lane%32)
 // through the epilogue, at rows m_base.., columns n_base..
+// Accumulators start at the bias (EPI_GELU, EPI_LINEAR: the bias is the first
+// term of every output's sum) or at zero.  Column n_base + lane % 32 in the
+// v_mfma_f32_32x32x16_bf16 layout.
+template <int EPI>
+__device__ __forceinline__ float acc_start(const GemmArgs& g, int n_base, int lane) {
+    if constexpr (EPI == EPI_GELU || EPI == EPI_LINEAR) return g.bias[n_base + (lane & 31)];
+    else return 0.0f;
+}
+
+// The accumulator tile of one wave (32 x 32, v_mfma_f32_32x32x16_bf16 layout:
+// element v of a lane is row 8*(v/4) + 4*(lane/32) + v%4, column lane%32)
+// through the epilogue, at rows m_base.., columns n_base..
 template <int EPI>
 __device__ __forceinline__ void epilogue(const GemmArgs& g, const f32x16& acc, int m_base, int n_base, int lane) {
     const int n = n_base + (lane & 31);
-    float bias = 0.0f;
-    if constexpr (EPI == EPI_GELU || EPI == EPI_LINEAR) bias = g.bias[n];
 #pragma unroll
     for (int v = 0; v < 16; v++) {
         const int m = m_base + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
         const long o = (long)m * g.ldo + n;
         float x = acc[v];
         if constexpr (EPI == EPI_GELU) {
-            float z = x + bias;
-            g.zout[o] = f2bf(z);
-            g.out[o] = bf16_rne(gelu(z));
+            g.zout[o] = f2bf(x);
+            g.out[o] = bf16_rne(gelu(x));
         } else if constexpr (EPI == EPI_LINEAR) {
-            g.outf[o] = x + bias;
+            g.outf[o] = x;
         } else if constexpr (EPI == EPI_DGELU) {
             g.out[o] = f2bf(x * gelu_grad(bf2f(g.zin[o])));
         } else {
@@ -174,8 +183,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
     const int lr = lane & 31, lk = (lane >> 5) * 8;
     f32x16 acc;
+    const float a0 = acc_start<EPI>(g, n0 + wn, lane);
 #pragma unroll
-    for (int v = 0; v < 16; v++) acc[v] = 0.0f;
+    for (int v = 0; v < 16; v++) acc[v] = a0;
 
     u32x4 ra[2], rb[2];
     load_pieces<AK>(ra, g.A, g.lda, m0, kb, t);
@@ -262,11 +272,13 @@ __global__ __launch_bounds__(256) void gemm128_kernel(GemmArgs g) {
     const int lr = lane & 31, lk = (lane >> 5) * 8;
     f32x16 acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2; j++) {
+        const float a0 = acc_start<EPI>(g, n0 + wn + 32 * j, lane);
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++)
 #pragma unroll
-            for (int v = 0; v < 16; v++) acc[i][j][v] = 0.0f;
+            for (int v = 0; v < 16; v++) acc[i][j][v] = a0;
+    }
 
     u32x4 ra[2], rb[2];
     load_pieces2<AK>(ra, g.A, g.lda, m0, kb, t);
@@ -433,23 +445,28 @@ __global__ void to_bf16_kernel(const float* src, uint16_t* dst, long count) {
 // Products are computed transposed, D[n][m] = sum_k W[n][k] X[m][k], so a
 // lane's accumulator holds four consecutive output columns of one row: the
 // epilogue writes 8-byte bf16 quads to LDS and 16-byte fp32 quads of the head.
-// The k order (ascending, 16 per instruction, from zero) and the epilogue
-// arithmetic are those of the per-layer GEMMs, so both give the same bits.
+// Accumulators start at the bias, the k order is ascending (16 per
+// instruction) and the epilogue arithmetic is that of the per-layer GEMMs, so
+// both give the same bits.
 //
 // A block holds R = kTileRows/32 row tiles and 2R waves.  A layer with
 // NT = fout/32 column tiles gives each wave NT R / (2R) = NT/2 accumulator
 // tiles: all R row tiles of NT/(2R) column tiles when 2R divides NT, else one
 // row tile (wave % R) of NT/2 column tiles.
+//
+// Overlap of the GELU epilogue with the matrix products: a hidden layer whose
+// waves own an even number of column tiles runs them in two passes.  The
+// second pass's k loop also finishes the first pass's accumulators (bias was
+// their start; GELU and bf16 rounding remain), a few elements per k step,
+// into packed registers, so that VALU work issues while the matrix core is
+// busy.  Only the second pass's epilogue runs after the layer's barrier, and
+// both halves are written to LDS then (the layer is computed in place: every
+// wave must have read its input before any output is stored).
 
 constexpr int kTileRows = 64;                 // rows per block: kTileRows / 32 row tiles
-constexpr int kInferWaves = kTileRows / 16;   // waves per block: each owns half a row tile's worth of work
+constexpr int kInferWaves = kTileRows / 16;
 constexpr int kInferThreads = 64 * kInferWaves;
 constexpr int kRowTiles = kTileRows / 32;
-
-template <int HMAX>
-struct InferShape {
-    static constexpr int HS = HMAX + 8;  // halfwords per LDS activation row (16-byte aligned, rotates banks)
-};
 
 struct InferArgs {
     const float* x;           // [batch][nin] fp32
@@ -457,115 +474,171 @@ struct InferArgs {
     const uint16_t* wp;       // packed bf16 weights, layer l at woff[l]
     const float* w;           // fp32 parameters (biases at boff[l])
     float* out;               // [batch][head] fp32
-    int layers, hidden, head;
+    int layers, head;
     long woff[9], boff[9];
 };
 
-// one layer's products into acc[CT][RT]
+// accumulator tiles acc[j][i] (column tile ct + j, any row tile) start at the bias
+template <int RT, int CT>
+__device__ __forceinline__ void infer_start(f32x16 (&acc)[CT][RT], const float* bias, int ct, int lane) {
+#pragma unroll
+    for (int j = 0; j < CT; j++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float4 b = *reinterpret_cast<const float4*>(bias + 32 * (ct + j) + 8 * q + 4 * (lane >> 5));
+#pragma unroll
+            for (int i = 0; i < RT; i++) {
+                acc[j][i][4 * q] = b.x;
+                acc[j][i][4 * q + 1] = b.y;
+                acc[j][i][4 * q + 2] = b.z;
+                acc[j][i][4 * q + 3] = b.w;
+            }
+        }
+}
+
+// GELU + bf16 of elements e, e + 1 (flattened over pend[j][i][v]) into held
+template <int RT, int CP>
+__device__ __forceinline__ void infer_finish_pair(const f32x16 (&pend)[CP][RT], uint32_t (&held)[CP * RT * 8], int e) {
+    const int p = e >> 4, v = e & 15;
+    held[e >> 1] = bf16x2_rne(gelu(pend[p / RT][p % RT][v]), gelu(pend[p / RT][p % RT][v + 1]));
+    // pins the value here: otherwise the compiler sinks the arithmetic to the
+    // value's only use, the LDS store after the layer's barrier
+    asm volatile("" : "+v"(held[e >> 1]));
+}
+
+// One pass of a layer's products over KS k steps: acc[j][i] += W(col tile
+// ct + j) X(row tile rt0 + i).  A weight fragment is requested three steps
+// before its products (a ring of four register sets), an activation fragment
+// one step before (a ring of two); the scheduling barriers keep each step's
+// requests ahead of its products.  CP > 0: also finishes pend into held.
+template <int HS, int RT, int CT, int KS, int CP, int RTP>
+__device__ __forceinline__ void infer_pass(f32x16 (&acc)[CT][RT], const uint16_t* wl, const uint16_t* act, int rt0,
+                                           int ct, int lane, const f32x16 (&pend)[CP > 0 ? CP : 1][RTP],
+                                           uint32_t (&held)[CP > 0 ? CP * RTP * 8 : 1]) {
+    constexpr int E = CP * RTP * 16 / KS;  // pending elements finished per k step
+    static_assert(CP == 0 || (E % 2 == 0 && E * KS == CP * RTP * 16), "pending elements must split into pairs per step");
+    // a compiler-level fence: this pass's first requests are not hoisted into
+    // the previous pass, where they would only occupy registers
+    asm volatile("" ::: "memory");
+    const bf16x8* wf = reinterpret_cast<const bf16x8*>(wl) + (long)ct * KS * 64 + lane;
+    const uint16_t* ap = act + (32 * rt0 + (lane & 31)) * HS + 8 * (lane >> 5);
+    bf16x8 w[4][CT], a[2][RT];
+#pragma unroll
+    for (int s = 0; s < 3 && s < KS; s++)
+#pragma unroll
+        for (int j = 0; j < CT; j++) w[s][j] = wf[(j * KS + s) * 64];
+#pragma unroll
+    for (int i = 0; i < RT; i++) a[0][i] = *reinterpret_cast<const bf16x8*>(ap + 32 * i * HS);
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+        if (s + 3 < KS) {
+#pragma unroll
+            for (int j = 0; j < CT; j++) w[(s + 3) & 3][j] = wf[(j * KS + s + 3) * 64];
+        }
+        if (s + 1 < KS) {
+#pragma unroll
+            for (int i = 0; i < RT; i++)
+                a[(s + 1) & 1][i] = *reinterpret_cast<const bf16x8*>(ap + 32 * i * HS + (s + 1) * 16);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < CT; j++)
+#pragma unroll
+            for (int i = 0; i < RT; i++)
+                acc[j][i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[s & 3][j], a[s & 1][i], acc[j][i], 0, 0, 0);
+        if constexpr (CP > 0) {
+#pragma unroll
+            for (int e = s * E; e < (s + 1) * E; e += 2) infer_finish_pair<RTP, CP>(pend, held, e);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// bf16 quads of held (pairs of acc[j][i][v]) into the LDS activations
 template <int HS, int RT, int CT>
-__device__ __forceinline__ void infer_products(f32x16 (&acc)[CT][RT], const uint16_t* wl, int ks_count,
-                                               const uint16_t* act, int rt0, int ct0, int lane) {
+__device__ __forceinline__ void infer_store_held(const uint32_t (&held)[CT * RT * 8], uint16_t* act, int rt0, int ct,
+                                                 int lane) {
 #pragma unroll
     for (int j = 0; j < CT; j++)
 #pragma unroll
         for (int i = 0; i < RT; i++)
 #pragma unroll
-            for (int v = 0; v < 16; v++) acc[j][i][v] = 0.0f;
-    const bf16x8* wf = reinterpret_cast<const bf16x8*>(wl) + (long)ct0 * ks_count * 64 + lane;
-    const uint16_t* ap = act + (32 * rt0 + (lane & 31)) * HS + 8 * (lane >> 5);
-    // four k steps per trip: a weight fragment is requested three steps before
-    // its products (a ring of four register sets), an activation fragment one
-    // step before (a ring of two).  Requests past the last step re-read the
-    // last fragment (unused) rather than branch.  The scheduling barriers keep
-    // each step's requests ahead of its products, which the scheduler would
-    // otherwise sink to just before their use.
-    bf16x8 w[4][CT], a[2][RT];
-    const int last = ks_count - 1;
-#define XMC_WLOAD(S, K)                                                                   \
-    _Pragma("unroll") for (int j = 0; j < CT; j++) w[S][j] = wf[((long)j * ks_count + (K)) * 64];
-#define XMC_ALOAD(S, K)                                                                   \
-    _Pragma("unroll") for (int i = 0; i < RT; i++) a[S][i] =                              \
-        *reinterpret_cast<const bf16x8*>(ap + 32 * i * HS + (K) * 16);
-#define XMC_STEP(WS, AS)                                                                  \
-    __builtin_amdgcn_sched_barrier(0);                                                    \
-    _Pragma("unroll") for (int j = 0; j < CT; j++)                                        \
-    _Pragma("unroll") for (int i = 0; i < RT; i++)                                        \
-        acc[j][i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w[WS][j], a[AS][i], acc[j][i], 0, 0, 0); \
-    __builtin_amdgcn_sched_barrier(0);
-    XMC_WLOAD(0, 0)
-    XMC_WLOAD(1, 1)
-    XMC_WLOAD(2, 2)
-    XMC_ALOAD(0, 0)
-    for (int ks = 0; ks < ks_count; ks += 4) {
-        XMC_WLOAD(3, ks + 3)
-        XMC_ALOAD(1, ks + 1)
-        XMC_STEP(0, 0)
-        XMC_WLOAD(0, min(ks + 4, last))
-        XMC_ALOAD(0, ks + 2)
-        XMC_STEP(1, 1)
-        XMC_WLOAD(1, min(ks + 5, last))
-        XMC_ALOAD(1, ks + 3)
-        XMC_STEP(2, 0)
-        XMC_WLOAD(2, min(ks + 6, last))
-        XMC_ALOAD(0, min(ks + 4, last))
-        XMC_STEP(3, 1)
-    }
-#undef XMC_WLOAD
-#undef XMC_ALOAD
-#undef XMC_STEP
+            for (int q = 0; q < 4; q++) {
+                const int n = 32 * (ct + j) + 8 * q + 4 * (lane >> 5);
+                const int m = 32 * (rt0 + i) + (lane & 31);
+                const int h = (j * RT + i) * 8 + 2 * q;
+                *reinterpret_cast<uint2*>(act + m * HS + n) = make_uint2(held[h], held[h + 1]);
+            }
 }
 
-// bias + GELU into the LDS activations (hidden layers) or bias into the
-// caller's fp32 rows (head); element 4g + e of a lane's accumulator is
-// column 8g + 4(lane/32) + e of its column tile, row lane % 32 of its row tile
-template <int HS, int RT, int CT, bool HEAD>
-__device__ __forceinline__ void infer_epilogue(const f32x16 (&acc)[CT][RT], const float* bias, uint16_t* act,
-                                               const InferArgs& g, int r0, int rt0, int ct0, int lane) {
+// fp32 head rows (accumulators already hold bias + products)
+template <int RT, int CT>
+__device__ __forceinline__ void infer_store_head(const f32x16 (&acc)[CT][RT], const InferArgs& g, int r0, int rt0,
+                                                 int ct, int lane) {
 #pragma unroll
     for (int j = 0; j < CT; j++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int n = 32 * (ct0 + j) + 8 * q + 4 * (lane >> 5);
-            const float4 b = *reinterpret_cast<const float4*>(bias + n);
+        for (int i = 0; i < RT; i++) {
+            const int m = 32 * (rt0 + i) + (lane & 31);
+            if (r0 + m < g.batch) {
 #pragma unroll
-            for (int i = 0; i < RT; i++) {
-                const int m = 32 * (rt0 + i) + (lane & 31);
-                float z0 = acc[j][i][4 * q] + b.x, z1 = acc[j][i][4 * q + 1] + b.y;
-                float z2 = acc[j][i][4 * q + 2] + b.z, z3 = acc[j][i][4 * q + 3] + b.w;
-                if constexpr (HEAD) {
-                    if (r0 + m < g.batch) {
-                        f32x4 o = {z0, z1, z2, z3};
-                        __builtin_nontemporal_store(o, reinterpret_cast<f32x4*>(g.out + (long)(r0 + m) * g.head + n));
-                    }
-                } else {
-                    uint2 p;
-                    p.x = bf16x2_rne(gelu(z0), gelu(z1));
-                    p.y = bf16x2_rne(gelu(z2), gelu(z3));
-                    *reinterpret_cast<uint2*>(act + m * HS + n) = p;
+                for (int q = 0; q < 4; q++) {
+                    const int n = 32 * (ct + j) + 8 * q + 4 * (lane >> 5);
+                    f32x4 o = {acc[j][i][4 * q], acc[j][i][4 * q + 1], acc[j][i][4 * q + 2], acc[j][i][4 * q + 3]};
+                    __builtin_nontemporal_store(o, reinterpret_cast<f32x4*>(g.out + (long)(r0 + m) * g.head + n));
                 }
             }
         }
 }
 
-template <int HS, int RT, int CT, bool HEAD>
-__device__ __forceinline__ void infer_layer(const InferArgs& g, int l, int ks_count, uint16_t* act, int r0, int w,
-                                            int lane) {
-    // all row tiles of CT column tiles, or one row tile (w % kRowTiles) of CT
+// A hidden layer (fin = 16 KS, GELU into LDS) or the head (fp32 rows out)
+template <int HS, int RT, int CT, int KS, bool HEAD>
+__device__ __forceinline__ void infer_layer(const InferArgs& g, int l, uint16_t* act, int r0, int w, int lane) {
     const int rt0 = RT == kRowTiles ? 0 : w % kRowTiles;
     const int ct0 = RT == kRowTiles ? w * CT : (w / kRowTiles) * CT;
-    f32x16 acc[CT][RT];
-    infer_products<HS, RT, CT>(acc, g.wp + g.woff[l], ks_count, act, rt0, ct0, lane);
-    if constexpr (!HEAD) __syncthreads();  // every wave has read the layer's input before it is overwritten
-    infer_epilogue<HS, RT, CT, HEAD>(acc, g.w + g.boff[l], act, g, r0, rt0, ct0, lane);
-    if constexpr (!HEAD) __syncthreads();
+    const uint16_t* wl = g.wp + g.woff[l];
+    const float* bias = g.w + g.boff[l];
+    f32x16 none[1][RT];
+    uint32_t none_held[1];
+    if constexpr (HEAD || CT % 2) {
+        f32x16 acc[CT][RT];
+        infer_start<RT, CT>(acc, bias, ct0, lane);
+        infer_pass<HS, RT, CT, KS, 0, RT>(acc, wl, act, rt0, ct0, lane, none, none_held);
+        if constexpr (HEAD) {
+            infer_store_head<RT, CT>(acc, g, r0, rt0, ct0, lane);
+        } else {
+            uint32_t held[CT * RT * 8];
+#pragma unroll
+            for (int e = 0; e < CT * RT * 16; e += 2) infer_finish_pair<RT, CT>(acc, held, e);
+            __syncthreads();  // every wave has read the layer's input before it is overwritten
+            infer_store_held<HS, RT, CT>(held, act, rt0, ct0, lane);
+            __syncthreads();
+        }
+    } else {
+        constexpr int CH = CT / 2;
+        f32x16 first[CH][RT], second[CH][RT];
+        uint32_t held1[CH * RT * 8], held2[CH * RT * 8];
+        infer_start<RT, CH>(first, bias, ct0, lane);
+        infer_pass<HS, RT, CH, KS, 0, RT>(first, wl, act, rt0, ct0, lane, none, none_held);
+        infer_start<RT, CH>(second, bias, ct0 + CH, lane);
+        infer_pass<HS, RT, CH, KS, CH, RT>(second, wl, act, rt0, ct0 + CH, lane, first, held1);
+#pragma unroll
+        for (int e = 0; e < CH * RT * 16; e += 2) infer_finish_pair<RT, CH>(second, held2, e);
+        __syncthreads();
+        infer_store_held<HS, RT, CH>(held1, act, rt0, ct0, lane);
+        infer_store_held<HS, RT, CH>(held2, act, rt0, ct0 + CH, lane);
+        __syncthreads();
+    }
 }
 
-// RTH x CTH: accumulator tiles per wave of the hidden layers, RTO x CTO of the head
-template <int HMAX, int RTH, int CTH, int RTO, int CTO>
+// H: hidden width; RTH x CTH accumulator tiles per wave in the hidden layers,
+// RTO x CTO in the head
+template <int H, int RTH, int CTH, int RTO, int CTO>
 __global__ __launch_bounds__(kInferThreads) __attribute__((amdgpu_waves_per_eu(2))) void infer_kernel(InferArgs g) {
-    constexpr int HS = InferShape<HMAX>::HS;
+    constexpr int HS = H + 8;  // halfwords per LDS row: 16-byte aligned, consecutive rows on rotating banks
     __shared__ __attribute__((aligned(16))) uint16_t act[kTileRows * HS];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    // the wave index as a scalar: weight addresses are then a scalar base plus the lane's 16 bytes
+    const int t = threadIdx.x, lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int r0 = blockIdx.x * kTileRows;
     // input tile: columns 0..63, zero beyond nin and beyond the batch
     for (int e = t; e < kTileRows * 64; e += kInferThreads) {
@@ -574,9 +647,9 @@ __global__ __launch_bounds__(kInferThreads) __attribute__((amdgpu_waves_per_eu(2
         act[r * HS + c] = bf16_rne(v);
     }
     __syncthreads();
-    for (int l = 0; l < g.layers - 1; l++)
-        infer_layer<HS, RTH, CTH, false>(g, l, (l == 0 ? 64 : g.hidden) / 16, act, r0, w, lane);
-    infer_layer<HS, RTO, CTO, true>(g, g.layers - 1, g.hidden / 16, act, r0, w, lane);
+    infer_layer<HS, RTH, CTH, 4, false>(g, 0, act, r0, w, lane);
+    for (int l = 1; l < g.layers - 1; l++) infer_layer<HS, RTH, CTH, H / 16, false>(g, l, act, r0, w, lane);
+    infer_layer<HS, RTO, CTO, H / 16, true>(g, g.layers - 1, act, r0, w, lane);
 }
 
 // row-major bf16 weights [fout][fin] -> fragment order (see above)
@@ -699,33 +772,32 @@ int refresh_mirror(xmc_mlp* m, hipStream_t s) {
     return 0;
 }
 
-// The fused kernel for this model's widths, or nullptr: [128][hidden + 8]
+// The fused kernel for this model's widths, or nullptr: [kTileRows][hidden + 8]
 // bf16 activations in LDS, each wave's accumulator tiles fixed at compile time
-// (a layer of fout columns gives each of the 8 waves fout/64 32 x 32 tiles)
 typedef void (*InferKernel)(InferArgs);
 
 // accumulator tiles per wave of a layer with nt column tiles
 constexpr int shape_rt(int nt) { return nt % kInferWaves == 0 ? kRowTiles : 1; }
 constexpr int shape_ct(int nt) { return nt % kInferWaves == 0 ? nt / kInferWaves : nt / 2; }
 
-template <int HM, int NTH, int NTO>
+template <int H, int NTO>
 InferKernel pick() {
-    return infer_kernel<HM, shape_rt(NTH), shape_ct(NTH), shape_rt(NTO), shape_ct(NTO)>;
+    return infer_kernel<H, shape_rt(H / 32), shape_ct(H / 32), shape_rt(NTO), shape_ct(NTO)>;
 }
 
 InferKernel fused_kernel(const xmc_mlp* m) {
     const int h = m->d.hidden, o = m->head;
-#define XMC_INFER(HM, NTH)                                                    \
+#define XMC_INFER(H)                                                          \
     switch (o) {                                                              \
-        case 64: return pick<HM, NTH, 2>();                                   \
-        case 128: return pick<HM, NTH, 4>();                                  \
-        case 192: return pick<HM, NTH, 6>();                                  \
-        case 256: return pick<HM, NTH, 8>();                                  \
+        case 64: return pick<H, 2>();                                         \
+        case 128: return pick<H, 4>();                                        \
+        case 192: return pick<H, 6>();                                        \
+        case 256: return pick<H, 8>();                                        \
         default: return nullptr;                                              \
     }
-    if (h == 128) XMC_INFER(256, 4)
-    if (h == 256) XMC_INFER(256, 8)
-    if (h == 512) XMC_INFER(512, 16)
+    if (h == 128) XMC_INFER(128)
+    if (h == 256) XMC_INFER(256)
+    if (h == 512) XMC_INFER(512)
 #undef XMC_INFER
     return nullptr;
 }
@@ -750,7 +822,6 @@ int forward_fused(xmc_mlp* m, const float* x, int batch, float* out, hipStream_t
     g.w = m->w;
     g.out = out;
     g.layers = m->layers;
-    g.hidden = m->d.hidden;
     g.head = m->head;
     for (int l = 0; l < m->layers; l++) {
         g.woff[l] = m->woff[l];
