@@ -327,6 +327,8 @@ def main(argv=None) -> int:
     ap.add_argument("--batch-scale", type=float, default=1.0, help="batch workload: photon budgets times this")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--no-slab-grid", action="store_true",
+                    help="build no separating slabs near the mesh (A/B of the walk skipping; same results)")
     ap.add_argument("--seed", type=int, default=777)
     ap.add_argument("--surrogate-model", type=Path, default=ROOT / "xmcml_amd" / "data" / "surrogate_layered.npz")
     ap.add_argument("--surrogate-queries", type=int, default=1 << 20)
@@ -359,7 +361,7 @@ def main(argv=None) -> int:
     absorption = args.workload == "slab"
     spec = make(args.photons)
     print(f"bench: building scene ({spec.num_triangles} triangles)", file=sys.stderr, flush=True)
-    sc = engine.Scene(spec, absorption=absorption, block_size=args.block)
+    sc = engine.Scene(spec, absorption=absorption, block_size=args.block, slab_grid=not args.no_slab_grid)
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib << 20)
         print(f"bench: calibration copy of {args.copy_calibration_mib} MiB done", file=sys.stderr, flush=True)

@@ -259,6 +259,8 @@ typedef struct xmc_options {
     int noEntryGrid;     /* 1 = step-limited walks start at the BVH root, not on a cell's entry tree (the entry
                           * trees of a 1.6M-triangle mesh take ~30 s of host time and ~5 GB of device
                           * memory to build; results are the same either way) */
+    int noSlabGrid;      /* 1 = no separating slabs near the mesh (steps that cannot cross the nearby surfaces
+                          * skip their walk; results are the same either way) */
 } xmc_options;
 
 typedef struct xmc_scene xmc_scene;
@@ -351,6 +353,17 @@ int xmc_batch_get_launch(xmc_batch* batch, int* blockSize, int* blocks);
  * sizes. */
 int xmc_distance_grid(const xmc_input* input, uint64_t targetCells, float* lo, float* h, float* stepQ,
                       int* dims, uint8_t* q);
+
+/* The separating slabs the scene would build on that grid for steps of mean
+ * length meanStep (host only; for tests).  words[(z*dims[1]+y)*dims[0]+x]
+ * packs, low bits first: a direction n as two int8 octahedral coordinates
+ * (u, v) / 127 -> (u, v, 1 - |u| - |v|), folded where that is negative; the
+ * bands a1 <= b1 <= a2 <= b2 as int8 multiples of R / 127; R as a uint8
+ * multiple of *unit.  Every triangle point within R of the cell centre c
+ * has n.(x - c) in [a1, b1] or [a2, b2]; 0 = no slab.  words may be NULL to
+ * query the sizes. */
+int xmc_slab_grid(const xmc_input* input, uint64_t targetCells, float meanStep, float* lo, float* h, float* unit,
+                  int* dims, uint64_t* words);
 
 /* --- Weight integrals of the biasing (FP64, on the GPU). --- */
 /* ComputeWeightIntegral (mcml_integration.cpp:7-82) at numberOfDots values of

@@ -211,6 +211,7 @@ class Options(C.Structure):
         ("trajectories", C.c_int),
         ("trajectoryCapacity", C.c_int),
         ("noEntryGrid", C.c_int),
+        ("noSlabGrid", C.c_int),
     ]
 
 

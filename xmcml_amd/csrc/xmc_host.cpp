@@ -128,6 +128,7 @@ struct xmc_scene {
     DevBuf<int> perm_dev;
     DevBuf<uint32_t> dist;  // walk-skipping grid cells
     DevBuf<uint32_t> dist2;  // second-level entries (steps of up to 3 cells)
+    DevBuf<uint64_t> slabs;  // separating slabs of the cells near the mesh
     DevBuf<int> entry_ref;
     DevBuf<float> entry_box;
     // voxel-plane crossings: scattering grid, detector trajectory counts and
@@ -454,6 +455,24 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
         P.dist.entry_box = sc->entry_box.p;
         P.dist.cells2 = want_entry ? sc->dist2.p : nullptr;
         P.dist.margin2 = want_entry ? eg.margin2 : 0.0f;
+        if (want_dist && o.noSlabGrid == 0) {
+            // slabs chosen for steps of the scene's typical length: the median mean free path
+            // of its scattering layers
+            std::vector<double> mfp;
+            for (int l = 1; l < in->numberOfLayers; l++) {
+                double mut = in->layerInfo[l].absorptionCoefficient + in->layerInfo[l].scatteringCoefficient;
+                if (mut > 0.0) mfp.push_back(1.0 / mut);
+            }
+            std::sort(mfp.begin(), mfp.end());
+            double ms = mfp.empty() ? (double)dg.h : mfp[mfp.size() / 2];
+            ms = std::clamp(ms, 0.25 * (double)dg.h, 8.0 * (double)dg.h);
+            xmc::SlabGrid sg;
+            xmc::build_slab_grid(in->surface, in->numberOfSurfaces, dg, (float)ms, sg);
+            HIP_TRY(sc->slabs.alloc(sg.words.size()));
+            HIP_TRY(hipMemcpy(sc->slabs.p, sg.words.data(), sg.words.size() * 8, hipMemcpyHostToDevice));
+            P.dist.slabs = reinterpret_cast<const uint2*>(sc->slabs.p);
+            P.dist.slab_unit = sg.unit;
+        }
     }
 
     // layer boundary table: FindIntersectionLayer (mcml_kernel.cpp:566-580)
@@ -1043,9 +1062,8 @@ int xmc_launch(const xmc_input* in, xmc_output* out, xmc_rng* rng, const xmc_opt
     return rc;
 }
 
-int xmc_distance_grid(const xmc_input* in, uint64_t target_cells, float* lo, float* h, float* step_q, int* dims,
-                      uint8_t* q) {
-    if (!in || !lo || !h || !step_q || !dims) return fail(XMC_ERR_ARG, "NULL argument");
+// the distance grid a scene of `in` builds (over the BVH root's box, padded)
+static int scene_distance_grid(const xmc_input* in, uint64_t target_cells, xmc::DistanceGrid& dg) {
     xmc::RefBvh ref;
     const char* err = nullptr;
     if (!xmc::build_reference_bvh(in->surface, in->numberOfSurfaces, ref, &err))
@@ -1058,8 +1076,16 @@ int xmc_distance_grid(const xmc_input* in, uint64_t target_cells, float* lo, flo
         glo[a] = gb.root_lo[a] - pad;
         ghi[a] = gb.root_hi[a] + pad;
     }
-    xmc::DistanceGrid dg;
     xmc::build_distance_grid(in->surface, in->numberOfSurfaces, glo, ghi, target_cells ? target_cells : (16u << 20), dg);
+    return XMC_OK;
+}
+
+int xmc_distance_grid(const xmc_input* in, uint64_t target_cells, float* lo, float* h, float* step_q, int* dims,
+                      uint8_t* q) {
+    if (!in || !lo || !h || !step_q || !dims) return fail(XMC_ERR_ARG, "NULL argument");
+    xmc::DistanceGrid dg;
+    int rc = scene_distance_grid(in, target_cells, dg);
+    if (rc != XMC_OK) return rc;
     for (int a = 0; a < 3; a++) {
         lo[a] = dg.lo[a];
         dims[a] = dg.dims[a];
@@ -1067,6 +1093,29 @@ int xmc_distance_grid(const xmc_input* in, uint64_t target_cells, float* lo, flo
     *h = dg.h;
     *step_q = dg.step_q;
     if (q) std::memcpy(q, dg.q.data(), dg.q.size());
+    return XMC_OK;
+}
+
+int xmc_slab_grid(const xmc_input* in, uint64_t target_cells, float mean_step, float* lo, float* h, float* unit,
+                  int* dims, uint64_t* words) {
+    if (!in || !lo || !h || !unit || !dims) return fail(XMC_ERR_ARG, "NULL argument");
+    if (!(mean_step > 0.0f)) return fail(XMC_ERR_ARG, "meanStep must be > 0");
+    xmc::DistanceGrid dg;
+    int rc = scene_distance_grid(in, target_cells, dg);
+    if (rc != XMC_OK) return rc;
+    for (int a = 0; a < 3; a++) {
+        lo[a] = dg.lo[a];
+        dims[a] = dg.dims[a];
+    }
+    *h = dg.h;
+    if (!words) {
+        *unit = dg.h * 0.125f;
+        return XMC_OK;
+    }
+    xmc::SlabGrid sg;
+    xmc::build_slab_grid(in->surface, in->numberOfSurfaces, dg, mean_step, sg);
+    *unit = sg.unit;
+    std::memcpy(words, sg.words.data(), sg.words.size() * 8);
     return XMC_OK;
 }
 

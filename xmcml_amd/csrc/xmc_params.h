@@ -111,6 +111,10 @@ struct DistGrid {
     const float* entry_box;
     const uint32_t* cells2;  // null without entries
     float margin2;
+    // per cell a separating slab near the mesh (64-bit words, layout of
+    // SlabGrid in xmc_distance.h): radius unit h / 8; null = none
+    const uint2* slabs;
+    float slab_unit;
 };
 
 // A box of absorption voxels (around the beam entry, where most weight is

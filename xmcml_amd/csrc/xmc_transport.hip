@@ -755,6 +755,48 @@ __device__ __forceinline__ int walk_start(const DistGrid& G, V3 p, V3 d, float s
     return (int)e - 1;
 }
 
+// The separating slab of p's cell (see SlabGrid in xmc_distance.h): true
+// when the step from p along d over `step` stays within the cell's ball of
+// radius R about its centre c and both of its ends lie in one free interval
+// of n.(x - c), each by the margin of walk_start's distance bound -- the step
+// then crosses no triangle and its walk would find nothing.
+__device__ __forceinline__ int sbyte(uint32_t w, int k) { return (int)(w << (24 - 8 * k)) >> 24; }
+
+__device__ __forceinline__ bool slab_skips(const DistGrid& G, V3 p, V3 d, float step) {
+    float fx = (p.x - G.lo[0]) * G.inv_h;
+    float fy = (p.y - G.lo[1]) * G.inv_h;
+    float fz = (p.z - G.lo[2]) * G.inv_h;
+    if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f)) return false;
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    if (ix >= G.dims[0] || iy >= G.dims[1] || iz >= G.dims[2]) return false;
+    const uint2 w = G.slabs[((size_t)iz * G.dims[1] + iy) * G.dims[0] + ix];
+    float ox = (fx - ((float)ix + 0.5f)) * G.h;
+    float oy = (fy - ((float)iy + 0.5f)) * G.h;
+    float oz = (fz - ((float)iz + 0.5f)) * G.h;
+    float off = __builtin_sqrtf(ox * ox + oy * oy + oz * oz);
+    float m = 1.0e-4f + 16.0f * offset_scale(p);
+    float R = (float)((w.y >> 16) & 0xFFu) * G.slab_unit;
+    float room = R - m - off;
+    bool in_ball = room > 0.0f && step * step * dot(d, d) < room * room;
+    // octahedral direction (slab_direction in xmc_distance.h)
+    float u = (float)sbyte(w.x, 0) * (1.0f / 127.0f), v = (float)sbyte(w.x, 1) * (1.0f / 127.0f);
+    float nz = 1.0f - fabsf(u) - fabsf(v);
+    if (nz < 0.0f) {
+        float fu = (1.0f - fabsf(v)) * (u >= 0.0f ? 1.0f : -1.0f);
+        float fv = (1.0f - fabsf(u)) * (v >= 0.0f ? 1.0f : -1.0f);
+        u = fu;
+        v = fv;
+    }
+    float t0 = u * ox + v * oy + nz * oz;
+    float t1 = t0 + step * (u * d.x + v * d.y + nz * d.z);
+    float tmin = fminf(t0, t1), tmax = fmaxf(t0, t1);
+    float qf = R * (1.0f / 127.0f);
+    bool free = tmax < (float)sbyte(w.x, 2) * qf - m ||
+                (tmin > (float)sbyte(w.x, 3) * qf + m && tmax < (float)sbyte(w.y, 0) * qf - m) ||
+                tmin > (float)sbyte(w.y, 1) * qf + m;
+    return in_ball && free;
+}
+
 template <bool TRACE>
 struct Recorder {
     xmc_trace_event* ev;
@@ -1038,11 +1080,21 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                     n_entry++;
                 }
             }
+            // a step that cannot cross the surfaces near its cell skips its walk
+            bool skip = false;
+            if constexpr (INTERLEAVE) {
+                if (C.dist.slabs != nullptr) skip = slab_skips(C.dist, pos, dir, step);
+            }
             Ray ray = make_ray(pos, dir);
             if constexpr (INTERLEAVE) {
                 rinv = ray.inv;
                 raxes = (ray.ax ? 1u : 0u) | (ray.ay ? 2u : 0u) | (ray.az ? 4u : 0u);
-                phase = walk_begin(geo, ray, step, true, st, h, cur, best) ? kWalk : kResolve;
+                if (skip) {
+                    n_culled++;
+                    phase = kResolve;
+                } else {
+                    phase = walk_begin(geo, ray, step, true, st, h, cur, best) ? kWalk : kResolve;
+                }
             } else {
                 h = walk(geo, ray, step, true, st);
                 phase = kResolve;

@@ -32,7 +32,7 @@ EXPORTS = [
     "xmc_scene_create", "xmc_scene_destroy", "xmc_scene_run", "xmc_scene_sync", "xmc_scene_collect", "xmc_scene_stats",
     "xmc_scene_trace", "xmc_scene_query", "xmc_scene_query_device", "xmc_scene_bvh", "xmc_scene_set_launch", "xmc_scene_get_launch",
     "xmc_batch_create", "xmc_batch_destroy", "xmc_batch_run", "xmc_batch_collect", "xmc_batch_stats",
-    "xmc_batch_set_launch", "xmc_batch_get_launch", "xmc_distance_grid", "xmc_launch", "xmc_init_output",
+    "xmc_batch_set_launch", "xmc_batch_get_launch", "xmc_distance_grid", "xmc_slab_grid", "xmc_launch", "xmc_init_output",
     "xmc_free_output", "xmc_specular_reflectance", "xmc_device_name", "xmc_last_error", "xmc_version",
     # surrogate model (include/xmcml_surrogate.h)
     "xmc_mlp_create", "xmc_mlp_destroy", "xmc_mlp_param_count", "xmc_mlp_get_params", "xmc_mlp_set_params",
@@ -87,6 +87,8 @@ def lib():
     sig("xmc_batch_get_launch", _I, [_P, _PI, _PI])
     sig("xmc_distance_grid", _I, [_P, _U64, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                   _PI, _P])
+    sig("xmc_slab_grid", _I, [_P, _U64, C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                              _PI, _P])
     sig("xmc_launch", _I, [_P, _P, _P, _P])
     sig("xmc_init_output", _I, [_P, _P])
     sig("xmc_free_output", None, [_P])
@@ -153,11 +155,12 @@ class Scene:
     def __init__(self, spec: SceneSpec, block_size: int = 0, blocks_per_cu: int = 0,
                  absorption: bool = True, device: int = 0, distance_grid: bool = True,
                  scattering: bool = False, trajectories: bool = False, trajectory_capacity: int = 0,
-                 entry_grid: bool = True):
+                 entry_grid: bool = True, slab_grid: bool = True):
         self.spec = spec
         self.input = InputHandle(spec)
         opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1, 0 if distance_grid else 1,
-                      int(scattering), int(trajectories), trajectory_capacity, 0 if entry_grid else 1)
+                      int(scattering), int(trajectories), trajectory_capacity, 0 if entry_grid else 1,
+                      0 if slab_grid else 1)
         self.scattering, self.trajectories = scattering, trajectories
         h = _P()
         _check(lib().xmc_scene_create(self.input.ptr, C.byref(opt), C.byref(h)), "xmc_scene_create")
@@ -308,6 +311,22 @@ def distance_grid(spec: SceneSpec, target_cells: int = 0) -> dict:
            "xmc_distance_grid")
     return dict(lo=np.array(lo[:], np.float32), h=h.value, step_q=q.value, dims=tuple(dims[:]),
                 q=grid.reshape(dims[2], dims[1], dims[0]))
+
+
+def slab_grid(spec: SceneSpec, mean_step: float, target_cells: int = 0) -> dict:
+    """The separating slabs a scene builds near its mesh (host computation):
+    per cell the packed 64-bit word of xmc_slab_grid."""
+    inp = InputHandle(spec)
+    lo = (C.c_float * 3)()
+    h, unit = C.c_float(), C.c_float()
+    dims = (C.c_int * 3)()
+    _check(lib().xmc_slab_grid(inp.ptr, target_cells, mean_step, lo, C.byref(h), C.byref(unit), dims, None),
+           "xmc_slab_grid")
+    words = np.zeros(dims[0] * dims[1] * dims[2], np.uint64)
+    _check(lib().xmc_slab_grid(inp.ptr, target_cells, mean_step, lo, C.byref(h), C.byref(unit), dims,
+                               words.ctypes.data), "xmc_slab_grid")
+    return dict(lo=np.array(lo[:], np.float32), h=h.value, unit=unit.value, dims=tuple(dims[:]),
+                words=words.reshape(dims[2], dims[1], dims[0]))
 
 
 def weight_integrals(dots, anisotropy: float, attractive_factor: float, precision: int, device: int = 0):
