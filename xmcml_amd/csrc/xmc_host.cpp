@@ -61,10 +61,15 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// dynamic LDS of the transport kernels: short stacks, then the block's ledger
-size_t transport_lds(int block, int tile_count = 0) {
-    return (size_t)xmc::kStackLds * block * sizeof(int2) + 8 * sizeof(double) + (size_t)tile_count * sizeof(double);
+// dynamic LDS of the transport kernels: short stacks, the block's ledger, its detector sums
+// (three words per detector), then its absorption tile
+size_t transport_lds(int block, int tile_count = 0, int det_lds = 0) {
+    return (size_t)xmc::kStackLds * block * sizeof(int2) + 8 * sizeof(double) + (size_t)det_lds * 3 * sizeof(double) +
+           (size_t)tile_count * sizeof(double);
 }
+
+// detectors up to which a block keeps its detector sums in LDS
+constexpr int kDetLds = 64;
 
 // LDS absorption tile: at most this many voxels per block (24 kB; four 256-thread
 // blocks per CU with their stacks -- measured faster on the slab than 13.5 kB at
@@ -287,7 +292,7 @@ static int configure_launch(int device, int depth, int block, int blocks_per_cu,
                             DevBuf<int2>& spill, xmc::Params& P) {
     if (block <= 0) block = 256;
     if (block % 64 != 0 || block > 256) return fail(XMC_ERR_ARG, "blockSize must be 64, 128, 192 or 256");
-    size_t lds = transport_lds(block, P.abs_tile.count);
+    size_t lds = transport_lds(block, P.abs_tile.count, P.det_lds);
     int per_cu = 0;
     HIP_TRY(kernel_occupancy(block, lds, &per_cu));
     if (per_cu < 1) per_cu = 1;
@@ -559,6 +564,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     if (!filt.empty()) HIP_TRY(hipMemcpy(sc->filter.p, filt.data(), filt.size() * 4, hipMemcpyHostToDevice));
     P.det.n_cube = in->numberOfCubeDetectors;
     P.det.n_ring = in->numberOfRingDetectors;
+    P.det_lds = (sc->n_det > 0 && sc->n_det <= kDetLds) ? sc->n_det : 0;
     P.det.cube = sc->cube.p;
     P.det.ring = sc->ring.p;
     P.det.filter = sc->filter.p;
@@ -582,6 +588,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     P.det.T = T;
     P.det.t0 = in->timeStart;
     P.det.dt = dt;
+    P.det.inv_dt = dt > 0.0 ? 1.0 / dt : 0.0;
     P.det.t_start = sc->t_start.p;
     P.det.t_finish = sc->t_finish.p;
 
@@ -735,7 +742,7 @@ static int run_impl(xmc_scene* sc, uint64_t seed, uint64_t first, uint64_t n, hi
     P.seed_value = (2 * seed + 1) & xmc::kMcgMask;
     if (tr) P.trace = *tr;
     HIP_TRY(xmc_launch_zero(P.queue, 1, s));
-    size_t lds = transport_lds(sc->block, P.abs_tile.count);
+    size_t lds = transport_lds(sc->block, P.abs_tile.count, P.det_lds);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1264,6 +1271,7 @@ int xmc_batch_create(const xmc_input* configs, int n, const xmc_options* opt, xm
     HIP_TRY(b->counters.alloc(17));
     HIP_TRY(hipMemset(b->counters.p, 0, 17 * 8));
     b->run = host[0];
+    b->run.det_lds = 0;  // a batch lane's detectors are its configuration's: sums go straight to its tallies
     b->run.tally.stats = b->counters.p;
     b->run.queue = b->counters.p + 16;
     b->run.first_photon = 0;

@@ -46,7 +46,7 @@ struct Detectors {
     const float* ring;  // 5 per ring: center.xy, rs, rb, cos target
     const uint32_t* filter;  // per detector (cubes first): bit l = filterLayers[l]
     int T;                   // time bins per detector
-    double t0, dt;
+    double t0, dt, inv_dt;   // inv_dt = 1 / dt (0 when dt = 0)
     const double* t_start;   // T
     const double* t_finish;  // T
 };
@@ -173,6 +173,10 @@ struct Params {
     Paths paths;
     AbsTile abs_tile;
     Bias bias;
+    // detectors whose weight, target weight and count each block sums in LDS
+    // and adds to the tallies once at its end (0: every detection goes
+    // straight to the tallies); the time bins always go straight
+    int det_lds;
 };
 
 struct QueryParams {

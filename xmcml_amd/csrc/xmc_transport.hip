@@ -39,10 +39,10 @@ constexpr float kInf = __builtin_inff();
 #define XMC_PARK_WAIT 16
 #endif
 #ifndef XMC_START_MIN
-#define XMC_START_MIN 4
+#define XMC_START_MIN 8
 #endif
 #ifndef XMC_START_WAIT
-#define XMC_START_WAIT 4
+#define XMC_START_WAIT 8
 #endif
 #ifndef XMC_VISITS_PER_ROUND
 #define XMC_VISITS_PER_ROUND 1
@@ -674,39 +674,44 @@ __device__ void path_crossings(const Params& C, V3 p0, V3 p1, float w, uint32_t*
     }
 }
 
-// GetDetectorId (:663-742): first matching cube, then ring.
+// GetDetectorId (:663-742): first matching cube, then ring.  The direction
+// test's cosine -d.z / |d|^2 does not depend on the detector and a ring's
+// distance only on its centre, so each is computed once (concentric rings
+// share one distance); the values and the first match are the same.
 __device__ int detector_id(const Detectors& D, V3 p, V3 d, uint32_t visited) {
+    float len = d.x * d.x + d.y * d.y + d.z * d.z;
+    float ac = (-d.z) / len;
     for (int i = 0; i < D.n_cube; ++i) {
         const float* c = D.cube + 7 * i;
         bool in = (p.x >= c[0]) && (p.x < c[3]) && (p.y >= c[1]) && (p.y < c[4]) && (p.z >= c[2]) && (p.z < c[5]);
-        if (in && (visited & D.filter[i])) {
-            float len = d.x * d.x + d.y * d.y + d.z * d.z;
-            float ac = (-d.z) / len;
-            if (ac > c[6]) return i;
-        }
+        if (in && (visited & D.filter[i]) && ac > c[6]) return i;
     }
+    float cx = __builtin_nanf(""), cy = 0.0f, dist = 0.0f;
     for (int i = 0; i < D.n_ring; ++i) {
         const float* r = D.ring + 5 * i;
-        float dist = __builtin_sqrtf(((p.x - r[0]) * (p.x - r[0])) + ((p.y - r[1]) * (p.y - r[1])));
-        bool in = (dist >= r[2]) && (dist < r[3]);
-        if (in && (visited & D.filter[D.n_cube + i])) {
-            float len = d.x * d.x + d.y * d.y + d.z * d.z;
-            float ac = (-d.z) / len;
-            if (ac > r[4]) return D.n_cube + i;
+        if (!(r[0] == cx && r[1] == cy)) {
+            cx = r[0];
+            cy = r[1];
+            dist = __builtin_sqrtf(((p.x - cx) * (p.x - cx)) + ((p.y - cy) * (p.y - cy)));
         }
+        bool in = (dist >= r[2]) && (dist < r[3]);
+        if (in && (visited & D.filter[D.n_cube + i]) && ac > r[4]) return D.n_cube + i;
     }
     return -1;
 }
 
 // UpdateDetectorTimeScale (:812-849): first bin i < T-1 with
-// start_i <= t < finish_i (candidates around (t - t0)/dt), else the open
+// start_i <= t < finish_i (candidates around (t - t0)/dt: a bin overlaps at
+// most its neighbours, so the first match is among them), else the open
 // last bin when t >= start_{T-1}.
 __device__ __forceinline__ int time_bin(const Detectors& D, double t) {
     int T = D.T;
     if (T <= 0) return -1;
     if (T > 1) {
         if (D.dt > 0.0) {
-            double xb = (t - D.t0) / D.dt;
+            // the guess only picks the three candidates; the bin is decided by the comparisons,
+            // so a reciprocal (within an ulp of the quotient) picks the same bin
+            double xb = (t - D.t0) * D.inv_dt;
             int k = xb < 0.0 ? -1 : (xb >= (double)(T - 1) ? T - 1 : (int)xb);
             int lo = k - 1 < 0 ? 0 : k - 1;
             int hi = k + 1 > T - 2 ? T - 2 : k + 1;
@@ -854,8 +859,10 @@ constexpr int kVisitsPerRound = XMC_VISITS_PER_ROUND;
 constexpr bool kParkHits = XMC_PARK_HITS != 0;
 constexpr int kParkMin = XMC_PARK_MIN;    // lanes that make a boundary round worth running
 constexpr uint32_t kParkWait = XMC_PARK_WAIT;  // rounds a boundary interaction may wait
-// Starting a walk (reciprocal direction, entry lookup, first box test) is parked the same
-// way: heightfield 2.90e7 photons/s without, 2.93e7 with 4 lanes or 4 rounds.
+// Starting a walk (reciprocal direction, entry lookup, separating-slab and first box
+// tests) is parked the same way.  Heightfield photons/s with the slabs (two launches of 1e8
+// each): none 3.76e7; 2 lanes or 2 rounds 3.86e7; 4 or 4 4.01e7; 6 or 6 4.08e7; 8 or 8
+// 4.09e7; 10 or 10 4.08e7; 12 or 12 4.03e7; 16 or 16 3.92e7; 24 or 24 3.67e7.
 constexpr int kStartMin = XMC_START_MIN;
 constexpr uint32_t kStartWait = XMC_START_WAIT;
 
@@ -870,8 +877,13 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
              P.spill_depth, 0, 0};
     double* ledger = reinterpret_cast<double*>(lds_stack + kStackLds * blockDim.x);
     if (tid < kLedger) ledger[tid] = 0.0;
-    // the block's absorption tile follows the ledger in LDS
-    double* tile = ledger + kLedger;
+    // the block's detector sums (weight, target weight, count per detector) follow the
+    // ledger in LDS, then its absorption tile
+    const int ndl = BATCHED ? 0 : P.det_lds;
+    double* dsum = ledger + kLedger;
+    unsigned long long* dcount = reinterpret_cast<unsigned long long*>(dsum + 2 * ndl);
+    for (int i = tid; i < 3 * ndl; i += blockDim.x) dsum[i] = 0.0;
+    double* tile = dsum + 3 * ndl;
     const AbsTile& T = P.abs_tile;
     constexpr bool kTile = !BATCHED && !INTERLEAVE;  // the slab-like scenes where deposits crowd
     if constexpr (kTile)
@@ -1170,8 +1182,15 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                                 double wd = (double)w;
                                 float total = trange + orange;
                                 float tw = trange * (w / total);
-                                unsafeAtomicAdd(C.tally.det_weight + id, wd);
-                                atomicAdd(C.tally.det_count + id, 1ull);
+                                if (ndl) {
+                                    atomicAdd(dsum + id, wd);
+                                    atomicAdd(dsum + ndl + id, (double)tw);
+                                    atomicAdd(dcount + id, 1ull);
+                                } else {
+                                    unsafeAtomicAdd(C.tally.det_weight + id, wd);
+                                    unsafeAtomicAdd(C.tally.det_target + id, (double)tw);
+                                    atomicAdd(C.tally.det_count + id, 1ull);
+                                }
                                 int b = time_bin(C.det, (double)tim);
                                 if (b >= 0) {
                                     size_t k = (size_t)id * C.det.T + b;
@@ -1179,7 +1198,6 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                                     atomicAdd(C.tally.bin_count + k, 1ull);
                                     unsafeAtomicAdd(C.tally.bin_weight + k, wd);
                                 }
-                                unsafeAtomicAdd(C.tally.det_target + id, (double)tw);
                                 atomicAdd(ledger + kDet, wd);
                                 if constexpr (PATHS) {
                                     if (vox) {  // UpdateDetectorTrajectory (:794-810)
@@ -1329,6 +1347,12 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     atomicAdd(ledger + kAbs, w_abs);
     __syncthreads();
     if (tid < kLedger) unsafeAtomicAdd(reinterpret_cast<double*>(P.tally.stats + 8) + tid, ledger[tid]);
+    for (int i = tid; i < ndl; i += blockDim.x) {
+        if (dcount[i] == 0) continue;
+        unsafeAtomicAdd(P.tally.det_weight + i, dsum[i]);
+        unsafeAtomicAdd(P.tally.det_target + i, dsum[ndl + i]);
+        atomicAdd(P.tally.det_count + i, dcount[i]);
+    }
     if constexpr (kTile) {
         const Area& A = P.area;
         for (int i = tid; i < T.count; i += blockDim.x) {

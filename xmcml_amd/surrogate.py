@@ -418,9 +418,13 @@ class Surrogate:
         R = np.zeros((n, N_OUTPUTS))
         t0 = time.perf_counter()
         if idx.size:
-            mu, sig = self.predict(X)
-            good = self.uncertainty(sig) <= self.sigma_max
-            R[idx] = np.exp(mu)
+            # predict() and uncertainty() with the same float32 operations, the standard
+            # deviations only for the 16 rings the uncertainty reads
+            out = self.predict_raw(X)
+            O = self.y_mean.size
+            sig16 = np.exp(0.5 * np.clip(out[:, O:O + 16], -10, 10)) * self.y_std[:16]
+            good = sig16.mean(axis=1) <= self.sigma_max
+            R[idx] = np.exp(out[:, :O] * self.y_std + self.y_mean)
             ok[idx[~good]] = False
         t["infer_s"] = time.perf_counter() - t0
         t0 = time.perf_counter()

@@ -77,7 +77,7 @@ def main(argv=None) -> int:
     ap.add_argument("--data-only", action="store_true", help="simulate and write --data-out, do not train")
     ap.add_argument("--eval-from", type=Path,
                     help="take the held-out configurations and their reference MC (layers, Wref, Kref) from this "
-                         "model or data file instead of simulating them")
+                         "model or data file instead of simulating them (also with --data-in)")
     ap.add_argument("--out", type=Path)
     ap.add_argument("--log-every", type=int, default=1000)
     args = ap.parse_args(argv)
@@ -97,9 +97,6 @@ def main(argv=None) -> int:
         X = np.concatenate([z["X"] for z in parts])
         Y = np.concatenate([z["Y"] for z in parts])
         M = np.concatenate([z["M"] for z in parts]).astype(np.float32)
-        z = next(z for z in parts if z["eval_count"].size > 0)
-        eval_a, eval_n, Wref, Kref = z["eval_layers"], z["eval_count"], z["Wref"], z["Kref"]
-        W_lo = z["W_lo"]
         for q in parts:
             for k, v in json.loads(bytes(q["timings"]).decode()).items():
                 timings[k] = timings.get(k, 0.0) + v
@@ -107,6 +104,22 @@ def main(argv=None) -> int:
         if len(photons) != 1:
             ap.error("the data sets were simulated with different photon budgets")
         train_photons = photons.pop()
+        if args.eval_from:
+            # the judgement set of an earlier model or data file; plain MC at this training budget
+            # on it is simulated here when the file does not hold it
+            z = np.load(args.eval_from, allow_pickle=False)
+            eval_a, eval_n, Wref, Kref = z["eval_layers"], z["eval_count"], z["Wref"], z["Kref"]
+            if "W_lo" in z.files and int(z["train_photons"]) == train_photons:
+                W_lo = z["W_lo"]
+            else:
+                t0 = time.perf_counter()
+                _, _, _, W_lo = S.simulate(configs_from_array(eval_a, eval_n), train_photons,
+                                           seed=args.eval_seed * 104729)
+                timings["eval_mc_s"] = timings.get("eval_mc_s", 0.0) + time.perf_counter() - t0
+        else:
+            z = next(z for z in parts if z["eval_count"].size > 0)
+            eval_a, eval_n, Wref, Kref = z["eval_layers"], z["eval_count"], z["Wref"], z["Kref"]
+            W_lo = z["W_lo"]
     else:
         train_cfg = S.draw_configs(args.train_configs, args.train_seed)
         eval_cfg = S.draw_configs(args.eval_configs, args.eval_seed)
