@@ -89,6 +89,35 @@ int xmc_mlp_train_sampled(xmc_mlp* m, const float* X_dev, const float* Y_dev, co
                           int batch, uint64_t seed, int64_t step, float scale, float lr, float beta1, float beta2,
                           float eps, float weightDecay, float* loss, void* stream);
 
+/* Feature rows of layered-tissue configurations, on the device.
+ * layers_dev is [n][slots][5] layer tuples in xmcml's order (refractive
+ * index, mua, mus, anisotropy g, thickness), count_dev [n] the layers used.
+ * Row b of x_dev ([n][6 * slots]) holds, for each used slot, the values
+ * (mua, mus (1 - g), g, n, thickness) mapped from [lo_k, hi_k] to [-1, 1]
+ * (on log scale where logScale[k]), 0 for unused slots, then one 0/1
+ * occupied flag per slot.  inRange_dev[b] = 1 when minLayers <= count <=
+ * slots and every used value lies in [lo_k (1 - tol), hi_k (1 + tol)].
+ * Double arithmetic, rounded to float at the end. */
+typedef struct xmc_layer_box {
+    int slots;
+    int minLayers;
+    double lo[5], hi[5];
+    int logScale[5];
+    double tol;
+} xmc_layer_box;
+
+int xmc_surrogate_encode(const double* layers_dev, const int32_t* count_dev, int64_t n, const xmc_layer_box* box,
+                         float* x_dev, uint8_t* inRange_dev, int device, void* stream);
+
+/* The model's answers for n feature rows on the device (any n; the forward
+ * runs in chunks of 65,536 rows on a buffer the model keeps):
+ *   values_dev[b][j] = exp(mu_j * scale[j] + shift[j]), j < outputs,
+ *   uncertain_dev[b] = 1 when mean_{j < flagColumns} exp(s_j / 2) * scale[j] > sigmaMax,
+ * mu and s the head's predicted means and clamped log-variances (host arrays
+ * scale/shift of `outputs` floats: the targets' standardisation). */
+int xmc_mlp_answer(xmc_mlp* m, const float* x_dev, int64_t n, const float* scale, const float* shift, int flagColumns,
+                   float sigmaMax, float* values_dev, uint8_t* uncertain_dev, void* stream);
+
 /* Device memory for the arrays above, on the model's kind of device (plain
  * hipMalloc/hipMemcpy; kind 1 = host to device, 2 = device to host,
  * 3 = device to device).  xmc_dev_sync waits for all work on `device`. */

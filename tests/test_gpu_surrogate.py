@@ -254,4 +254,63 @@ def test_answer_uses_the_network_or_monte_carlo(dev):
     assert np.array_equal(np.concatenate([rings[mc], tof[mc]], axis=1), W)
     ok = ~mc
     mu_ok, _ = sur.predict(S.encode_many([cfg[i] for i in range(40) if ok[i]]))
-    assert np.array_equal(np.concatenate([rings[ok], tof[ok]], axis=1), np.exp(mu_ok))
+    # exp on the device (the answer path) against numpy's float32 exp: within a few ulps
+    np.testing.assert_allclose(np.concatenate([rings[ok], tof[ok]], axis=1), np.exp(mu_ok), rtol=1e-6, atol=0)
+
+
+def test_device_encoding_equals_the_host_encoding(dev):
+    """xmc_surrogate_encode on 20,000 configurations of the sampling
+    distribution with a few pushed out of the box, one-layer and six-layer
+    counts: the range flags equal in_range_arrays and the feature rows equal
+    encode_arrays (double arithmetic rounded to float; the device's log may
+    differ from the host's in the last double bit, which moves at most one
+    float ulp)."""
+    import ctypes as C
+    from xmcml_amd import engine as E, surrogate as S
+    L, cnt = S.draw_config_arrays(20_000, 31)
+    L[::97, 0, 1] = 3.0        # mua above the box
+    L[5::113, 1, 4] = 0.01     # a thin layer below the box
+    cnt[7::501] = 1
+    n = len(cnt)
+    Ld = E.DeviceArray(L.shape, L, dtype=np.float64)
+    cd = E.DeviceArray(cnt.shape, cnt, dtype=np.int32)
+    X = E.DeviceArray((n, S.N_INPUTS))
+    ok = E.DeviceArray((n,), dtype=np.uint8)
+    E._check(E.lib().xmc_surrogate_encode(Ld.ptr, cd.ptr, n, C.byref(S._layer_box()), X.ptr, ok.ptr, 0, None), "encode")
+    Xg, okg = X.numpy(), ok.numpy() != 0
+    okh = S.in_range_arrays(L, cnt)
+    assert np.array_equal(okg, okh)
+    assert 0.9 * n < okh.sum() < 0.99 * n
+    Xh = S.encode_arrays(L, cnt)
+    ulp = np.spacing(np.abs(Xh).astype(np.float32))
+    assert np.all(np.abs(Xg - Xh) <= ulp)
+    assert np.mean(Xg == Xh) > 0.999
+
+
+def test_device_answers_equal_the_network(dev):
+    """xmc_mlp_answer on 70,000 rows (two chunks): exp of the predicted means
+    and the uncertainty flags equal Surrogate.predict / uncertainty on the host
+    (float32 exp and sums: within a few ulps; flags equal away from sigma_max)."""
+    from pathlib import Path
+    import ctypes as C
+    from xmcml_amd import engine as E, surrogate as S
+    model = Path(__file__).resolve().parents[1] / "xmcml_amd" / "data" / "surrogate_layered.npz"
+    sur = S.Surrogate.load(model, max_batch=4096)
+    L, cnt = S.draw_config_arrays(70_000, 8)
+    Xh = S.encode_arrays(L, cnt)
+    mu, sig = sur.predict(Xh)
+    unc = sur.uncertainty(sig)
+    sur.sigma_max = float(np.quantile(unc, 0.9))
+    n = len(cnt)
+    X = E.DeviceArray(Xh.shape, Xh)
+    V = E.DeviceArray((n, S.N_OUTPUTS))
+    U = E.DeviceArray((n,), dtype=np.uint8)
+    pf = C.POINTER(C.c_float)
+    std, mean = np.ascontiguousarray(sur.y_std), np.ascontiguousarray(sur.y_mean)
+    E._check(E.lib().xmc_mlp_answer(sur.mlp._h, X.ptr, n, std.ctypes.data_as(pf), mean.ctypes.data_as(pf),
+                                    S.FLAG_RINGS, sur.sigma_max, V.ptr, U.ptr, None), "answer")
+    np.testing.assert_allclose(V.numpy(), np.exp(mu), rtol=1e-6, atol=0)
+    flag = U.numpy() != 0
+    clear = np.abs(unc - sur.sigma_max) > 1e-5 * sur.sigma_max
+    assert np.array_equal(flag[clear], (unc > sur.sigma_max)[clear])
+    assert 0.05 * n < flag.sum() < 0.15 * n

@@ -36,6 +36,7 @@ typedef uint32_t u32x4 __attribute__((__vector_size__(16)));
 typedef float f32x4 __attribute__((__vector_size__(16)));
 
 constexpr int BM = 64, BN = 64, BK = 64;
+constexpr int kAnswerRows = 65536;  // rows per chunk of xmc_mlp_answer
 constexpr int LDS_STRIDE = BK + 8;  // halfwords per LDS tile row (144 bytes: 16-byte aligned rows)
 constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2/pi)
 constexpr float kGeluA = 0.044715f;
@@ -427,6 +428,69 @@ __global__ void gather_kernel(const float* X, const float* Y, const float* M, lo
     else mb[(long)r * nout + (c - nin - nout)] = M[i * nout + (c - nin - nout)];
 }
 
+// Feature rows of layered configurations: layer tuples (n, mua, mus, g, d) ->
+// per used slot the box-mapped (mua, mus (1 - g), g, n, d) in [-1, 1] (log
+// scale where the box says), unused slots 0, then the slots' occupied flags;
+// and whether the configuration lies in the box.  The same double operations
+// as the host encoding, rounded to float at the end.
+struct EncodeArgs {
+    const double* layers;
+    const int32_t* count;
+    long n;
+    int slots, min_layers;
+    double lo[5], span[5], loglo[5], logspan[5], lo_tol[5], hi_tol[5];
+    int logscale[5];
+    float* x;
+    uint8_t* in_range;
+};
+
+__global__ void encode_kernel(EncodeArgs a) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int c = a.count[i];
+    bool ok = c >= a.min_layers && c <= a.slots;
+    const int nf = 6 * a.slots;
+    float* x = a.x + i * nf;
+    for (int sl = 0; sl < a.slots; sl++) {
+        const bool used = sl < c;
+        const double* t = a.layers + (i * a.slots + sl) * 5;
+        const double v[5] = {t[1], t[2] * (1.0 - t[3]), t[3], t[0], t[4]};
+        for (int k = 0; k < 5; k++) {
+            double u = 0.0;
+            if (used) {
+                ok = ok && v[k] >= a.lo_tol[k] && v[k] <= a.hi_tol[k];
+                u = a.logscale[k] ? 2.0 * (log(v[k]) - a.loglo[k]) / a.logspan[k] - 1.0
+                                  : 2.0 * (v[k] - a.lo[k]) / a.span[k] - 1.0;
+            }
+            x[sl * 5 + k] = (float)u;
+        }
+        x[5 * a.slots + sl] = used ? 1.0f : 0.0f;
+    }
+    a.in_range[i] = ok ? 1 : 0;
+}
+
+// values[b][j] = exp(out[b][j] * scale[j] + shift[j]), j < outputs
+__global__ void answer_values_kernel(const float* out, long rows, int head, int outputs, const float* scale,
+                                     float* values) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * outputs) return;
+    const long b = e / outputs;
+    const int j = (int)(e - b * outputs);
+    values[e] = expf(out[b * head + j] * scale[j] + scale[outputs + j]);
+}
+
+// uncertain[b] = mean over j < flag_cols of exp(0.5 clamp(out[b][outputs + j], -10, 10)) * scale[j]
+// above sigma_max
+__global__ void answer_flags_kernel(const float* out, long rows, int head, int outputs, int flag_cols,
+                                    const float* scale, float sigma_max, uint8_t* uncertain) {
+    const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= rows) return;
+    const float* o = out + b * head + outputs;
+    float sum = 0.0f;
+    for (int j = 0; j < flag_cols; j++) sum += expf(0.5f * fminf(fmaxf(o[j], -10.0f), 10.0f)) * scale[j];
+    uncertain[b] = (sum / (float)flag_cols > sigma_max) ? 1 : 0;
+}
+
 __global__ void to_bf16_kernel(const float* src, uint16_t* dst, long count) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < count) dst[i] = f2bf(src[i]);
@@ -731,6 +795,8 @@ struct xmc_mlp {
     float *xb = nullptr, *yb = nullptr, *mb = nullptr;  // sampled batch (xmc_mlp_train_sampled)
     int loss_blocks = 0;
     int step = 0;
+    float* ans_out = nullptr;    // head outputs of one answer chunk [kAnswerRows][head]
+    float* ans_scale = nullptr;  // per output: scale, then shift
     std::vector<void*> allocs;
 };
 
@@ -1199,6 +1265,65 @@ int xmc_mlp_train_sampled(xmc_mlp* m, const float* X, const float* Y, const floa
     int rc = backward_impl(m, m->xb, m->yb, m->mb, batch, scale, loss, s);
     if (rc) return rc;
     return adam_impl(m, lr, b1, b2, eps, decay, s);
+}
+
+int xmc_surrogate_encode(const double* layers, const int32_t* count, int64_t n, const xmc_layer_box* box, float* x,
+                         uint8_t* in_range, int device, void* stream) {
+    if (!layers || !count || !box || !x || !in_range) return fail(-1, "NULL argument");
+    if (n < 0) return fail(-1, "negative count");
+    if (box->slots < 1 || box->slots > 8) return fail(-1, "slots must be 1..8");
+    EncodeArgs a{};
+    a.layers = layers;
+    a.count = count;
+    a.n = n;
+    a.slots = box->slots;
+    a.min_layers = box->minLayers;
+    for (int k = 0; k < 5; k++) {
+        if (!(box->lo[k] > 0.0 && box->hi[k] > box->lo[k])) return fail(-1, "box bounds must be 0 < lo < hi");
+        a.lo[k] = box->lo[k];
+        a.span[k] = box->hi[k] - box->lo[k];
+        a.loglo[k] = std::log(box->lo[k]);
+        a.logspan[k] = std::log(box->hi[k]) - std::log(box->lo[k]);
+        a.lo_tol[k] = box->lo[k] * (1.0 - box->tol);
+        a.hi_tol[k] = box->hi[k] * (1.0 + box->tol);
+        a.logscale[k] = box->logScale[k] != 0;
+    }
+    a.x = x;
+    a.in_range = in_range;
+    if (n == 0) return 0;
+    MLP_TRY(hipSetDevice(device));
+    hipLaunchKernelGGL(encode_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a);
+    MLP_TRY(hipGetLastError());
+    return 0;
+}
+
+int xmc_mlp_answer(xmc_mlp* m, const float* x, int64_t n, const float* scale, const float* shift, int flagColumns,
+                   float sigmaMax, float* values, uint8_t* uncertain, void* stream) {
+    if (!m || !x || !scale || !shift || !values || !uncertain) return fail(-1, "NULL argument");
+    if (n < 0) return fail(-1, "negative count");
+    const int O = m->d.outputs;
+    if (flagColumns < 1 || flagColumns > O) return fail(-1, "flagColumns must be 1..outputs");
+    MLP_TRY(hipSetDevice(m->d.device));
+    hipStream_t s = (hipStream_t)stream;
+    if (!m->ans_out) {
+        MLP_TRY(dalloc(m, (void**)&m->ans_out, (size_t)kAnswerRows * m->head * 4));
+        MLP_TRY(dalloc(m, (void**)&m->ans_scale, (size_t)2 * O * 4));
+    }
+    MLP_TRY(hipMemcpyAsync(m->ans_scale, scale, (size_t)O * 4, hipMemcpyHostToDevice, s));
+    MLP_TRY(hipMemcpyAsync(m->ans_scale + O, shift, (size_t)O * 4, hipMemcpyHostToDevice, s));
+    for (int64_t r0 = 0; r0 < n; r0 += kAnswerRows) {
+        const int nb = (int)std::min<int64_t>(kAnswerRows, n - r0);
+        int rc = xmc_mlp_forward(m, x + r0 * m->d.inputs, nb, m->ans_out, s);
+        if (rc) return rc;
+        const long e = (long)nb * O;
+        hipLaunchKernelGGL(answer_values_kernel, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, s, m->ans_out,
+                           (long)nb, m->head, O, m->ans_scale, values + r0 * O);
+        MLP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(answer_flags_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, m->ans_out,
+                           (long)nb, m->head, O, flagColumns, m->ans_scale, sigmaMax, uncertain + r0);
+        MLP_TRY(hipGetLastError());
+    }
+    return 0;
 }
 
 int xmc_dev_alloc(int device, uint64_t bytes, void** ptr) {

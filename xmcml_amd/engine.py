@@ -37,7 +37,7 @@ EXPORTS = [
     # surrogate model (include/xmcml_surrogate.h)
     "xmc_mlp_create", "xmc_mlp_destroy", "xmc_mlp_param_count", "xmc_mlp_get_params", "xmc_mlp_set_params",
     "xmc_mlp_forward", "xmc_mlp_forward_layers", "xmc_mlp_backward", "xmc_mlp_get_grads", "xmc_mlp_adam", "xmc_mlp_train_step",
-    "xmc_mlp_gemm_check", "xmc_mlp_train_sampled", "xmc_dev_alloc", "xmc_dev_free", "xmc_dev_copy", "xmc_dev_sync",
+    "xmc_mlp_gemm_check", "xmc_mlp_train_sampled", "xmc_surrogate_encode", "xmc_mlp_answer", "xmc_dev_alloc", "xmc_dev_free", "xmc_dev_copy", "xmc_dev_sync",
     "xmc_weight_integrals", "xmc_weight_table_compute",
     # file formats and backups (include/xmcml_io.h)
     "xmc_read_input_file", "xmc_read_surface_file", "xmc_document_input", "xmc_document_biasing",
@@ -111,6 +111,8 @@ def lib():
     sig("xmc_mlp_gemm_check", _I, [_PF, _PF, _I, _I, _I, _PF, _I, _I])
     sig("xmc_mlp_train_sampled", _I, [_P, _P, _P, _P, C.c_int64, _I, _U64, C.c_int64, _F, _F, _F, _F, _F, _F,
                                       _PF, _P])
+    sig("xmc_surrogate_encode", _I, [_P, _P, C.c_int64, _P, _P, _P, _I, _P])
+    sig("xmc_mlp_answer", _I, [_P, _P, C.c_int64, _PF, _PF, _I, _F, _P, _P, _P])
     sig("xmc_dev_alloc", _I, [_I, _U64, C.POINTER(_P)])
     sig("xmc_dev_free", _I, [_P])
     sig("xmc_dev_copy", _I, [_P, _P, _U64, _I])
@@ -246,15 +248,17 @@ class Scene:
 
 
 class DeviceArray:
-    """A float32 array in device memory, allocated by the engine library
-    (xmc_dev_alloc); upload() / numpy() copy through host memory."""
+    """An array in device memory (float32 unless dtype says otherwise),
+    allocated by the engine library (xmc_dev_alloc); upload() / numpy() copy
+    through host memory."""
 
-    def __init__(self, shape, data=None, device: int = 0):
+    def __init__(self, shape, data=None, device: int = 0, dtype=np.float32):
         self.shape = tuple(int(v) for v in shape)
         self.device = device
-        self.nbytes = 4 * int(np.prod(self.shape)) if self.shape else 4
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.dtype.itemsize * int(np.prod(self.shape)) if self.shape else self.dtype.itemsize
         p = _P()
-        _check(lib().xmc_dev_alloc(device, self.nbytes, C.byref(p)), "xmc_dev_alloc")
+        _check(lib().xmc_dev_alloc(device, max(self.nbytes, 1), C.byref(p)), "xmc_dev_alloc")
         self._p = p
         if data is not None:
             self.upload(data)
@@ -264,14 +268,16 @@ class DeviceArray:
         return self._p
 
     def upload(self, a):
-        a = np.ascontiguousarray(a, dtype=np.float32)
+        a = np.ascontiguousarray(a, dtype=self.dtype)
         if a.shape != self.shape:
             raise ValueError(f"shape {a.shape} != {self.shape}")
-        _check(lib().xmc_dev_copy(self._p, a.ctypes.data, self.nbytes, 1), "xmc_dev_copy")
+        if self.nbytes:
+            _check(lib().xmc_dev_copy(self._p, a.ctypes.data, self.nbytes, 1), "xmc_dev_copy")
 
     def numpy(self) -> np.ndarray:
-        a = np.empty(self.shape, np.float32)
-        _check(lib().xmc_dev_copy(a.ctypes.data, self._p, self.nbytes, 2), "xmc_dev_copy")
+        a = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            _check(lib().xmc_dev_copy(a.ctypes.data, self._p, self.nbytes, 2), "xmc_dev_copy")
         return a
 
     def close(self):

@@ -43,6 +43,22 @@ MIN_COUNT = 16  # detected packets below which a target is masked out
 
 # (low, high, log-scaled) of mua [1/mm], mus' [1/mm], g, n, thickness [mm]
 LAYER_BOX = [(1e-3, 1.0, True), (0.5, 5.0, True), (0.7, 0.95, False), (1.33, 1.5, False), (0.05, 10.0, True)]
+FLAG_RINGS = 16  # rings whose predicted spread decides the fallback to Monte Carlo
+RANGE_TOL = 1e-9
+
+
+class _LayerBox(C.Structure):
+    """xmc_layer_box (include/xmcml_surrogate.h)."""
+    _fields_ = [("slots", C.c_int), ("minLayers", C.c_int), ("lo", C.c_double * 5), ("hi", C.c_double * 5),
+                ("logScale", C.c_int * 5), ("tol", C.c_double)]
+
+
+def _layer_box() -> "_LayerBox":
+    b = _LayerBox()
+    b.slots, b.minLayers, b.tol = MAX_LAYERS, 2, RANGE_TOL
+    for k, (lo, hi, lg) in enumerate(LAYER_BOX):
+        b.lo[k], b.hi[k], b.logScale[k] = lo, hi, int(lg)
+    return b
 
 Layer = Tuple[float, float, float, float, float]  # (n, mua, mus, g, thickness), as scenes.layered_scene
 
@@ -102,7 +118,7 @@ def encode_many(configs: Sequence[Sequence[Layer]]) -> np.ndarray:
     return encode_arrays(*configs_to_arrays(configs)) if configs else np.zeros((0, N_INPUTS), np.float32)
 
 
-def in_range_arrays(L: np.ndarray, count: np.ndarray, tol: float = 1e-9) -> np.ndarray:
+def in_range_arrays(L: np.ndarray, count: np.ndarray, tol: float = RANGE_TOL) -> np.ndarray:
     """Per configuration: 2..5 layers, every used layer inside the sampling box."""
     count = np.asarray(count, np.int64)
     used = np.arange(MAX_LAYERS)[None, :] < count[:, None]
@@ -402,30 +418,49 @@ class Surrogate:
 
     def uncertainty(self, sig: np.ndarray) -> np.ndarray:
         """Per-configuration score: mean predicted sd of ln R over the inner 16 rings."""
-        return sig[:, :16].mean(axis=1)
+        return sig[:, :FLAG_RINGS].mean(axis=1)
 
     def answer_arrays(self, L: np.ndarray, count: np.ndarray, mc_photons: int = 1_000_000, seed: int = 1):
         """answer() on configurations given as (n, 5, 5) layer tuples and layer
-        counts.  Returns (rings (n, 32), TOF (n, 64), answered-by-MC flags (n,),
-        seconds spent in each stage)."""
+        counts.  Encoding, the range check, the network, exp of its means and
+        the uncertainty check run on the device (xmc_surrogate_encode,
+        xmc_mlp_answer); flagged configurations are simulated.  Returns (rings
+        (n, 32), TOF (n, 64), answered-by-MC flags (n,), seconds spent in each
+        stage)."""
         t = {}
-        t0 = time.perf_counter()
         n = len(count)
-        ok = in_range_arrays(L, count)
-        idx = np.nonzero(ok)[0]
-        X = encode_arrays(L[idx], count[idx])
-        t["encode_s"] = time.perf_counter() - t0
         R = np.zeros((n, N_OUTPUTS))
+        ok = np.zeros(n, bool)
         t0 = time.perf_counter()
-        if idx.size:
-            # predict() and uncertainty() with the same float32 operations, the standard
-            # deviations only for the 16 rings the uncertainty reads
-            out = self.predict_raw(X)
-            O = self.y_mean.size
-            sig16 = np.exp(0.5 * np.clip(out[:, O:O + 16], -10, 10)) * self.y_std[:16]
-            good = sig16.mean(axis=1) <= self.sigma_max
-            R[idx] = np.exp(out[:, :O] * self.y_std + self.y_mean)
-            ok[idx[~good]] = False
+        if n:
+            dev = self.mlp.device
+            L = np.ascontiguousarray(L, np.float64).reshape(n, MAX_LAYERS, 5)
+            Ld = engine.DeviceArray((n, MAX_LAYERS, 5), L, device=dev, dtype=np.float64)
+            cd = engine.DeviceArray((n,), count, device=dev, dtype=np.int32)
+            X = engine.DeviceArray((n, N_INPUTS), device=dev)
+            inr = engine.DeviceArray((n,), device=dev, dtype=np.uint8)
+            engine._check(engine.lib().xmc_surrogate_encode(Ld.ptr, cd.ptr, n, C.byref(_layer_box()), X.ptr, inr.ptr,
+                                                            dev, None), "xmc_surrogate_encode")
+            Ld.close()
+            cd.close()
+            engine.device_sync(dev)
+            t["encode_s"] = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            V = engine.DeviceArray((n, N_OUTPUTS), device=dev)
+            U = engine.DeviceArray((n,), device=dev, dtype=np.uint8)
+            std = np.ascontiguousarray(self.y_std, np.float32)
+            mean = np.ascontiguousarray(self.y_mean, np.float32)
+            pf = C.POINTER(C.c_float)
+            engine._check(engine.lib().xmc_mlp_answer(self.mlp._h, X.ptr, n, std.ctypes.data_as(pf),
+                                                      mean.ctypes.data_as(pf), FLAG_RINGS, self.sigma_max, V.ptr,
+                                                      U.ptr, None), "xmc_mlp_answer")
+            ok = (inr.numpy() != 0) & (U.numpy() == 0)
+            vals = V.numpy()
+            R[ok] = vals[ok]
+            for a in (X, inr, V, U):
+                a.close()
+        else:
+            t["encode_s"] = 0.0
         t["infer_s"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         fb = np.nonzero(~ok)[0]
