@@ -60,6 +60,9 @@ struct Tallies {
     double* bin_target;
     unsigned long long* bin_count;
     unsigned long long* stats;  // 6 event counters, trajectory overflows, entry walks, then 8 ledger doubles from word 8
+    // 1 or 8 copies of the detector tallies ([copy][detector] and [copy][detector][bin]);
+    // block b adds to copy b mod 8
+    int copies;
 };
 
 // Voxel-centre plane crossings of the photon path (UpdatePhotonTrajectory,
@@ -113,7 +116,7 @@ struct DistGrid {
     float margin2;
     // per cell a separating slab near the mesh (64-bit words, layout of
     // SlabGrid in xmc_distance.h): radius unit h / 8; null = none
-    const uint2* slabs;
+    const uint64_t* slabs;
     float slab_unit;
 };
 

@@ -774,7 +774,8 @@ __device__ __forceinline__ bool slab_skips(const DistGrid& G, V3 p, V3 d, float 
     if (!(fx >= 0.0f && fy >= 0.0f && fz >= 0.0f)) return false;
     int ix = (int)fx, iy = (int)fy, iz = (int)fz;
     if (ix >= G.dims[0] || iy >= G.dims[1] || iz >= G.dims[2]) return false;
-    const uint2 w = G.slabs[((size_t)iz * G.dims[1] + iy) * G.dims[0] + ix];
+    const uint64_t w64 = G.slabs[((size_t)iz * G.dims[1] + iy) * G.dims[0] + ix];
+    const uint2 w = make_uint2((uint32_t)w64, (uint32_t)(w64 >> 32));
     float ox = (fx - ((float)ix + 0.5f)) * G.h;
     float oy = (fy - ((float)iy + 0.5f)) * G.h;
     float oz = (fz - ((float)iz + 0.5f)) * G.h;
@@ -1182,18 +1183,21 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                                 double wd = (double)w;
                                 float total = trange + orange;
                                 float tw = trange * (w / total);
+                                // this block's copy of the tallies
+                                const int cid = (C.tally.copies > 1 ? (int)(blockIdx.x & 7u) : 0) *
+                                                    (C.det.n_cube + C.det.n_ring) + id;
                                 if (ndl) {
                                     atomicAdd(dsum + id, wd);
                                     atomicAdd(dsum + ndl + id, (double)tw);
                                     atomicAdd(dcount + id, 1ull);
                                 } else {
-                                    unsafeAtomicAdd(C.tally.det_weight + id, wd);
-                                    unsafeAtomicAdd(C.tally.det_target + id, (double)tw);
-                                    atomicAdd(C.tally.det_count + id, 1ull);
+                                    unsafeAtomicAdd(C.tally.det_weight + cid, wd);
+                                    unsafeAtomicAdd(C.tally.det_target + cid, (double)tw);
+                                    atomicAdd(C.tally.det_count + cid, 1ull);
                                 }
                                 int b = time_bin(C.det, (double)tim);
                                 if (b >= 0) {
-                                    size_t k = (size_t)id * C.det.T + b;
+                                    size_t k = (size_t)cid * C.det.T + b;
                                     unsafeAtomicAdd(C.tally.bin_target + k, (double)tw);
                                     atomicAdd(C.tally.bin_count + k, 1ull);
                                     unsafeAtomicAdd(C.tally.bin_weight + k, wd);
@@ -1349,9 +1353,10 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     if (tid < kLedger) unsafeAtomicAdd(reinterpret_cast<double*>(P.tally.stats + 8) + tid, ledger[tid]);
     for (int i = tid; i < ndl; i += blockDim.x) {
         if (dcount[i] == 0) continue;
-        unsafeAtomicAdd(P.tally.det_weight + i, dsum[i]);
-        unsafeAtomicAdd(P.tally.det_target + i, dsum[ndl + i]);
-        atomicAdd(P.tally.det_count + i, dcount[i]);
+        const int ci = (P.tally.copies > 1 ? (int)(blockIdx.x & 7u) : 0) * ndl + i;
+        unsafeAtomicAdd(P.tally.det_weight + ci, dsum[i]);
+        unsafeAtomicAdd(P.tally.det_target + ci, dsum[ndl + i]);
+        atomicAdd(P.tally.det_count + ci, dcount[i]);
     }
     if constexpr (kTile) {
         const Area& A = P.area;
