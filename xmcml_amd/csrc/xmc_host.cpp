@@ -71,10 +71,6 @@ size_t transport_lds(int block, int tile_count = 0, int det_lds = 0) {
 // detectors up to which a block keeps its detector sums in LDS
 constexpr int kDetLds = 64;
 
-// copies of a scene's detector tallies, when they stay below kTallyCopyWords words a copy
-constexpr int kTallyCopies = 8;
-constexpr size_t kTallyCopyWords = (size_t)1 << 18;
-
 // LDS absorption tile: at most this many voxels per block (24 kB; four 256-thread
 // blocks per CU with their stacks -- measured faster on the slab than 13.5 kB at
 // five blocks: 4.2e7 vs 3.8e7 photons/s, against 2.3e7 without a tile)
@@ -133,7 +129,6 @@ struct xmc_scene {
     DevBuf<unsigned long long> slab;
     size_t off_abs = 0, off_dw = 0, off_dt = 0, off_dc = 0, off_bw = 0, off_bt = 0, off_bc = 0, off_st = 0,
            off_q = 0, slab_words = 0;
-    int copies = 1;  // copies of the detector tallies (kTallyCopies or 1)
     DevBuf<int2> spill;
     DevBuf<int> perm_dev;
     DevBuf<uint32_t> dist;  // walk-skipping grid cells
@@ -597,28 +592,22 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     P.det.t_start = sc->t_start.p;
     P.det.t_finish = sc->t_finish.p;
 
-    // tallies in one zeroed slab of 64-bit words.  The detector tallies are kept in
-    // kTallyCopies copies, block b adding to copy b mod 8 (the round-robin dispatch puts
-    // consecutive blocks on different XCDs), so that the photons of one configuration, which
-    // mostly leave through the same inner rings at the same early times, do not all add to
-    // the same few words; collect sums the copies in order
-    sc->copies = ((size_t)sc->n_det * (T + 1) <= kTallyCopyWords) ? kTallyCopies : 1;
-    const size_t cp = (size_t)sc->copies;
+    // tallies in one zeroed slab of 64-bit words
     size_t w = 0;
     sc->off_abs = w;
     w += sc->absorption ? (size_t)sc->grid : 0;
     sc->off_dw = w;
-    w += cp * sc->n_det;
+    w += sc->n_det;
     sc->off_dt = w;
-    w += cp * sc->n_det;
+    w += sc->n_det;
     sc->off_dc = w;
-    w += cp * sc->n_det;
+    w += sc->n_det;
     sc->off_bw = w;
-    w += cp * sc->n_det * T;
+    w += (size_t)sc->n_det * T;
     sc->off_bt = w;
-    w += cp * sc->n_det * T;
+    w += (size_t)sc->n_det * T;
     sc->off_bc = w;
-    w += cp * sc->n_det * T;
+    w += (size_t)sc->n_det * T;
     sc->off_st = w;
     w += 16;
     sc->off_q = w;
@@ -634,7 +623,6 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     P.tally.bin_weight = reinterpret_cast<double*>(s + sc->off_bw);
     P.tally.bin_target = reinterpret_cast<double*>(s + sc->off_bt);
     P.tally.bin_count = s + sc->off_bc;
-    P.tally.copies = sc->copies;
     P.tally.stats = s + sc->off_st;
     P.queue = s + sc->off_q;
 
@@ -800,29 +788,14 @@ static int collect_impl(xmc_scene* sc, xmc_output* out, hipStream_t stream) {
     }
     for (int d = 0; d < sc->n_det; d++) {
         xmc_detector_info& di = out->detectorInfo[d];
-        double w = 0.0, tw = 0.0;
-        unsigned long long n = 0;
-        for (int c = 0; c < sc->copies; c++) {
-            const size_t i = (size_t)c * sc->n_det + d;
-            w += dv(sc->off_dw)[i];
-            tw += dv(sc->off_dt)[i];
-            n += h[sc->off_dc + i];
-        }
-        di.weight += w;
-        di.targetWeight += tw;
-        di.numberOfPhotons += n;
+        di.weight += dv(sc->off_dw)[d];
+        di.targetWeight += dv(sc->off_dt)[d];
+        di.numberOfPhotons += h[sc->off_dc + d];
         for (int b = 0; b < sc->T && b < di.timeScaleSize; b++) {
-            double bw = 0.0, bt = 0.0;
-            unsigned long long bn = 0;
-            for (int c = 0; c < sc->copies; c++) {
-                const size_t k = ((size_t)c * sc->n_det + d) * sc->T + b;
-                bw += dv(sc->off_bw)[k];
-                bt += dv(sc->off_bt)[k];
-                bn += h[sc->off_bc + k];
-            }
-            di.timeScale[b].weight += bw;
-            di.timeScale[b].targetWeight += bt;
-            di.timeScale[b].numberOfPhotons += bn;
+            size_t k = (size_t)d * sc->T + b;
+            di.timeScale[b].weight += dv(sc->off_bw)[k];
+            di.timeScale[b].targetWeight += dv(sc->off_bt)[k];
+            di.timeScale[b].numberOfPhotons += h[sc->off_bc + k];
         }
     }
     const size_t G = (size_t)sc->grid;

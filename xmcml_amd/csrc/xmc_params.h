@@ -60,9 +60,6 @@ struct Tallies {
     double* bin_target;
     unsigned long long* bin_count;
     unsigned long long* stats;  // 6 event counters, trajectory overflows, entry walks, then 8 ledger doubles from word 8
-    // 1 or 8 copies of the detector tallies ([copy][detector] and [copy][detector][bin]);
-    // block b adds to copy b mod 8
-    int copies;
 };
 
 // Voxel-centre plane crossings of the photon path (UpdatePhotonTrajectory,

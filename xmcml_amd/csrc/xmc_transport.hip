@@ -746,7 +746,9 @@ __device__ __forceinline__ int walk_start(const DistGrid& G, V3 p, V3 d, float s
     float ox = (fx - ((float)ix + 0.5f)) * G.h;
     float oy = (fy - ((float)iy + 0.5f)) * G.h;
     float oz = (fz - ((float)iz + 0.5f)) * G.h;
-    float off = __builtin_sqrtf(ox * ox + oy * oy + oz * oz);
+    // |p - centre| only enters the bound, whose margin is 1e-4 mm and more: the hardware
+    // square root (within an ulp) serves, in place of the correctly rounded sequence
+    float off = __builtin_amdgcn_sqrtf(ox * ox + oy * oy + oz * oz);
     float L = (float)(word & 0xFFu) * G.step_q;
     float room = L - off - (1.0e-4f + 16.0f * offset_scale(p));
     if (room > 0.0f && step * step * dot(d, d) < room * room) return -1;
@@ -779,7 +781,7 @@ __device__ __forceinline__ bool slab_skips(const DistGrid& G, V3 p, V3 d, float 
     float ox = (fx - ((float)ix + 0.5f)) * G.h;
     float oy = (fy - ((float)iy + 0.5f)) * G.h;
     float oz = (fz - ((float)iz + 0.5f)) * G.h;
-    float off = __builtin_sqrtf(ox * ox + oy * oy + oz * oz);
+    float off = __builtin_amdgcn_sqrtf(ox * ox + oy * oy + oz * oz);  // within an ulp, as in walk_start
     float m = 1.0e-4f + 16.0f * offset_scale(p);
     float R = (float)((w.y >> 16) & 0xFFu) * G.slab_unit;
     float room = R - m - off;
@@ -1183,21 +1185,18 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                                 double wd = (double)w;
                                 float total = trange + orange;
                                 float tw = trange * (w / total);
-                                // this block's copy of the tallies
-                                const int cid = (C.tally.copies > 1 ? (int)(blockIdx.x & 7u) : 0) *
-                                                    (C.det.n_cube + C.det.n_ring) + id;
                                 if (ndl) {
                                     atomicAdd(dsum + id, wd);
                                     atomicAdd(dsum + ndl + id, (double)tw);
                                     atomicAdd(dcount + id, 1ull);
                                 } else {
-                                    unsafeAtomicAdd(C.tally.det_weight + cid, wd);
-                                    unsafeAtomicAdd(C.tally.det_target + cid, (double)tw);
-                                    atomicAdd(C.tally.det_count + cid, 1ull);
+                                    unsafeAtomicAdd(C.tally.det_weight + id, wd);
+                                    unsafeAtomicAdd(C.tally.det_target + id, (double)tw);
+                                    atomicAdd(C.tally.det_count + id, 1ull);
                                 }
                                 int b = time_bin(C.det, (double)tim);
                                 if (b >= 0) {
-                                    size_t k = (size_t)cid * C.det.T + b;
+                                    size_t k = (size_t)id * C.det.T + b;
                                     unsafeAtomicAdd(C.tally.bin_target + k, (double)tw);
                                     atomicAdd(C.tally.bin_count + k, 1ull);
                                     unsafeAtomicAdd(C.tally.bin_weight + k, wd);
@@ -1353,10 +1352,9 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     if (tid < kLedger) unsafeAtomicAdd(reinterpret_cast<double*>(P.tally.stats + 8) + tid, ledger[tid]);
     for (int i = tid; i < ndl; i += blockDim.x) {
         if (dcount[i] == 0) continue;
-        const int ci = (P.tally.copies > 1 ? (int)(blockIdx.x & 7u) : 0) * ndl + i;
-        unsafeAtomicAdd(P.tally.det_weight + ci, dsum[i]);
-        unsafeAtomicAdd(P.tally.det_target + ci, dsum[ndl + i]);
-        atomicAdd(P.tally.det_count + ci, dcount[i]);
+        unsafeAtomicAdd(P.tally.det_weight + i, dsum[i]);
+        unsafeAtomicAdd(P.tally.det_target + i, dsum[ndl + i]);
+        atomicAdd(P.tally.det_count + i, dcount[i]);
     }
     if constexpr (kTile) {
         const Area& A = P.area;
