@@ -382,7 +382,7 @@ void build_slab_grid(const xmc_surface* s, int ns, const DistanceGrid& dg, float
             std::vector<uint32_t> stamp(tris.size(), 0xFFFFFFFFu);
             std::vector<std::pair<int, double>> near;  // triangle, squared distance (<= Rmax^2)
             std::vector<std::pair<double, double>> iv, bands;
-            const int kSamples = 32;
+            const int kSamples = 64;
             float sp[kSamples][3], sd[kSamples][3], ss[kSamples], soff[kSamples];
             for (;;) {
                 const size_t chunk = next.fetch_add(256);

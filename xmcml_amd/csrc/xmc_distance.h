@@ -42,7 +42,7 @@ void build_distance_grid(const xmc_surface* surfaces, int n_surfaces, const doub
 // a1, b1, a2, b2 as int8 multiples of R / 127, R as a uint8 multiple of
 // h / 8; 0 = no slab (R = 0 skips nothing).  Built for cells whose distance
 // bound is below 3.5 cells; per cell, R and n (the nearest triangle's normal
-// or the mean normal within R) are those that skip most of 32 sample steps
+// or the mean normal within R) are those that skip most of 64 sample steps
 // (uniform start in the cell, isotropic direction, exponential length of
 // mean `mean_step`).
 struct SlabGrid {
