@@ -527,7 +527,10 @@ __global__ void to_bf16_kernel(const float* src, uint16_t* dst, long count) {
 // both halves are written to LDS then (the layer is computed in place: every
 // wave must have read its input before any output is stored).
 
-constexpr int kTileRows = 64;                 // rows per block: kTileRows / 32 row tiles
+#ifndef XMC_INFER_ROWS
+#define XMC_INFER_ROWS 64
+#endif
+constexpr int kTileRows = XMC_INFER_ROWS;     // rows per block: kTileRows / 32 row tiles
 constexpr int kInferWaves = kTileRows / 16;
 constexpr int kInferThreads = 64 * kInferWaves;
 constexpr int kRowTiles = kTileRows / 32;
