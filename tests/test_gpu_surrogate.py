@@ -256,6 +256,14 @@ def test_answer_uses_the_network_or_monte_carlo(dev):
     mu_ok, _ = sur.predict(S.encode_many([cfg[i] for i in range(40) if ok[i]]))
     # exp on the device (the answer path) against numpy's float32 exp: within a few ulps
     np.testing.assert_allclose(np.concatenate([rings[ok], tof[ok]], axis=1), np.exp(mu_ok), rtol=1e-6, atol=0)
+    # the same answers when the device path runs in chunks of 7 configurations
+    chunk = S.ANSWER_CHUNK
+    try:
+        S.ANSWER_CHUNK = 7
+        rings7, tof7, mc7 = sur.answer(cfg, mc_photons=20_000, seed=5)
+    finally:
+        S.ANSWER_CHUNK = chunk
+    assert np.array_equal(mc7, mc) and np.array_equal(rings7, rings) and np.array_equal(tof7, tof)
 
 
 def test_device_encoding_equals_the_host_encoding(dev):

@@ -44,6 +44,7 @@ MIN_COUNT = 16  # detected packets below which a target is masked out
 # (low, high, log-scaled) of mua [1/mm], mus' [1/mm], g, n, thickness [mm]
 LAYER_BOX = [(1e-3, 1.0, True), (0.5, 5.0, True), (0.7, 0.95, False), (1.33, 1.5, False), (0.05, 10.0, True)]
 FLAG_RINGS = 16  # rings whose predicted spread decides the fallback to Monte Carlo
+ANSWER_CHUNK = 1 << 22  # configurations per pass of the device answer path (~2 GB of device buffers)
 RANGE_TOL = 1e-9
 
 
@@ -427,41 +428,42 @@ class Surrogate:
         xmc_mlp_answer); flagged configurations are simulated.  Returns (rings
         (n, 32), TOF (n, 64), answered-by-MC flags (n,), seconds spent in each
         stage)."""
-        t = {}
+        t = {"encode_s": 0.0, "infer_s": 0.0}
         n = len(count)
         R = np.zeros((n, N_OUTPUTS))
         ok = np.zeros(n, bool)
-        t0 = time.perf_counter()
-        if n:
-            dev = self.mlp.device
-            L = np.ascontiguousarray(L, np.float64).reshape(n, MAX_LAYERS, 5)
-            Ld = engine.DeviceArray((n, MAX_LAYERS, 5), L, device=dev, dtype=np.float64)
-            cd = engine.DeviceArray((n,), count, device=dev, dtype=np.int32)
-            X = engine.DeviceArray((n, N_INPUTS), device=dev)
-            inr = engine.DeviceArray((n,), device=dev, dtype=np.uint8)
-            engine._check(engine.lib().xmc_surrogate_encode(Ld.ptr, cd.ptr, n, C.byref(_layer_box()), X.ptr, inr.ptr,
+        L = np.ascontiguousarray(L, np.float64).reshape(n, MAX_LAYERS, 5)
+        count = np.asarray(count)
+        dev = self.mlp.device
+        std = np.ascontiguousarray(self.y_std, np.float32)
+        mean = np.ascontiguousarray(self.y_mean, np.float32)
+        pf = C.POINTER(C.c_float)
+        for c0 in range(0, n, ANSWER_CHUNK):  # device buffers of at most ANSWER_CHUNK rows
+            m = min(ANSWER_CHUNK, n - c0)
+            t0 = time.perf_counter()
+            Ld = engine.DeviceArray((m, MAX_LAYERS, 5), L[c0:c0 + m], device=dev, dtype=np.float64)
+            cd = engine.DeviceArray((m,), count[c0:c0 + m], device=dev, dtype=np.int32)
+            X = engine.DeviceArray((m, N_INPUTS), device=dev)
+            inr = engine.DeviceArray((m,), device=dev, dtype=np.uint8)
+            engine._check(engine.lib().xmc_surrogate_encode(Ld.ptr, cd.ptr, m, C.byref(_layer_box()), X.ptr, inr.ptr,
                                                             dev, None), "xmc_surrogate_encode")
             Ld.close()
             cd.close()
             engine.device_sync(dev)
-            t["encode_s"] = time.perf_counter() - t0
+            t["encode_s"] += time.perf_counter() - t0
             t0 = time.perf_counter()
-            V = engine.DeviceArray((n, N_OUTPUTS), device=dev)
-            U = engine.DeviceArray((n,), device=dev, dtype=np.uint8)
-            std = np.ascontiguousarray(self.y_std, np.float32)
-            mean = np.ascontiguousarray(self.y_mean, np.float32)
-            pf = C.POINTER(C.c_float)
-            engine._check(engine.lib().xmc_mlp_answer(self.mlp._h, X.ptr, n, std.ctypes.data_as(pf),
+            V = engine.DeviceArray((m, N_OUTPUTS), device=dev)
+            U = engine.DeviceArray((m,), device=dev, dtype=np.uint8)
+            engine._check(engine.lib().xmc_mlp_answer(self.mlp._h, X.ptr, m, std.ctypes.data_as(pf),
                                                       mean.ctypes.data_as(pf), FLAG_RINGS, self.sigma_max, V.ptr,
                                                       U.ptr, None), "xmc_mlp_answer")
-            ok = (inr.numpy() != 0) & (U.numpy() == 0)
+            okc = (inr.numpy() != 0) & (U.numpy() == 0)
             vals = V.numpy()
-            R[ok] = vals[ok]
+            ok[c0:c0 + m] = okc
+            R[c0:c0 + m][okc] = vals[okc]
             for a in (X, inr, V, U):
                 a.close()
-        else:
-            t["encode_s"] = 0.0
-        t["infer_s"] = time.perf_counter() - t0
+            t["infer_s"] += time.perf_counter() - t0
         t0 = time.perf_counter()
         fb = np.nonzero(~ok)[0]
         if fb.size:
