@@ -592,6 +592,13 @@ template <class R> struct Scene {
     std::vector<std::vector<double>> tables;
     std::vector<std::vector<float>> tables_f;  // the 32-bit mode's float copies
     std::vector<int> layer_table;
+    // the 32-bit mode's planar form (the engine's float rule for scenes whose every
+    // surface is an axis-aligned rectangle at one z: two triangles sharing a diagonal)
+    struct PlanarForm {
+        int n = 0;
+        float z[16], x0[16], x1[16], y0[16], y1[16];
+        int tri[16];
+    } planar;
 
     // GetLocalAttractiveVector (:325-362), statement for statement
     V3<R> attractive(const V3<R>& pos, int l) const {
@@ -679,6 +686,7 @@ template <class R> struct Scene {
             rings.push_back(q);
         }
         if (input->useBiasing) init_bias(input);
+        if constexpr (sizeof(R) == 4) init_planar(input);
         rsp = xmco_specular(input->layerInfo);
         w0 = (R)in_val(1.0 - rsp);
         min_weight = (R)in_val(input->minWeight);
@@ -738,6 +746,89 @@ template <class R> struct Scene {
         for (int l = 0; l < input->numberOfLayers; l++)
             for (size_t t = 0; t < tg.size(); t++)
                 if (tg[t] == input->layerInfo[l].anisotropy) layer_table[l] = (int)t;
+    }
+
+    // the planar form: every surface two triangles over the four corners of an
+    // axis-aligned rectangle at one z, sharing a diagonal; n stays 0 otherwise
+    void init_planar(const xmc_input* input) {
+        planar.n = 0;
+        const int ns = input->numberOfSurfaces;
+        if (ns < 1 || ns > 16) return;
+        for (int si = 0; si < ns; si++) {
+            const xmc_surface& S = input->surface[si];
+            if (S.numberOfTriangles != 2) return;
+            const int idx[6] = {S.triangles[0].x, S.triangles[0].y, S.triangles[0].z,
+                                S.triangles[1].x, S.triangles[1].y, S.triangles[1].z};
+            for (int k = 0; k < 6; k++)
+                if (idx[k] < 0 || idx[k] >= S.numberOfVertices) return;
+            const double z0 = S.vertices[idx[0]].z;
+            double xlo = S.vertices[idx[0]].x, xhi = xlo, ylo = S.vertices[idx[0]].y, yhi = ylo;
+            for (int k = 0; k < 6; k++) {
+                const xmc_double3& v = S.vertices[idx[k]];
+                if (v.z != z0) return;
+                xlo = std::min(xlo, v.x);
+                xhi = std::max(xhi, v.x);
+                ylo = std::min(ylo, v.y);
+                yhi = std::max(yhi, v.y);
+            }
+            if (!(xlo < xhi && ylo < yhi)) return;
+            int cover[2] = {0, 0};
+            for (int k = 0; k < 6; k++) {
+                const xmc_double3& v = S.vertices[idx[k]];
+                const int cx = v.x == xlo ? 0 : (v.x == xhi ? 1 : -1);
+                const int cy = v.y == ylo ? 0 : (v.y == yhi ? 1 : -1);
+                if (cx < 0 || cy < 0) return;
+                cover[k / 3] |= 1 << (cx + 2 * cy);
+            }
+            auto count = [](int m) { int c = 0; for (int b = 0; b < 4; b++) c += (m >> b) & 1; return c; };
+            if (count(cover[0]) != 3 || count(cover[1]) != 3 || (cover[0] | cover[1]) != 15) return;
+            const int shared = cover[0] & cover[1];
+            if (shared != 9 && shared != 6) return;
+            planar.z[si] = (float)z0;
+            planar.x0[si] = (float)xlo;
+            planar.x1[si] = (float)xhi;
+            planar.y0[si] = (float)ylo;
+            planar.y1[si] = (float)yhi;
+            planar.tri[si] = -1;
+        }
+        for (size_t k = 0; k < bvh.tris.size(); k++)
+            if (planar.tri[bvh.tris[k].surf] < 0) planar.tri[bvh.tris[k].surf] = (int)k;
+        planar.n = ns;
+    }
+
+    // the 32-bit mode's nearest surface of a planar scene: t = (z_s - o.z) * (1 / d.z)
+    // for |d.z| > EPSILON, hit when t >= 0 and the point lies in the rectangle (edges
+    // included); the smallest t, the first surface on ties; reported when t <= step
+    Hit<float> planar_hit(const V3<float>& o, const V3<float>& d, float step, bool limited) const {
+        Hit<float> res;
+        res.found = 0;
+        res.distance = Prec<float>::kMaxDistance;
+        res.surface = -1;
+        res.tri = -1;
+        res.normal = {0, 0, 0};
+        float best = std::numeric_limits<float>::infinity();
+        int tri = -1;
+        if (std::fabs(d.z) > 1.0e-6f) {
+            const float iz = 1.0f / d.z;
+            for (int si = 0; si < planar.n; si++) {
+                const float t = (planar.z[si] - o.z) * iz;
+                if (t >= 0.0f && t < best) {
+                    const float x = o.x + t * d.x, y = o.y + t * d.y;
+                    if (x >= planar.x0[si] && x <= planar.x1[si] && y >= planar.y0[si] && y <= planar.y1[si]) {
+                        best = t;
+                        tri = planar.tri[si];
+                    }
+                }
+            }
+        }
+        if (tri >= 0 && (!limited || best <= step)) {
+            res.found = 1;
+            res.distance = best;
+            res.tri = tri;
+            res.surface = bvh.tris[tri].surf;
+            res.normal = tri_normal(tri);
+        }
+        return res;
     }
 
     V3<R> tri_normal(int ti) const {
@@ -1407,11 +1498,19 @@ template <class R> struct Walker {
         }
         const auto& L = sc.layers[p.layer];
         Hit<R> h;
+        // the nearest surface: the BVH walk, or in the 32-bit mode of a planar scene the
+        // engine's analytic rule
+        auto nearest = [&](R step, bool limited) {
+            if constexpr (sizeof(R) == 4) {
+                if (sc.planar.n > 0) return sc.planar_hit(p.pos, p.dir, step, limited);
+            }
+            return sc.intersect(p.pos, p.dir, step, limited);
+        };
         if (L.glass) {
-            h = sc.intersect(p.pos, p.dir, step_unlimited, false);
+            h = nearest(step_unlimited, false);
         } else {
             p.step = step_size(L.mua, L.mus, rng);
-            h = sc.intersect(p.pos, p.dir, p.step, true);
+            h = nearest(p.step, true);
         }
         int v[3];
         if (h.found) {
@@ -1646,13 +1745,17 @@ int query_impl(const xmc_input* in, const double* o, const double* d, const doub
                int mode, int* found, double* dist, int* surf, int* tri, double* normal) {
     Scene<R> sc;
     sc.init(in);
+    if (mode < 0 || mode > 2 || (mode == 2 && (sizeof(R) != 4 || sc.planar.n == 0))) return XMC_ERR_ARG;
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < nr; ++i) {
         V3<R> oo = {(R)o[3 * i], (R)o[3 * i + 1], (R)o[3 * i + 2]};
         V3<R> dd = {(R)d[3 * i], (R)d[3 * i + 1], (R)d[3 * i + 2]};
         bool limited = steps[i] > 0.0;
         R st = limited ? (R)steps[i] : Prec<R>::kMaxDistance;
-        Hit<R> h = mode == 0 ? sc.intersect(oo, dd, st, limited) : sc.brute(oo, dd, st, limited);
+        Hit<R> h = mode == 1 ? sc.brute(oo, dd, st, limited) : sc.intersect(oo, dd, st, limited);
+        if constexpr (sizeof(R) == 4) {
+            if (mode == 2) h = sc.planar_hit(oo, dd, st, limited);
+        }
         found[i] = h.found;
         dist[i] = h.found ? (double)h.distance : 0.0;
         surf[i] = h.found ? h.surface : -1;

@@ -74,7 +74,9 @@ int xmco_bvh(const xmc_input* input, int* numberOfNodes, int* root, int* numberO
 
 /* Nearest-hit queries. mode 0 = BVH walk (ComputeBVHIntersection when
  * step > 0, ComputeBVHIntersectionWithoutStep when step <= 0), mode 1 = brute
- * force over every triangle with the same triangle test and tie rule.
+ * force over every triangle with the same triangle test and tie rule, mode 2
+ * (precision 32, planar scenes only) = the engine's analytic rule for scenes
+ * whose every surface is an axis-aligned rectangle at one z.
  * triangleId is the triangle's index inside its surface. */
 int xmco_query(const xmc_input* input, const double* origins, const double* directions,
                const double* steps, int numberOfRays, int precision, int mode, int* found,

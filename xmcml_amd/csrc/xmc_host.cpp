@@ -285,6 +285,62 @@ int xmc_scene_create(const xmc_input* in, const xmc_options* opt, xmc_scene** ou
     return scene_init(in, opt, true, out);
 }
 
+// The planar form of a scene (see Planar in xmc_params.h): every surface two
+// triangles over the four corners of an axis-aligned rectangle at one z,
+// sharing a diagonal.  Returns false (and n = 0) for any other scene.
+static bool planar_form(const xmc_input* in, const xmc::GpuBvh& gb, xmc::Planar& pl) {
+    std::memset(&pl, 0, sizeof(pl));
+    const int ns = in->numberOfSurfaces;
+    if (ns < 1 || ns > xmc::kPlanarMax) return false;
+    for (int s = 0; s < ns; s++) {
+        const xmc_surface& S = in->surface[s];
+        if (S.numberOfTriangles != 2) return false;
+        const xmc_int3 t[2] = {S.triangles[0], S.triangles[1]};
+        int used[6] = {t[0].x, t[0].y, t[0].z, t[1].x, t[1].y, t[1].z};
+        for (int k = 0; k < 6; k++)
+            if (used[k] < 0 || used[k] >= S.numberOfVertices) return false;
+        const xmc_double3& v0 = S.vertices[used[0]];
+        double xs[2] = {v0.x, v0.x}, ys[2] = {v0.y, v0.y};
+        for (int k = 0; k < 6; k++) {
+            const xmc_double3& v = S.vertices[used[k]];
+            if (v.z != v0.z) return false;
+            xs[0] = std::min(xs[0], v.x);
+            xs[1] = std::max(xs[1], v.x);
+            ys[0] = std::min(ys[0], v.y);
+            ys[1] = std::max(ys[1], v.y);
+        }
+        if (!(xs[0] < xs[1] && ys[0] < ys[1])) return false;
+        // each vertex a corner; each triangle three distinct corners; the shared two opposite
+        auto corner = [&](int i) {
+            const xmc_double3& v = S.vertices[i];
+            int cx = v.x == xs[0] ? 0 : (v.x == xs[1] ? 1 : -1);
+            int cy = v.y == ys[0] ? 0 : (v.y == ys[1] ? 1 : -1);
+            return (cx < 0 || cy < 0) ? -1 : cx + 2 * cy;
+        };
+        int c[6];
+        for (int k = 0; k < 6; k++)
+            if ((c[k] = corner(used[k])) < 0) return false;
+        int mask[2] = {0, 0};
+        for (int k = 0; k < 6; k++) mask[k / 3] |= 1 << c[k];
+        auto bits = [](int m) { return __builtin_popcount((unsigned)m); };
+        if (bits(mask[0]) != 3 || bits(mask[1]) != 3 || (mask[0] | mask[1]) != 15) return false;
+        const int shared = mask[0] & mask[1];
+        if (shared != (1 | 8) && shared != (2 | 4)) return false;  // corners 0-3 or 1-2: a diagonal
+        pl.z[s] = (float)v0.z;
+        pl.x0[s] = (float)xs[0];
+        pl.x1[s] = (float)xs[1];
+        pl.y0[s] = (float)ys[0];
+        pl.y1[s] = (float)ys[1];
+        pl.tri[s] = -1;
+    }
+    for (size_t k = 0; k < gb.tris.size(); k++) {
+        const int s = gb.tris[k].surface;
+        if (pl.tri[s] < 0) pl.tri[s] = (int)k;
+    }
+    pl.n = ns;
+    return true;
+}
+
 // Persistent launch shape and the stack spill it needs.  kernel_occupancy is
 // the occupancy query of the kernel that will run.
 static int configure_launch(int device, int depth, int block, int blocks_per_cu,
@@ -393,6 +449,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     P.nodes = sc->nodes.p;
     P.tris = sc->tris.p;
     P.normals = sc->normals.p;
+    planar_form(in, gb, P.planar);
     P.root_ref = gb.root_ref;
     for (int a = 0; a < 3; a++) {
         P.root_lo[a] = gb.root_lo[a];

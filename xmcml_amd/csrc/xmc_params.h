@@ -117,6 +117,17 @@ struct DistGrid {
     float slab_unit;
 };
 
+// Scenes whose every surface is an axis-aligned rectangle in a plane z = const
+// (two triangles sharing a diagonal, as the reference's plane generator makes
+// them): the step's nearest surface is found analytically instead of by a BVH
+// walk.  n = 0 for every other scene.
+constexpr int kPlanarMax = 16;
+struct Planar {
+    int n;
+    float z[kPlanarMax], x0[kPlanarMax], x1[kPlanarMax], y0[kPlanarMax], y1[kPlanarMax];
+    int tri[kPlanarMax];  // the surface's first triangle in BVH leaf order (its normal and surface id)
+};
+
 // A box of absorption voxels (around the beam entry, where most weight is
 // deposited) that each block accumulates in LDS and adds to the grid once at
 // the end; deposits elsewhere go straight to the grid.  count = 0: no tile.
@@ -177,6 +188,7 @@ struct Params {
     // and adds to the tallies once at its end (0: every detection goes
     // straight to the tallies); the time bins always go straight
     int det_lds;
+    Planar planar;
 };
 
 struct QueryParams {

@@ -494,6 +494,36 @@ __device__ __forceinline__ Hit walk(const Geo& g, const Ray& r, float step, bool
     return h;
 }
 
+// Float rule of the planar scenes (Planar in xmc_params.h): the nearest
+// surface is found analytically.  For |d.z| > EPSILON, surface s at distance
+// t = (z_s - p.z) * (1 / d.z) is hit when t >= 0 and (p.x + t d.x,
+// p.y + t d.y) lies in its rectangle (edges included); the smallest such t
+// wins (the first surface on ties), and a step-limited search reports it
+// when t <= step.  Parallel to the planes nothing is hit.  The hit reports
+// the surface's first triangle (its normal is (0, 0, +-1), whose sign the
+// boundary interaction does not use).
+__device__ __forceinline__ Hit planar_hit(const Planar& S, V3 p, V3 d, float step) {
+    Hit h;
+    h.found = false;
+    h.dist = kInf;
+    h.tri = -1;
+    if (fabsf(d.z) > kEpsF) {
+        const float iz = 1.0f / d.z;
+        for (int s = 0; s < S.n; ++s) {
+            const float t = (S.z[s] - p.z) * iz;
+            if (t >= 0.0f && t < h.dist) {
+                const float x = p.x + t * d.x, y = p.y + t * d.y;
+                if (x >= S.x0[s] && x <= S.x1[s] && y >= S.y0[s] && y <= S.y1[s]) {
+                    h.dist = t;
+                    h.tri = S.tri[s];
+                }
+            }
+        }
+    }
+    h.found = h.tri >= 0 && h.dist <= step;
+    return h;
+}
+
 // --------------------------------------------------------------------------
 // photon physics (mcml_kernel.cpp), float
 // --------------------------------------------------------------------------
@@ -1111,7 +1141,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                     phase = walk_begin(geo, ray, step, true, st, h, cur, best) ? kWalk : kResolve;
                 }
             } else {
-                h = walk(geo, ray, step, true, st);
+                h = C.planar.n > 0 ? planar_hit(C.planar, pos, dir, step) : walk(geo, ray, step, true, st);
                 phase = kResolve;
             }
         }
