@@ -329,6 +329,9 @@ def main(argv=None) -> int:
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-slab-grid", action="store_true",
                     help="build no separating slabs near the mesh (A/B of the walk skipping; same results)")
+    ap.add_argument("--no-lattice-rule", action="store_true",
+                    help="walk the BVH on the heightfield and slab workloads instead of testing the triangles of "
+                         "the lattice cells each step covers (the interleaved walk kernel; same hits)")
     ap.add_argument("--seed", type=int, default=777)
     ap.add_argument("--surrogate-model", type=Path, default=ROOT / "xmcml_amd" / "data" / "surrogate_layered.npz")
     ap.add_argument("--surrogate-queries", type=int, default=1 << 20)
@@ -361,7 +364,12 @@ def main(argv=None) -> int:
     absorption = args.workload == "slab"
     spec = make(args.photons)
     print(f"bench: building scene ({spec.num_triangles} triangles)", file=sys.stderr, flush=True)
-    sc = engine.Scene(spec, absorption=absorption, block_size=args.block, slab_grid=not args.no_slab_grid)
+    sc = engine.Scene(spec, absorption=absorption, block_size=args.block, slab_grid=not args.no_slab_grid,
+                      lattice_rule=not args.no_lattice_rule)
+    # the heightfield's surfaces are lattice heightfields and a rectangle, the slab's rectangles: both
+    # run the plain kernel under the surface rule; the head (closed meshes), and the heightfield
+    # without its lattice rule, run the interleaved walk kernel
+    walking = args.workload == "head" or (args.workload == "heightfield" and args.no_lattice_rule)
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib << 20)
         print(f"bench: calibration copy of {args.copy_calibration_mib} MiB done", file=sys.stderr, flush=True)
@@ -399,8 +407,11 @@ def main(argv=None) -> int:
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP32_TFLOPS,
         "frac_of_nonpacked_peak": achieved / PEAK_FP32_NONPACKED_TFLOPS,
-        "traffic": traffic_from_profiles(args.photons, args.workload),
-        "kernel": "xmc::transport_kernel<false, false, true>" if spec.num_triangles > 256 else "xmc::transport_kernel<false, false, false>",
+        "traffic": traffic_from_profiles(args.photons, args.workload + ("_walk" if walking and args.workload != "head"
+                                                                         else "")),
+        "kernel": "xmc::transport_kernel<false, false, true>" if walking else "xmc::transport_kernel<false, false, false>",
+        "nearest_surface": "BVH walk (interleaved, walk-skipping grids)" if walking else "surface rule (lattice cells / "
+                           "rectangles)",
         "kernel_ms_per_launch": launch_s * 1e3,
         "flop_per_launch": flop / args.steps,
         "scatter_events_per_photon": st["scatterEvents"] / photons,

@@ -261,6 +261,9 @@ typedef struct xmc_options {
                           * memory to build; results are the same either way) */
     int noSlabGrid;      /* 1 = no separating slabs near the mesh (steps that cannot cross the nearby surfaces
                           * skip their walk; results are the same either way) */
+    int noLatticeRule;   /* 1 = scenes of heightfields on a shared uniform lattice (and axis-aligned rectangles)
+                          * walk the BVH instead of testing the triangles of the lattice cells each step covers
+                          * (hits are the walk's but for float ties; scenes of rectangles only keep their rule) */
 } xmc_options;
 
 typedef struct xmc_scene xmc_scene;

@@ -203,6 +203,8 @@ template <class R> struct Prec;
 // walk draws (the same recorded sequence), for float-vs-double comparisons
 // that isolate arithmetic from uniform quantisation.
 static bool g_uniform24 = false;
+// the 32-bit mode's lattice rule (xmco_set_lattice_rule); rectangles keep theirs
+static bool g_lattice_rule = true;
 
 template <> struct Prec<double> {
     static constexpr double kCosZero = 1.0 - 1.0e-12;  // mcml_kernel.cpp:11
@@ -592,13 +594,21 @@ template <class R> struct Scene {
     std::vector<std::vector<double>> tables;
     std::vector<std::vector<float>> tables_f;  // the 32-bit mode's float copies
     std::vector<int> layer_table;
-    // the 32-bit mode's planar form (the engine's float rule for scenes whose every
-    // surface is an axis-aligned rectangle at one z: two triangles sharing a diagonal)
+    // the 32-bit mode's surface rule (the engine's float rule for scenes whose every
+    // surface is an axis-aligned rectangle at one z -- two triangles sharing a
+    // diagonal -- or a heightfield on one uniform lattice shared by all of them)
+    bool rule = false;
     struct PlanarForm {
         int n = 0;
         float z[16], x0[16], x1[16], y0[16], y1[16];
         int tri[16];
     } planar;
+    struct LatticeRule {
+        int ns = 0, nx = 0, ny = 0;
+        float x0 = 0, y0 = 0, inv_dx = 0, inv_dy = 0, mxy = 0, mz = 0, zlo = 0, zhi = 0;
+        std::vector<float> zmin, zmax;  // per (cell, heightfield)
+        std::vector<int> ta, tb;        // its triangles (bvh positions, lower first; -1 = none)
+    } lattice;
 
     // GetLocalAttractiveVector (:325-362), statement for statement
     V3<R> attractive(const V3<R>& pos, int l) const {
@@ -686,7 +696,7 @@ template <class R> struct Scene {
             rings.push_back(q);
         }
         if (input->useBiasing) init_bias(input);
-        if constexpr (sizeof(R) == 4) init_planar(input);
+        if constexpr (sizeof(R) == 4) init_rule(input);
         rsp = xmco_specular(input->layerInfo);
         w0 = (R)in_val(1.0 - rsp);
         min_weight = (R)in_val(input->minWeight);
@@ -748,52 +758,176 @@ template <class R> struct Scene {
                 if (tg[t] == input->layerInfo[l].anisotropy) layer_table[l] = (int)t;
     }
 
-    // the planar form: every surface two triangles over the four corners of an
-    // axis-aligned rectangle at one z, sharing a diagonal; n stays 0 otherwise
-    void init_planar(const xmc_input* input) {
-        planar.n = 0;
-        const int ns = input->numberOfSurfaces;
-        if (ns < 1 || ns > 16) return;
-        for (int si = 0; si < ns; si++) {
-            const xmc_surface& S = input->surface[si];
-            if (S.numberOfTriangles != 2) return;
-            const int idx[6] = {S.triangles[0].x, S.triangles[0].y, S.triangles[0].z,
-                                S.triangles[1].x, S.triangles[1].y, S.triangles[1].z};
-            for (int k = 0; k < 6; k++)
-                if (idx[k] < 0 || idx[k] >= S.numberOfVertices) return;
-            const double z0 = S.vertices[idx[0]].z;
-            double xlo = S.vertices[idx[0]].x, xhi = xlo, ylo = S.vertices[idx[0]].y, yhi = ylo;
-            for (int k = 0; k < 6; k++) {
-                const xmc_double3& v = S.vertices[idx[k]];
-                if (v.z != z0) return;
-                xlo = std::min(xlo, v.x);
-                xhi = std::max(xhi, v.x);
-                ylo = std::min(ylo, v.y);
-                yhi = std::max(yhi, v.y);
-            }
-            if (!(xlo < xhi && ylo < yhi)) return;
-            int cover[2] = {0, 0};
-            for (int k = 0; k < 6; k++) {
-                const xmc_double3& v = S.vertices[idx[k]];
-                const int cx = v.x == xlo ? 0 : (v.x == xhi ? 1 : -1);
-                const int cy = v.y == ylo ? 0 : (v.y == yhi ? 1 : -1);
-                if (cx < 0 || cy < 0) return;
-                cover[k / 3] |= 1 << (cx + 2 * cy);
-            }
-            auto count = [](int m) { int c = 0; for (int b = 0; b < 4; b++) c += (m >> b) & 1; return c; };
-            if (count(cover[0]) != 3 || count(cover[1]) != 3 || (cover[0] | cover[1]) != 15) return;
-            const int shared = cover[0] & cover[1];
-            if (shared != 9 && shared != 6) return;
-            planar.z[si] = (float)z0;
-            planar.x0[si] = (float)xlo;
-            planar.x1[si] = (float)xhi;
-            planar.y0[si] = (float)ylo;
-            planar.y1[si] = (float)yhi;
-            planar.tri[si] = -1;
+    // a rectangle: two triangles over the four corners of an axis-aligned
+    // rectangle at one z, sharing a diagonal (z, x0, x1, y0, y1 in float)
+    static bool rect_form(const xmc_surface& S, float out[5]) {
+        if (S.numberOfTriangles != 2) return false;
+        const int idx[6] = {S.triangles[0].x, S.triangles[0].y, S.triangles[0].z,
+                            S.triangles[1].x, S.triangles[1].y, S.triangles[1].z};
+        for (int k = 0; k < 6; k++)
+            if (idx[k] < 0 || idx[k] >= S.numberOfVertices) return false;
+        const double z0 = S.vertices[idx[0]].z;
+        double xlo = S.vertices[idx[0]].x, xhi = xlo, ylo = S.vertices[idx[0]].y, yhi = ylo;
+        for (int k = 0; k < 6; k++) {
+            const xmc_double3& v = S.vertices[idx[k]];
+            if (v.z != z0) return false;
+            xlo = std::min(xlo, v.x);
+            xhi = std::max(xhi, v.x);
+            ylo = std::min(ylo, v.y);
+            yhi = std::max(yhi, v.y);
         }
-        for (size_t k = 0; k < bvh.tris.size(); k++)
-            if (planar.tri[bvh.tris[k].surf] < 0) planar.tri[bvh.tris[k].surf] = (int)k;
-        planar.n = ns;
+        if (!(xlo < xhi && ylo < yhi)) return false;
+        int cover[2] = {0, 0};
+        for (int k = 0; k < 6; k++) {
+            const xmc_double3& v = S.vertices[idx[k]];
+            const int cx = v.x == xlo ? 0 : (v.x == xhi ? 1 : -1);
+            const int cy = v.y == ylo ? 0 : (v.y == yhi ? 1 : -1);
+            if (cx < 0 || cy < 0) return false;
+            cover[k / 3] |= 1 << (cx + 2 * cy);
+        }
+        auto count = [](int m) { int c = 0; for (int b = 0; b < 4; b++) c += (m >> b) & 1; return c; };
+        if (count(cover[0]) != 3 || count(cover[1]) != 3 || (cover[0] | cover[1]) != 15) return false;
+        const int shared = cover[0] & cover[1];
+        if (shared != 9 && shared != 6) return false;
+        out[0] = (float)z0;
+        out[1] = (float)xlo;
+        out[2] = (float)xhi;
+        out[3] = (float)ylo;
+        out[4] = (float)yhi;
+        return true;
+    }
+
+    // the distinct values of a lattice axis, uniformly spaced within 1e-6 of a step
+    static bool lattice_axis(std::vector<double>& v, double& origin, double& step) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        if (v.size() < 2 || v.size() > 65537) return false;
+        const int n = (int)v.size() - 1;
+        origin = v.front();
+        step = (v.back() - v.front()) / n;
+        if (!(step > 0.0) || !std::isfinite(step)) return false;
+        for (int i = 0; i <= n; i++)
+            if (std::fabs(v[i] - (origin + i * step)) > 1e-6 * step) return false;
+        return true;
+    }
+
+    // the heightfields' lattice: every triangle spans one cell along x and y, at
+    // most two per (cell, heightfield); margins as the engine states them
+    bool init_lattice(const xmc_input* input, const std::vector<int>& hs) {
+        const xmc_surface* S = input->surface;
+        std::vector<double> xs, ys;
+        double axy = 0.0, az = 0.0;
+        for (int s : hs)
+            for (int t = 0; t < S[s].numberOfTriangles; t++) {
+                const int id[3] = {S[s].triangles[t].x, S[s].triangles[t].y, S[s].triangles[t].z};
+                for (int k = 0; k < 3; k++) {
+                    if (id[k] < 0 || id[k] >= S[s].numberOfVertices) return false;
+                    const xmc_double3& v = S[s].vertices[id[k]];
+                    if (!std::isfinite(v.x) || !std::isfinite(v.y) || !std::isfinite(v.z)) return false;
+                    xs.push_back(v.x);
+                    ys.push_back(v.y);
+                    axy = std::max(axy, std::max(std::fabs(v.x), std::fabs(v.y)));
+                    az = std::max(az, std::fabs(v.z));
+                }
+            }
+        double x0, y0, dx, dy;
+        if (!lattice_axis(xs, x0, dx) || !lattice_axis(ys, y0, dy)) return false;
+        LatticeRule& L = lattice;
+        L.nx = (int)xs.size() - 1;
+        L.ny = (int)ys.size() - 1;
+        L.ns = (int)hs.size();
+        if ((double)L.nx * L.ny * L.ns > (double)(1 << 26)) return false;
+        const size_t nrec = (size_t)L.nx * L.ny * L.ns;
+        const float inf = std::numeric_limits<float>::infinity();
+        L.zmin.assign(nrec, inf);
+        L.zmax.assign(nrec, -inf);
+        L.ta.assign(nrec, -1);
+        L.tb.assign(nrec, -1);
+        std::vector<int> first(input->numberOfSurfaces + 1, 0);
+        for (int s = 0; s < input->numberOfSurfaces; s++) first[s + 1] = first[s] + S[s].numberOfTriangles;
+        std::vector<int> pos(first.back(), -1);
+        for (size_t i = 0; i < bvh.tris.size(); i++) pos[first[bvh.tris[i].surf] + bvh.tris[i].index] = (int)i;
+        float zlo = inf, zhi = -inf;
+        for (int l = 0; l < L.ns; l++) {
+            const int s = hs[l];
+            for (int t = 0; t < S[s].numberOfTriangles; t++) {
+                const int id[3] = {S[s].triangles[t].x, S[s].triangles[t].y, S[s].triangles[t].z};
+                int ix[3], iy[3];
+                float lo = inf, hi = -inf;
+                for (int k = 0; k < 3; k++) {
+                    const xmc_double3& v = S[s].vertices[id[k]];
+                    ix[k] = (int)std::lround((v.x - x0) / dx);
+                    iy[k] = (int)std::lround((v.y - y0) / dy);
+                    lo = std::min(lo, (float)v.z);
+                    hi = std::max(hi, (float)v.z);
+                }
+                const int cx = std::min(ix[0], std::min(ix[1], ix[2])), cy = std::min(iy[0], std::min(iy[1], iy[2]));
+                if (std::max(ix[0], std::max(ix[1], ix[2])) != cx + 1 || std::max(iy[0], std::max(iy[1], iy[2])) != cy + 1)
+                    return false;
+                const size_t r = ((size_t)cy * L.nx + cx) * L.ns + l;
+                const int p = pos[first[s] + t];
+                if (L.tb[r] >= 0) return false;
+                if (L.ta[r] < 0) {
+                    L.ta[r] = p;
+                } else {
+                    L.tb[r] = std::max(L.ta[r], p);
+                    L.ta[r] = std::min(L.ta[r], p);
+                }
+                L.zmin[r] = std::min(L.zmin[r], lo);
+                L.zmax[r] = std::max(L.zmax[r], hi);
+                zlo = std::min(zlo, lo);
+                zhi = std::max(zhi, hi);
+            }
+        }
+        const double ulp = std::ldexp(1.0, -22);
+        L.x0 = (float)x0;
+        L.y0 = (float)y0;
+        L.inv_dx = (float)(1.0 / dx);
+        L.inv_dy = (float)(1.0 / dy);
+        L.mxy = (float)(1e-3 * std::min(dx, dy) + 4.0 * std::max(L.nx, L.ny) * ulp * std::max(dx, dy) + 8.0 * ulp * axy);
+        L.mz = (float)(1e-4 * (1.0 + ((double)zhi - (double)zlo)) + 8.0 * ulp * az);
+        L.zlo = zlo;
+        L.zhi = zhi;
+        return true;
+    }
+
+    // the surface rule: rectangles in surface order (each with its first triangle
+    // in bvh order) and the other surfaces as heightfields on one lattice; off
+    // (rule false, both forms empty) for any other scene, and for scenes with
+    // heightfields while the lattice rule is switched off
+    void init_rule(const xmc_input* input) {
+        rule = false;
+        planar = PlanarForm();
+        lattice = LatticeRule();
+        const int ns = input->numberOfSurfaces;
+        if (ns < 1) return;
+        std::vector<int> rect_of(ns, -1), hs;
+        for (int si = 0; si < ns; si++) {
+            float f[5];
+            if (rect_form(input->surface[si], f)) {
+                if (planar.n == 16) { planar = PlanarForm(); return; }
+                rect_of[si] = planar.n;
+                planar.z[planar.n] = f[0];
+                planar.x0[planar.n] = f[1];
+                planar.x1[planar.n] = f[2];
+                planar.y0[planar.n] = f[3];
+                planar.y1[planar.n] = f[4];
+                planar.tri[planar.n] = -1;
+                planar.n++;
+            } else {
+                hs.push_back(si);
+            }
+        }
+        if (!hs.empty() && (!g_lattice_rule || hs.size() > 8 || !init_lattice(input, hs))) {
+            planar = PlanarForm();
+            lattice = LatticeRule();
+            return;
+        }
+        for (size_t k = 0; k < bvh.tris.size(); k++) {
+            const int r = rect_of[bvh.tris[k].surf];
+            if (r >= 0 && planar.tri[r] < 0) planar.tri[r] = (int)k;
+        }
+        rule = true;
     }
 
     // the 32-bit mode's nearest surface of a planar scene: t = (z_s - o.z) * (1 / d.z)
@@ -822,6 +956,78 @@ template <class R> struct Scene {
             }
         }
         if (tri >= 0 && (!limited || best <= step)) {
+            res.found = 1;
+            res.distance = best;
+            res.tri = tri;
+            res.surface = bvh.tris[tri].surf;
+            res.normal = tri_normal(tri);
+        }
+        return res;
+    }
+
+    // the 32-bit mode's surface rule: the rectangles' nearest hit as planar_hit
+    // finds it; then, for a limited step, over the lattice cells of the step's xy
+    // bounding box widened by mxy, per heightfield whose cell z-range meets the
+    // step's z-range widened by mz, the cell's triangles with the BVH leaves' test:
+    // a nearer one, or one as near with a lower bvh position, replaces the hit.
+    // Unlimited steps, and boxes of more than 16 cells, take the BVH walk over
+    // every surface.
+    Hit<float> rule_hit(const V3<float>& o, const V3<float>& d, float step, bool limited) const {
+        const LatticeRule& L = lattice;
+        if (L.ns == 0) return planar_hit(o, d, step, limited);
+        if (!limited) return intersect(o, d, step, false);
+        float best = std::numeric_limits<float>::infinity();
+        int tri = -1;
+        if (std::fabs(d.z) > 1.0e-6f) {
+            const float iz = 1.0f / d.z;
+            for (int si = 0; si < planar.n; si++) {
+                const float t = (planar.z[si] - o.z) * iz;
+                if (t >= 0.0f && t < best) {
+                    const float x = o.x + t * d.x, y = o.y + t * d.y;
+                    if (x >= planar.x0[si] && x <= planar.x1[si] && y >= planar.y0[si] && y <= planar.y1[si]) {
+                        best = t;
+                        tri = planar.tri[si];
+                    }
+                }
+            }
+        }
+        auto search = [&]() -> bool {  // false: the box is too large for the lattice
+            const float ez = std::fmaf(step, d.z, o.z);
+            const float az = std::fmin(o.z, ez) - L.mz, bz = std::fmax(o.z, ez) + L.mz;
+            if (bz < L.zlo || az > L.zhi) return true;
+            const float ex = std::fmaf(step, d.x, o.x), ey = std::fmaf(step, d.y, o.y);
+            const int i0 = std::max(0, (int)std::floor(std::fmax((std::fmin(o.x, ex) - L.mxy - L.x0) * L.inv_dx, -1.0f)));
+            const int i1 = std::min(L.nx - 1, (int)std::floor(std::fmin((std::fmax(o.x, ex) + L.mxy - L.x0) * L.inv_dx,
+                                                                        (float)L.nx)));
+            const int j0 = std::max(0, (int)std::floor(std::fmax((std::fmin(o.y, ey) - L.mxy - L.y0) * L.inv_dy, -1.0f)));
+            const int j1 = std::min(L.ny - 1, (int)std::floor(std::fmin((std::fmax(o.y, ey) + L.mxy - L.y0) * L.inv_dy,
+                                                                        (float)L.ny)));
+            if (i0 > i1 || j0 > j1) return true;
+            if ((i1 - i0 + 1) * (j1 - j0 + 1) > 16) return false;
+            for (int j = j0; j <= j1; j++)
+                for (int i = i0; i <= i1; i++)
+                    for (int l = 0; l < L.ns; l++) {
+                        const size_t r = ((size_t)j * L.nx + i) * L.ns + l;
+                        if (!(L.zmin[r] <= bz && L.zmax[r] >= az)) continue;
+                        for (int k : {L.ta[r], L.tb[r]}) {
+                            if (k < 0) continue;
+                            const float t = tri_t(k, o, d);
+                            if (t >= 0.0f && (t < best || (t == best && k < tri))) {
+                                best = t;
+                                tri = k;
+                            }
+                        }
+                    }
+            return true;
+        };
+        if (!search()) return intersect(o, d, step, true);
+        Hit<float> res;
+        res.found = 0;
+        res.distance = Prec<float>::kMaxDistance;
+        res.surface = -1;
+        res.tri = -1;
+        res.normal = {0, 0, 0};
+        if (tri >= 0 && best <= step) {
             res.found = 1;
             res.distance = best;
             res.tri = tri;
@@ -1498,11 +1704,11 @@ template <class R> struct Walker {
         }
         const auto& L = sc.layers[p.layer];
         Hit<R> h;
-        // the nearest surface: the BVH walk, or in the 32-bit mode of a planar scene the
-        // engine's analytic rule
+        // the nearest surface: the BVH walk, or in the 32-bit mode of a scene of
+        // rectangles and lattice heightfields the engine's surface rule
         auto nearest = [&](R step, bool limited) {
             if constexpr (sizeof(R) == 4) {
-                if (sc.planar.n > 0) return sc.planar_hit(p.pos, p.dir, step, limited);
+                if (sc.rule) return sc.rule_hit(p.pos, p.dir, step, limited);
             }
             return sc.intersect(p.pos, p.dir, step, limited);
         };
@@ -1745,7 +1951,7 @@ int query_impl(const xmc_input* in, const double* o, const double* d, const doub
                int mode, int* found, double* dist, int* surf, int* tri, double* normal) {
     Scene<R> sc;
     sc.init(in);
-    if (mode < 0 || mode > 2 || (mode == 2 && (sizeof(R) != 4 || sc.planar.n == 0))) return XMC_ERR_ARG;
+    if (mode < 0 || mode > 2 || (mode == 2 && (sizeof(R) != 4 || !sc.rule))) return XMC_ERR_ARG;
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < nr; ++i) {
         V3<R> oo = {(R)o[3 * i], (R)o[3 * i + 1], (R)o[3 * i + 2]};
@@ -1754,7 +1960,7 @@ int query_impl(const xmc_input* in, const double* o, const double* d, const doub
         R st = limited ? (R)steps[i] : Prec<R>::kMaxDistance;
         Hit<R> h = mode == 1 ? sc.brute(oo, dd, st, limited) : sc.intersect(oo, dd, st, limited);
         if constexpr (sizeof(R) == 4) {
-            if (mode == 2) h = sc.planar_hit(oo, dd, st, limited);
+            if (mode == 2) h = sc.rule_hit(oo, dd, st, limited);
         }
         found[i] = h.found;
         dist[i] = h.found ? (double)h.distance : 0.0;
@@ -1850,6 +2056,8 @@ int xmco_bvh(const xmc_input* input, int* nn, int* root, int* nt, double* boxes,
     }
     return XMC_OK;
 }
+
+void xmco_set_lattice_rule(int on) { g_lattice_rule = on != 0; }
 
 int xmco_query(const xmc_input* input, const double* o, const double* d, const double* steps, int nr,
                int precision, int mode, int* found, double* dist, int* surf, int* tri, double* normal) {

@@ -75,9 +75,16 @@ int xmco_bvh(const xmc_input* input, int* numberOfNodes, int* root, int* numberO
 /* Nearest-hit queries. mode 0 = BVH walk (ComputeBVHIntersection when
  * step > 0, ComputeBVHIntersectionWithoutStep when step <= 0), mode 1 = brute
  * force over every triangle with the same triangle test and tie rule, mode 2
- * (precision 32, planar scenes only) = the engine's analytic rule for scenes
- * whose every surface is an axis-aligned rectangle at one z.
+ * (precision 32, scenes under the surface rule only) = the engine's surface
+ * rule for scenes whose every surface is an axis-aligned rectangle at one z or
+ * a heightfield on a uniform lattice shared by all of them.
  * triangleId is the triangle's index inside its surface. */
+/* Switches the 32-bit mode's lattice rule on (default) or off for the calls
+ * that follow (process-wide): off, scenes with heightfields walk the BVH, as
+ * the engine does with xmc_options.noLatticeRule; scenes of rectangles only
+ * keep their rule either way. */
+void xmco_set_lattice_rule(int on);
+
 int xmco_query(const xmc_input* input, const double* origins, const double* directions,
                const double* steps, int numberOfRays, int precision, int mode, int* found,
                double* distance, int* surfaceId, int* triangleId, double* normal);

@@ -64,6 +64,7 @@ def oracle():
         _sig(lib, "xmco_bvh", I, [P, PI, PI, PI, PD, PI, PI, PI])
         _sig(lib, "xmco_shadow_trace", I, [P, U64, U64, I, P, I, P, PI])
         _sig(lib, "xmco_query", I, [P, PD, PD, PD, I, I, I, PI, PD, PI, PI, PD])
+        _sig(lib, "xmco_set_lattice_rule", None, [I])
         _sig(lib, "xmco_mcg59_init", U64, [U64, C.c_uint32, C.c_uint32, PU64])
         _sig(lib, "xmco_weight_integral", D, [D, D, D, C.c_uint])
         _sig(lib, "xmco_expf_many", None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64])
@@ -89,6 +90,22 @@ def oracle():
         _sig(lib, "xmco_sincos2pi_many", None, [P, P, P, C.c_int64])
         _oracle = lib
     return _oracle
+
+
+class lattice_rule:
+    """Context manager: the 32-bit oracle's lattice rule on or off inside it
+    (as engine.Scene(lattice_rule=...) sets the engine's), on again after."""
+
+    def __init__(self, on: bool):
+        self.on = on
+
+    def __enter__(self):
+        oracle().xmco_set_lattice_rule(int(self.on))
+        return self
+
+    def __exit__(self, *exc):
+        oracle().xmco_set_lattice_rule(1)
+        return False
 
 
 def ref_available() -> bool:

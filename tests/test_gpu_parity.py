@@ -27,7 +27,19 @@ SMALL = {
     "glass": lambda n: scenes.small_scene("glass", n),
     "mismatch": lambda n: scenes.small_scene("mismatch", n),
     "heightfield": lambda n: scenes.small_scene("heightfield", n),
+    "heightfield-walk": lambda n: scenes.small_scene("heightfield", n),
 }
+# the heightfield with its lattice rule switched off on both sides: the
+# interleaved kernel's BVH walks with the walk-skipping grids
+WALK = {"heightfield-walk"}
+
+
+def opts(name):
+    return dict(lattice_rule=False) if name in WALK else {}
+
+
+def fixture_name(name):
+    return "heightfield" if name in WALK else name
 
 
 def _fields_equal(a, b):
@@ -86,11 +98,12 @@ def test_bvh_hits_bit_exact_vs_brute_force(engine):
 def test_traces_bit_identical_to_float_oracle(engine, name):
     spec = SMALL[name](64)
     inp = InputHandle(spec)
-    with engine.Scene(spec) as sc:
+    with engine.Scene(spec, **opts(name)) as sc:
         traces, counts = sc.trace(2024, 0, 64, max_events=20000)
     assert counts.max() < 20000
-    bad = [p for p in range(64) if not _fields_equal(traces[p], ol.trace(ol.oracle().xmco_trace, inp, 2024, p,
-                                                                          20000, precision=32))]
+    with ol.lattice_rule(name not in WALK):
+        bad = [p for p in range(64) if not _fields_equal(traces[p], ol.trace(ol.oracle().xmco_trace, inp, 2024, p,
+                                                                              20000, precision=32))]
     assert not bad, f"photons whose GPU trace differs from the float oracle: {bad[:10]}"
 
 
@@ -99,12 +112,15 @@ def test_tallies_match_float_oracle(engine, name):
     n = 4000
     spec = SMALL[name](n)
     inp = InputHandle(spec)
-    with engine.Scene(spec) as sc:
+    with engine.Scene(spec, **opts(name)) as sc:
         sc.run(5, 100, n)
         g = sc.collect()
         st = sc.stats()
     o = OutputHandle(spec)
-    c = ol.run_oracle(spec, inp, o, 5, 100, n, 32, 8)
+    with ol.lattice_rule(name not in WALK):
+        c = ol.run_oracle(spec, inp, o, 5, 100, n, 32, 8)
+    if name in WALK:
+        assert st["culledWalks"] > 0 and st["entryWalks"] > 0
     assert st["photons"] == n
     assert st["scatterEvents"] == c.scatterEvents
     assert st["boundaryEvents"] == c.boundaryEvents
@@ -138,10 +154,10 @@ def test_traces_match_the_reference_on_the_same_uniforms(engine, name):
     z = np.load(TAPE_FIXTURE, allow_pickle=False)
     seed, nphot = int(z["seed"]), int(z["photons"])
     spec = SMALL[name](64)
-    with engine.Scene(spec) as sc:
+    with engine.Scene(spec, **opts(name)) as sc:
         traces, _ = sc.trace(seed, 0, nphot, max_events=200000)
     for p in range(nphot):
-        r = ol.compare_walks(z[f"{name}_{p}"], traces[p])
+        r = ol.compare_walks(z[f"{fixture_name(name)}_{p}"], traces[p])
         assert r["full"], (p, r)
         assert r["weight"] < 1e-5 and r["early"] < 2e-5 and r["whole"] < 1e-3, (p, r)
 
@@ -155,7 +171,7 @@ def test_every_gpu_step_matches_double(engine, name):
     nphot = 64
     spec = SMALL[name](nphot)
     inp = InputHandle(spec)
-    with engine.Scene(spec) as sc:
+    with engine.Scene(spec, **opts(name)) as sc:
         traces, _ = sc.trace(4242, 0, nphot, max_events=50000)
     worst = 0.0
     for p in range(nphot):
@@ -202,7 +218,7 @@ def test_statistics_within_three_sigma(engine):
 def test_weight_ledger_closes(engine, name):
     n = 20000
     spec = SMALL[name](n)
-    with engine.Scene(spec) as sc:
+    with engine.Scene(spec, **opts(name)) as sc:
         sc.run(3, 0, n)
         g = sc.collect()
         st = sc.stats()
@@ -252,8 +268,8 @@ def test_walk_skipping_grids_change_nothing(engine, grids):
     n = 6000
     spec = scenes.small_scene("heightfield", n)
     res = []
-    for opts in (dict(distance_grid=False, entry_grid=False), dict(distance_grid=dist, entry_grid=entry)):
-        with engine.Scene(spec, **opts) as sc:
+    for o in (dict(distance_grid=False, entry_grid=False), dict(distance_grid=dist, entry_grid=entry)):
+        with engine.Scene(spec, lattice_rule=False, **o) as sc:
             sc.run(31, 0, n)
             out = sc.collect()
             res.append((out, sc.stats()))
@@ -279,8 +295,8 @@ def test_entry_trees_change_nothing_on_the_full_heightfield(engine):
     n = 20_000
     spec = scenes.heightfield_scene(n)
     res = []
-    for opts in (dict(distance_grid=False, entry_grid=False), dict()):
-        with engine.Scene(spec, **opts) as sc:
+    for o in (dict(distance_grid=False, entry_grid=False), dict()):
+        with engine.Scene(spec, lattice_rule=False, **o) as sc:
             traces, counts = sc.trace(8, 0, 64, max_events=20000)
             assert counts.max() < 20000
             sc.run(9, 0, n)
@@ -310,7 +326,7 @@ def test_separating_slabs_change_nothing(engine, kind):
     print(f"slabs on/off: {kind}, {spec.num_triangles} triangles", flush=True)
     res = []
     for slabs in (False, True):
-        with engine.Scene(spec, absorption=False, slab_grid=slabs) as sc:
+        with engine.Scene(spec, absorption=False, slab_grid=slabs, lattice_rule=False) as sc:
             traces = None
             if kind == "heightfield":
                 traces, counts = sc.trace(41, 0, 48, max_events=20000)

@@ -15,6 +15,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+import oracle_lib as ol
 from xmcml_amd import scenes
 from xmcml_amd.abi import InputHandle
 
@@ -59,7 +60,8 @@ def test_kernel_walk_on_host_equals_brute_force(tmp_path, mode):
     so = tmp_path / "walk_host.so"
     subprocess.run([CLANG, *flags, "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
                     f"-I{ROOT / 'xmcml_amd' / 'csrc'}", f"-I{ROOT / 'include'}", str(tmp_path / "walk_host.cpp"),
-                    str(ROOT / "xmcml_amd" / "csrc" / "xmc_bvh.cpp"), "-o", str(so)], check=True)
+                    str(ROOT / "xmcml_amd" / "csrc" / "xmc_bvh.cpp"), str(ROOT / "xmcml_amd" / "csrc" / "xmc_lattice.cpp"),
+                    "-o", str(so)], check=True)
     env = {"ASAN_OPTIONS": "detect_leaks=0:halt_on_error=1", "UBSAN_OPTIONS": "halt_on_error=1:print_stacktrace=1",
            "PATH": "/usr/bin:/bin"}
     if mode == "sanitizers":
@@ -71,3 +73,80 @@ def test_kernel_walk_on_host_equals_brute_force(tmp_path, mode):
     rows = [list(map(int, l.split())) for l in r.stdout.strip().split("\n")]
     for found, bad_walk, bad_visit in rows:
         assert found > 1000 and bad_walk == 0 and bad_visit == 0, rows
+
+
+def lattice_scenes():
+    """Heightfield scenes under the surface rule: the 98,306-triangle
+    heightfield (three lattice surfaces and a rectangle), the small one, and
+    the small one with a hole (cells with one or no triangle)."""
+    holey = scenes.small_scene("heightfield", 10)
+    v, t = holey.surfaces[1]
+    keep = np.ones(len(t), bool)
+    keep[100:160] = False  # thirty cells without triangles
+    keep[301:340:2] = False  # and twenty with one
+    holey.surfaces[1] = (v, t[keep])
+    return {"heightfield": (scenes.heightfield_scene(10), 20.5),
+            "small": (scenes.small_scene("heightfield", 10), 5.5),
+            "holey": (holey, 5.5)}
+
+
+@pytest.fixture(scope="module")
+def rule_lib(tmp_path_factory):
+    tmp = tmp_path_factory.mktemp("rule")
+    (tmp / "walk_region.inc").write_text(walk_region())
+    shutil.copy(ROOT / "tests" / "walk_host.cpp", tmp / "walk_host.cpp")
+    so = tmp / "rule_host.so"
+    subprocess.run([CLANG, "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                    f"-I{ROOT / 'xmcml_amd' / 'csrc'}", f"-I{ROOT / 'include'}", str(tmp / "walk_host.cpp"),
+                    str(ROOT / "xmcml_amd" / "csrc" / "xmc_bvh.cpp"), str(ROOT / "xmcml_amd" / "csrc" / "xmc_lattice.cpp"),
+                    "-o", str(so)], check=True)
+    lib = C.CDLL(str(so))
+    lib.rule_query.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 3 + [C.c_int] + [C.c_void_p] * 4
+    return lib
+
+
+def oracle_rule(spec, o, d, s):
+    inp = InputHandle(spec)
+    n = len(s)
+    found = np.zeros(n, np.int32)
+    dist = np.zeros(n)
+    surf = np.zeros(n, np.int32)
+    tri = np.zeros(n, np.int32)
+    nrm = np.zeros(3 * n)
+    rc = ol.oracle().xmco_query(inp.ptr, ol.dptr(o), ol.dptr(d), ol.dptr(s), n, 32, 2, ol.iptr(found),
+                                ol.dptr(dist), ol.iptr(surf), ol.iptr(tri), ol.dptr(nrm))
+    return rc, found, dist, surf, tri
+
+
+@pytest.mark.parametrize("kind", ["heightfield", "small", "holey"])
+def test_kernel_surface_rule_on_host_equals_oracle(rule_lib, kind):
+    """The kernel's rule_hit (its own source, built for the host, with the
+    engine's lattice builder) against the 32-bit oracle's surface rule on
+    random rays through the layers: bit-identical hits, distances and
+    triangles, for steps inside one cell, across up to 4 x 4 cells, longer
+    (the walk), and unlimited."""
+    spec, half = lattice_scenes()[kind]
+    rng = np.random.default_rng(17)
+    n = 40000
+    o = np.stack([rng.uniform(-half, half, n), rng.uniform(-half, half, n), rng.uniform(-1.6, 3.6, n)], 1)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = o.astype(np.float32).astype(np.float64)
+    d = d.astype(np.float32).astype(np.float64)
+    s = rng.exponential(0.2, n) * np.where(rng.random(n) < 0.1, 10.0, 1.0)
+    s[::7] = 0.0  # unlimited
+    inp = InputHandle(spec)
+    found = np.zeros(n, np.int32)
+    dist = np.zeros(n)
+    surf = np.zeros(n, np.int32)
+    idx = np.zeros(n, np.int32)
+    assert rule_lib.rule_query(C.cast(inp.c.surface, C.c_void_p), inp.c.numberOfSurfaces, o.ctypes.data,
+                               d.ctypes.data, s.ctypes.data, n, found.ctypes.data, dist.ctypes.data,
+                               surf.ctypes.data, idx.ctypes.data) == 0
+    rc, f2, d2, s2, t2 = oracle_rule(spec, o, d, s)
+    assert rc == 0
+    assert found.sum() > 3000
+    assert np.array_equal(found, f2)
+    assert np.array_equal(dist, d2)
+    assert np.array_equal(surf, s2)
+    assert np.array_equal(idx, t2)

@@ -25,8 +25,11 @@ inline int __float_as_int(float f) { int i; std::memcpy(&i, &f, 4); return i; }
 inline float __int_as_float(int i) { float f; std::memcpy(&f, &i, 4); return f; }
 inline uint32_t __float_as_uint(float f) { uint32_t i; std::memcpy(&i, &f, 4); return i; }
 inline float __uint_as_float(uint32_t i) { float f; std::memcpy(&f, &i, 4); return f; }
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
 #include "xmc_params.h"
 #include "xmc_bvh.h"
+#include "xmc_lattice.h"
 namespace xmc {
 #include "walk_region.inc"
 // the whole walk as a loop over walk_visit (the form the interleaved transport
@@ -80,5 +83,77 @@ extern "C" int walk_compare(const xmc_surface* surf, int ns, const double* o, co
     stats[0] = found;
     stats[1] = bad_walk;
     stats[2] = bad_visit;
+    return 0;
+}
+
+// The transport's surface rule (rule_hit) for every ray: the scene's forms
+// built as the engine builds them (xmc_lattice.cpp), found / distance /
+// surface / triangle index within the surface per ray.  Returns 1 when the
+// scene is not under the rule.
+extern "C" int rule_query(const xmc_surface* surf, int ns, const double* o, const double* d, const double* steps,
+                          int n, int* found, double* dist, int* surface, int* index) {
+    RefBvh ref;
+    const char* err = nullptr;
+    if (!build_reference_bvh(surf, ns, ref, &err)) return -1;
+    GpuBvh gb;
+    flatten_bvh(ref, surf, gb);
+    std::vector<int> base(ns + 1, 0);
+    for (int s = 0; s < ns; s++) base[s + 1] = base[s] + surf[s].numberOfTriangles;
+    std::vector<int> leaf_of(base[ns]);
+    for (size_t k = 0; k < gb.tris.size(); k++) leaf_of[base[ref.tri_surface[k]] + ref.tri_index[k]] = (int)k;
+    static Params P;
+    std::memset(&P, 0, sizeof(P));
+    std::vector<int> rect_of(ns, -1), heights;
+    for (int s = 0; s < ns; s++) {
+        RectForm r;
+        if (planar_rectangle(surf[s], r)) {
+            const int k = P.planar.n++;
+            rect_of[s] = k;
+            P.planar.z[k] = r.z;
+            P.planar.x0[k] = r.x0;
+            P.planar.x1[k] = r.x1;
+            P.planar.y0[k] = r.y0;
+            P.planar.y1[k] = r.y1;
+            P.planar.tri[k] = -1;
+        } else {
+            heights.push_back(s);
+        }
+    }
+    LatticeForm lf;
+    if (!heights.empty() && !lattice_form(surf, ns, heights, leaf_of, lf)) return 1;
+    for (size_t k = 0; k < gb.tris.size(); k++) {
+        const int r = rect_of[gb.tris[k].surface];
+        if (r >= 0 && P.planar.tri[r] < 0) P.planar.tri[r] = (int)k;
+    }
+    P.surface_rule = 1;
+    Lattice& L = P.lattice;
+    L.ns = lf.ns;
+    L.nx = lf.nx;
+    L.ny = lf.ny;
+    L.x0 = lf.x0;
+    L.y0 = lf.y0;
+    L.inv_dx = lf.inv_dx;
+    L.inv_dy = lf.inv_dy;
+    L.mxy = lf.mxy;
+    L.mz = lf.mz;
+    L.zlo = lf.zlo;
+    L.zhi = lf.zhi;
+    L.cells = reinterpret_cast<const float4*>(lf.cells.data());
+    int depth = gb.depth + 4;
+    std::vector<int2> lds(kStackLds), spill(depth > 0 ? depth : 1);
+    Geo geo{gb.nodes.data(), gb.tris.data(), gb.root_ref, {gb.root_lo[0], gb.root_lo[1], gb.root_lo[2]},
+            {gb.root_hi[0], gb.root_hi[1], gb.root_hi[2]}};
+    for (int i = 0; i < n; i++) {
+        V3 oo = {(float)o[3 * i], (float)o[3 * i + 1], (float)o[3 * i + 2]};
+        V3 dd = {(float)d[3 * i], (float)d[3 * i + 1], (float)d[3 * i + 2]};
+        const float step = steps[i] > 0.0 ? (float)steps[i] : kInf;
+        Ray r = make_ray(oo, dd);
+        Stack st{lds.data(), 1, 0, spill.data(), 1, 0, depth, 0, 0};
+        Hit h = rule_hit(P, geo, r, oo, dd, step, st);
+        found[i] = h.found ? 1 : 0;
+        dist[i] = h.found ? (double)h.dist : 0.0;
+        surface[i] = h.found ? ref.tri_surface[h.tri] : -1;
+        index[i] = h.found ? ref.tri_index[h.tri] : -1;
+    }
     return 0;
 }

@@ -212,6 +212,7 @@ class Options(C.Structure):
         ("trajectoryCapacity", C.c_int),
         ("noEntryGrid", C.c_int),
         ("noSlabGrid", C.c_int),
+        ("noLatticeRule", C.c_int),
     ]
 
 

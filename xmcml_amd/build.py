@@ -23,8 +23,8 @@ REFERENCE = Path("/root/reference")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("XMC_OFFLOAD_ARCH", "gfx950")
-SOURCES = ["xmc_transport.hip", "xmc_mlp.hip", "xmc_host.cpp", "xmc_bvh.cpp", "xmc_distance.cpp", "xmc_entry.cpp",
-           "xmc_io.cpp", "xmc_bias.hip"]
+SOURCES = ["xmc_transport.hip", "xmc_mlp.hip", "xmc_host.cpp", "xmc_bvh.cpp", "xmc_distance.cpp", "xmc_lattice.cpp",
+           "xmc_entry.cpp", "xmc_io.cpp", "xmc_bias.hip"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-fno-fast-math", "-munsafe-fp-atomics",

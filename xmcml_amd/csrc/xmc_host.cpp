@@ -17,6 +17,7 @@
 #include "../../include/xmcml.h"
 #include "xmc_bvh.h"
 #include "xmc_distance.h"
+#include "xmc_lattice.h"
 #include "xmc_entry.h"
 #include "xmc_params.h"
 
@@ -67,6 +68,12 @@ size_t transport_lds(int block, int tile_count = 0, int det_lds = 0) {
     return (size_t)xmc::kStackLds * block * sizeof(int2) + 8 * sizeof(double) + (size_t)det_lds * 3 * sizeof(double) +
            (size_t)tile_count * sizeof(double);
 }
+
+// what a scene under the surface rule keeps of the walk-skipping grid: 0 nothing,
+// 1 the distance bound, 2 the bound and the separating slabs
+#ifndef XMC_RULE_GRID
+#define XMC_RULE_GRID 0
+#endif
 
 // detectors up to which a block keeps its detector sums in LDS
 constexpr int kDetLds = 64;
@@ -134,6 +141,7 @@ struct xmc_scene {
     DevBuf<uint32_t> dist;  // walk-skipping grid cells
     DevBuf<uint32_t> dist2;  // second-level entries (steps of up to 3 cells)
     DevBuf<uint64_t> slabs;  // separating slabs of the cells near the mesh
+    DevBuf<float> lattice_cells;  // the surface rule's heightfield cell records
     DevBuf<int> entry_ref;
     DevBuf<float> entry_box;
     // voxel-plane crossings: scattering grid, detector trajectory counts and
@@ -285,60 +293,66 @@ int xmc_scene_create(const xmc_input* in, const xmc_options* opt, xmc_scene** ou
     return scene_init(in, opt, true, out);
 }
 
-// The planar form of a scene (see Planar in xmc_params.h): every surface two
-// triangles over the four corners of an axis-aligned rectangle at one z,
-// sharing a diagonal.  Returns false (and n = 0) for any other scene.
-static bool planar_form(const xmc_input* in, const xmc::GpuBvh& gb, xmc::Planar& pl) {
-    std::memset(&pl, 0, sizeof(pl));
+// The surface rule of a scene (Planar and Lattice in xmc_params.h): every
+// surface an axis-aligned rectangle or a heightfield on one shared lattice.
+// planar gets the rectangles in surface order, each with its first triangle in
+// leaf order; lattice gets the heightfields' cell records (uploaded to `cells`).
+// Returns false, with both forms empty, for any other scene (or for a scene with
+// heightfields when with_lattice is false).
+static int surface_rule(const xmc_input* in, const xmc::GpuBvh& gb, const std::vector<int>& leaf_of,
+                        bool with_lattice, DevBuf<float>& cells, xmc::Params& P, bool* on) {
+    *on = false;
+    std::memset(&P.planar, 0, sizeof(P.planar));
+    std::memset(&P.lattice, 0, sizeof(P.lattice));
+    P.surface_rule = 0;
     const int ns = in->numberOfSurfaces;
-    if (ns < 1 || ns > xmc::kPlanarMax) return false;
+    std::vector<int> rect_of(ns, -1), heights;
+    xmc::Planar pl;
+    std::memset(&pl, 0, sizeof(pl));
     for (int s = 0; s < ns; s++) {
-        const xmc_surface& S = in->surface[s];
-        if (S.numberOfTriangles != 2) return false;
-        const xmc_int3 t[2] = {S.triangles[0], S.triangles[1]};
-        int used[6] = {t[0].x, t[0].y, t[0].z, t[1].x, t[1].y, t[1].z};
-        for (int k = 0; k < 6; k++)
-            if (used[k] < 0 || used[k] >= S.numberOfVertices) return false;
-        const xmc_double3& v0 = S.vertices[used[0]];
-        double xs[2] = {v0.x, v0.x}, ys[2] = {v0.y, v0.y};
-        for (int k = 0; k < 6; k++) {
-            const xmc_double3& v = S.vertices[used[k]];
-            if (v.z != v0.z) return false;
-            xs[0] = std::min(xs[0], v.x);
-            xs[1] = std::max(xs[1], v.x);
-            ys[0] = std::min(ys[0], v.y);
-            ys[1] = std::max(ys[1], v.y);
+        xmc::RectForm r;
+        if (xmc::planar_rectangle(in->surface[s], r)) {
+            if (pl.n == xmc::kPlanarMax) return XMC_OK;
+            rect_of[s] = pl.n;
+            pl.z[pl.n] = r.z;
+            pl.x0[pl.n] = r.x0;
+            pl.x1[pl.n] = r.x1;
+            pl.y0[pl.n] = r.y0;
+            pl.y1[pl.n] = r.y1;
+            pl.tri[pl.n] = -1;
+            pl.n++;
+        } else {
+            heights.push_back(s);
         }
-        if (!(xs[0] < xs[1] && ys[0] < ys[1])) return false;
-        // each vertex a corner; each triangle three distinct corners; the shared two opposite
-        auto corner = [&](int i) {
-            const xmc_double3& v = S.vertices[i];
-            int cx = v.x == xs[0] ? 0 : (v.x == xs[1] ? 1 : -1);
-            int cy = v.y == ys[0] ? 0 : (v.y == ys[1] ? 1 : -1);
-            return (cx < 0 || cy < 0) ? -1 : cx + 2 * cy;
-        };
-        int c[6];
-        for (int k = 0; k < 6; k++)
-            if ((c[k] = corner(used[k])) < 0) return false;
-        int mask[2] = {0, 0};
-        for (int k = 0; k < 6; k++) mask[k / 3] |= 1 << c[k];
-        auto bits = [](int m) { return __builtin_popcount((unsigned)m); };
-        if (bits(mask[0]) != 3 || bits(mask[1]) != 3 || (mask[0] | mask[1]) != 15) return false;
-        const int shared = mask[0] & mask[1];
-        if (shared != (1 | 8) && shared != (2 | 4)) return false;  // corners 0-3 or 1-2: a diagonal
-        pl.z[s] = (float)v0.z;
-        pl.x0[s] = (float)xs[0];
-        pl.x1[s] = (float)xs[1];
-        pl.y0[s] = (float)ys[0];
-        pl.y1[s] = (float)ys[1];
-        pl.tri[s] = -1;
     }
+    xmc::LatticeForm lf;
+    if (!heights.empty() && (!with_lattice || !xmc::lattice_form(in->surface, ns, heights, leaf_of, lf)))
+        return XMC_OK;
     for (size_t k = 0; k < gb.tris.size(); k++) {
-        const int s = gb.tris[k].surface;
-        if (pl.tri[s] < 0) pl.tri[s] = (int)k;
+        const int r = rect_of[gb.tris[k].surface];
+        if (r >= 0 && pl.tri[r] < 0) pl.tri[r] = (int)k;
     }
-    pl.n = ns;
-    return true;
+    P.planar = pl;
+    if (lf.ns > 0) {
+        HIP_TRY(cells.alloc(lf.cells.size()));
+        HIP_TRY(hipMemcpy(cells.p, lf.cells.data(), lf.cells.size() * sizeof(float), hipMemcpyHostToDevice));
+        xmc::Lattice& L = P.lattice;
+        L.ns = lf.ns;
+        L.nx = lf.nx;
+        L.ny = lf.ny;
+        L.x0 = lf.x0;
+        L.y0 = lf.y0;
+        L.inv_dx = lf.inv_dx;
+        L.inv_dy = lf.inv_dy;
+        L.mxy = lf.mxy;
+        L.mz = lf.mz;
+        L.zlo = lf.zlo;
+        L.zhi = lf.zhi;
+        L.cells = reinterpret_cast<const float4*>(cells.p);
+    }
+    P.surface_rule = 1;
+    *on = true;
+    return XMC_OK;
 }
 
 // Persistent launch shape and the stack spill it needs.  kernel_occupancy is
@@ -368,7 +382,7 @@ static int configure_launch(int device, int depth, int block, int blocks_per_cu,
 // The occupancy query of the transport kernel xmc_launch_transport picks for
 // this scene (interleaved when it has a walk-skipping grid).
 static auto transport_occupancy_of(const xmc::Params& P) -> hipError_t (*)(int, size_t, int*) {
-    return P.dist.cells != nullptr ? xmc_transport_occupancy : xmc_transport_occupancy_plain;
+    return P.dist.cells != nullptr && P.surface_rule == 0 ? xmc_transport_occupancy : xmc_transport_occupancy_plain;
 }
 
 // One trajectory list of traj_cap voxel indices per persistent lane.
@@ -449,7 +463,8 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     P.nodes = sc->nodes.p;
     P.tris = sc->tris.p;
     P.normals = sc->normals.p;
-    planar_form(in, gb, P.planar);
+    bool rule = false;
+    if ((rc = surface_rule(in, gb, sc->perm, o.noLatticeRule == 0, sc->lattice_cells, P, &rule))) return rc;
     P.root_ref = gb.root_ref;
     for (int a = 0; a < 3; a++) {
         P.root_lo[a] = gb.root_lo[a];
@@ -458,7 +473,11 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
 
     // walk-skipping grid: per cell a distance bound (skip walks that cannot hit) and an
     // entry (short walks run on the cell's restricted tree)
-    const bool want_dist = o.noDistanceGrid == 0, want_entry = o.noEntryGrid == 0 && gb.root_ref >= 0;
+    // a scene under the surface rule keeps the distance bound (and the slabs, with
+    // XMC_RULE_GRID 2) ahead of its rule; its walks (steps too long for the lattice)
+    // start at the root
+    const bool want_dist = o.noDistanceGrid == 0 && (!rule || XMC_RULE_GRID > 0);
+    const bool want_entry = o.noEntryGrid == 0 && gb.root_ref >= 0 && !rule;
     if (nt > 256 && (want_dist || want_entry)) {
         xmc::DistanceGrid dg;
         double lo[3], hi[3];
@@ -517,7 +536,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
         P.dist.entry_box = sc->entry_box.p;
         P.dist.cells2 = want_entry ? sc->dist2.p : nullptr;
         P.dist.margin2 = want_entry ? eg.margin2 : 0.0f;
-        if (want_dist && o.noSlabGrid == 0) {
+        if (want_dist && o.noSlabGrid == 0 && (!rule || XMC_RULE_GRID > 1)) {
             // slabs chosen for steps of the scene's typical length: the median mean free path
             // of its scattering layers
             std::vector<double> mfp;
@@ -686,7 +705,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     // LDS absorption tile: up to kAbsTileVoxels voxels from the beam's entry voxel down,
     // centred on it across (most of the weight is deposited within a few voxels of it)
     std::memset(&P.abs_tile, 0, sizeof(P.abs_tile));
-    if (launchable && sc->absorption && P.dist.cells == nullptr) {  // the kernel variant without the grid
+    if (launchable && sc->absorption && (P.dist.cells == nullptr || P.surface_rule)) {  // the plain kernel
         const xmc_area* ar = in->area;
         const int pn[3] = {part.x, part.y, part.z};
         const double sp[3] = {in->startPosition.x, in->startPosition.y, in->startPosition.z};

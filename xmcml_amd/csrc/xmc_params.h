@@ -117,15 +117,33 @@ struct DistGrid {
     float slab_unit;
 };
 
-// Scenes whose every surface is an axis-aligned rectangle in a plane z = const
-// (two triangles sharing a diagonal, as the reference's plane generator makes
-// them): the step's nearest surface is found analytically instead of by a BVH
-// walk.  n = 0 for every other scene.
+// Surfaces that are axis-aligned rectangles in a plane z = const (two
+// triangles sharing a diagonal, as the reference's plane generator makes
+// them): their nearest hit is found analytically.
 constexpr int kPlanarMax = 16;
 struct Planar {
     int n;
     float z[kPlanarMax], x0[kPlanarMax], x1[kPlanarMax], y0[kPlanarMax], y1[kPlanarMax];
     int tri[kPlanarMax];  // the surface's first triangle in BVH leaf order (its normal and surface id)
+};
+
+// Heightfield surfaces on a shared uniform lattice (every triangle inside one
+// lattice cell, at most two per cell and surface): the step's nearest surface
+// among them is found by testing the triangles of the cells that the step's xy
+// bounding box (widened by mxy) covers, in cells whose z-range (per surface)
+// meets the step's z-range (widened by mz).  Triangles are tested exactly as
+// the BVH leaves test them, so a hit is the walk's hit (on equal distances the
+// lower leaf index).  ns = 0: none.
+constexpr int kLatticeMaxCells = 16;    // larger boxes (long steps) take the BVH walk
+constexpr int kLatticeMaxSurfaces = 8;  // heightfields per lattice
+struct Lattice {
+    int ns;              // lattice surfaces
+    int nx, ny;          // cells along x and y
+    float x0, y0;        // lattice origin
+    float inv_dx, inv_dy;
+    float mxy, mz;       // margins of the bounding box
+    float zlo, zhi;      // z-range of every lattice triangle
+    const float4* cells; // [(iy * nx + ix) * ns + s] = (zmin, zmax, triangle a, triangle b) (bits; -1 = none)
 };
 
 // A box of absorption voxels (around the beam entry, where most weight is
@@ -188,7 +206,11 @@ struct Params {
     // and adds to the tallies once at its end (0: every detection goes
     // straight to the tallies); the time bins always go straight
     int det_lds;
+    // 1: the step's nearest surface comes from the planar and lattice forms
+    // (planar.n rectangles, lattice.ns heightfields) instead of the BVH walk
+    int surface_rule;
     Planar planar;
+    Lattice lattice;
 };
 
 struct QueryParams {

@@ -524,6 +524,76 @@ __device__ __forceinline__ Hit planar_hit(const Planar& S, V3 p, V3 d, float ste
     return h;
 }
 
+// Float rule of the scenes whose every surface is a rectangle or a lattice
+// heightfield (surface_rule = 1): the rectangles' nearest hit as planar_hit
+// finds it; then every triangle of the heightfields whose cell lies in the
+// step's xy bounding box widened by L.mxy and whose cell z-range (per
+// heightfield) meets the step's z-range widened by L.mz, tested as a BVH leaf
+// tests it: a triangle nearer than the hit, or as near with a lower leaf
+// index, replaces it (so the order of the tests does not matter).  A step
+// whose box covers more than kLatticeMaxCells cells, or an unlimited one,
+// takes the BVH walk over every surface instead.
+__device__ __forceinline__ void lattice_take(const GpuTri* tris, int k, const Ray& ray, Hit& h) {
+    int last;
+    const float t = tri_distance(tris, k, ray, last);
+    if (t >= 0.0f && (t < h.dist || (t == h.dist && k < h.tri))) {
+        h.dist = t;
+        h.tri = k;
+    }
+}
+
+__device__ __forceinline__ Hit rule_hit(const Params& C, const Geo& geo, const Ray& ray, V3 p, V3 d, float step,
+                                        Stack& st) {
+    Hit h = planar_hit(C.planar, p, d, step);
+    const Lattice& L = C.lattice;
+    if (L.ns == 0) return h;
+    if (!(step < kInf)) return walk(geo, ray, step, true, st);
+    const float ez = __builtin_fmaf(step, d.z, p.z);
+    const float az = fminf(p.z, ez) - L.mz, bz = fmaxf(p.z, ez) + L.mz;
+    if (bz < L.zlo || az > L.zhi) return h;
+    const float ex = __builtin_fmaf(step, d.x, p.x), ey = __builtin_fmaf(step, d.y, p.y);
+    const float fx = (float)L.nx, fy = (float)L.ny;
+    const int i0 = max(0, (int)floorf(fmaxf((fminf(p.x, ex) - L.mxy - L.x0) * L.inv_dx, -1.0f)));
+    const int i1 = min(L.nx - 1, (int)floorf(fminf((fmaxf(p.x, ex) + L.mxy - L.x0) * L.inv_dx, fx)));
+    const int j0 = max(0, (int)floorf(fmaxf((fminf(p.y, ey) - L.mxy - L.y0) * L.inv_dy, -1.0f)));
+    const int j1 = min(L.ny - 1, (int)floorf(fminf((fmaxf(p.y, ey) + L.mxy - L.y0) * L.inv_dy, fy)));
+    if (i0 > i1 || j0 > j1) return h;
+    if ((i1 - i0 + 1) * (j1 - j0 + 1) > kLatticeMaxCells) return walk(geo, ray, step, true, st);
+    // the first two candidate cells' triangle pairs are tested after the scan, so that
+    // lanes with a candidate in different cells share one test; further ones (rare) at once
+    int a0 = -1, b0 = -1, a1 = -1, b1 = -1;
+    for (int j = j0; j <= j1; ++j) {
+        for (int i = i0; i <= i1; ++i) {
+            const float4* R = L.cells + (size_t)(j * L.nx + i) * L.ns;
+#pragma unroll
+            for (int s = 0; s < kLatticeMaxSurfaces; ++s) {
+                if (s < L.ns) {
+                    const float4 c = R[s];
+                    if (c.x <= bz && c.y >= az) {
+                        const int ta = __float_as_int(c.z), tb = __float_as_int(c.w);
+                        if (a0 < 0) {
+                            a0 = ta;
+                            b0 = tb;
+                        } else if (a1 < 0) {
+                            a1 = ta;
+                            b1 = tb;
+                        } else {
+                            lattice_take(geo.tris, ta, ray, h);
+                            if (tb >= 0) lattice_take(geo.tris, tb, ray, h);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (a0 >= 0) lattice_take(geo.tris, a0, ray, h);
+    if (b0 >= 0) lattice_take(geo.tris, b0, ray, h);
+    if (a1 >= 0) lattice_take(geo.tris, a1, ray, h);
+    if (b1 >= 0) lattice_take(geo.tris, b1, ray, h);
+    h.found = h.tri >= 0 && h.dist <= step;
+    return h;
+}
+
 // --------------------------------------------------------------------------
 // photon physics (mcml_kernel.cpp), float
 // --------------------------------------------------------------------------
@@ -1141,7 +1211,13 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                     phase = walk_begin(geo, ray, step, true, st, h, cur, best) ? kWalk : kResolve;
                 }
             } else {
-                h = C.planar.n > 0 ? planar_hit(C.planar, pos, dir, step) : walk(geo, ray, step, true, st);
+                // a scene under the surface rule may carry the grid's distance bound (applied
+                // in the begin phase) and separating slabs: a step they clear has no hit
+                if (C.dist.slabs != nullptr && slab_skips(C.dist, pos, dir, step)) {
+                    n_culled++;
+                } else {
+                    h = C.surface_rule ? rule_hit(C, geo, ray, pos, dir, step, st) : walk(geo, ray, step, true, st);
+                }
                 phase = kResolve;
             }
         }
@@ -1402,9 +1478,13 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
 // VGPRs, five waves per SIMD instead of four, at the price of a few spills: the
 // heightfield goes from 2.93e7 to 3.29e7 photons/s; six waves (80 VGPRs) spill
 // enough to fall to 2.54e7.  The other variants keep the registers they need.
+#ifndef XMC_PLAIN_WAVES
+#define XMC_PLAIN_WAVES 5
+#endif
 template <bool TRACE, bool PATHS, bool INTERLEAVE>
 __global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu((!TRACE && !PATHS && INTERLEAVE) ? 5 : 1))) transport_kernel(Params P) {
+    __attribute__((amdgpu_waves_per_eu((!TRACE && !PATHS) ? (INTERLEAVE ? 5 : XMC_PLAIN_WAVES) : 1)))
+    transport_kernel(Params P) {
     transport_body<TRACE, false, PATHS, INTERLEAVE>(P, nullptr, nullptr, 0);
 }
 
@@ -1525,7 +1605,7 @@ extern "C" {
 // walks run to the end within the round otherwise.
 hipError_t xmc_launch_transport(const xmc::Params* p, int grid, int block, size_t lds, hipStream_t s, int trace) {
     const bool paths = p->paths.scattering != nullptr || p->paths.det_traj != nullptr;
-    const bool interleave = p->dist.cells != nullptr;
+    const bool interleave = p->dist.cells != nullptr && p->surface_rule == 0;
     if (trace && paths) launch_transport<true, true>(p, interleave, grid, block, lds, s);
     else if (trace) launch_transport<true, false>(p, interleave, grid, block, lds, s);
     else if (paths) launch_transport<false, true>(p, interleave, grid, block, lds, s);

@@ -157,12 +157,12 @@ class Scene:
     def __init__(self, spec: SceneSpec, block_size: int = 0, blocks_per_cu: int = 0,
                  absorption: bool = True, device: int = 0, distance_grid: bool = True,
                  scattering: bool = False, trajectories: bool = False, trajectory_capacity: int = 0,
-                 entry_grid: bool = True, slab_grid: bool = True):
+                 entry_grid: bool = True, slab_grid: bool = True, lattice_rule: bool = True):
         self.spec = spec
         self.input = InputHandle(spec)
         opt = Options(device, block_size, blocks_per_cu, 0 if absorption else 1, 0 if distance_grid else 1,
                       int(scattering), int(trajectories), trajectory_capacity, 0 if entry_grid else 1,
-                      0 if slab_grid else 1)
+                      0 if slab_grid else 1, 0 if lattice_rule else 1)
         self.scattering, self.trajectories = scattering, trajectories
         h = _P()
         _check(lib().xmc_scene_create(self.input.ptr, C.byref(opt), C.byref(h)), "xmc_scene_create")
