@@ -263,7 +263,8 @@ typedef struct xmc_options {
                           * skip their walk; results are the same either way) */
     int noLatticeRule;   /* 1 = scenes of heightfields on a shared uniform lattice (and axis-aligned rectangles)
                           * walk the BVH instead of testing the triangles of the lattice cells each step covers
-                          * (hits are the walk's but for float ties; scenes of rectangles only keep their rule) */
+                          * (hits are the walk's but for float ties; scenes of rectangles only keep their rule).
+                          * Batches always walk heightfields. */
 } xmc_options;
 
 typedef struct xmc_scene xmc_scene;

@@ -69,12 +69,6 @@ size_t transport_lds(int block, int tile_count = 0, int det_lds = 0) {
            (size_t)tile_count * sizeof(double);
 }
 
-// what a scene under the surface rule keeps of the walk-skipping grid: 0 nothing,
-// 1 the distance bound, 2 the bound and the separating slabs
-#ifndef XMC_RULE_GRID
-#define XMC_RULE_GRID 0
-#endif
-
 // detectors up to which a block keeps its detector sums in LDS
 constexpr int kDetLds = 64;
 
@@ -350,7 +344,7 @@ static int surface_rule(const xmc_input* in, const xmc::GpuBvh& gb, const std::v
         L.zhi = lf.zhi;
         L.cells = reinterpret_cast<const float4*>(cells.p);
     }
-    P.surface_rule = 1;
+    P.surface_rule = lf.ns > 0 ? 2 : 1;
     *on = true;
     return XMC_OK;
 }
@@ -382,7 +376,7 @@ static int configure_launch(int device, int depth, int block, int blocks_per_cu,
 // The occupancy query of the transport kernel xmc_launch_transport picks for
 // this scene (interleaved when it has a walk-skipping grid).
 static auto transport_occupancy_of(const xmc::Params& P) -> hipError_t (*)(int, size_t, int*) {
-    return P.dist.cells != nullptr && P.surface_rule == 0 ? xmc_transport_occupancy : xmc_transport_occupancy_plain;
+    return P.dist.cells != nullptr ? xmc_transport_occupancy : xmc_transport_occupancy_plain;
 }
 
 // One trajectory list of traj_cap voxel indices per persistent lane.
@@ -473,10 +467,10 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
 
     // walk-skipping grid: per cell a distance bound (skip walks that cannot hit) and an
     // entry (short walks run on the cell's restricted tree)
-    // a scene under the surface rule keeps the distance bound (and the slabs, with
-    // XMC_RULE_GRID 2) ahead of its rule; its walks (steps too long for the lattice)
-    // start at the root
-    const bool want_dist = o.noDistanceGrid == 0 && (!rule || XMC_RULE_GRID > 0);
+    // a scene under the surface rule has no grid: its plain kernel ran slower with the
+    // distance bound ahead of the rule (one more dependent load per step); its few walks
+    // (steps too long for the lattice) start at the root
+    const bool want_dist = o.noDistanceGrid == 0 && !rule;
     const bool want_entry = o.noEntryGrid == 0 && gb.root_ref >= 0 && !rule;
     if (nt > 256 && (want_dist || want_entry)) {
         xmc::DistanceGrid dg;
@@ -536,7 +530,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
         P.dist.entry_box = sc->entry_box.p;
         P.dist.cells2 = want_entry ? sc->dist2.p : nullptr;
         P.dist.margin2 = want_entry ? eg.margin2 : 0.0f;
-        if (want_dist && o.noSlabGrid == 0 && (!rule || XMC_RULE_GRID > 1)) {
+        if (want_dist && o.noSlabGrid == 0) {
             // slabs chosen for steps of the scene's typical length: the median mean free path
             // of its scattering layers
             std::vector<double> mfp;
@@ -705,7 +699,7 @@ static int scene_init(const xmc_input* in, const xmc_options* opt, bool launchab
     // LDS absorption tile: up to kAbsTileVoxels voxels from the beam's entry voxel down,
     // centred on it across (most of the weight is deposited within a few voxels of it)
     std::memset(&P.abs_tile, 0, sizeof(P.abs_tile));
-    if (launchable && sc->absorption && (P.dist.cells == nullptr || P.surface_rule)) {  // the plain kernel
+    if (launchable && sc->absorption && P.dist.cells == nullptr) {  // the kernel variant without the grid
         const xmc_area* ar = in->area;
         const int pn[3] = {part.x, part.y, part.z};
         const double sp[3] = {in->startPosition.x, in->startPosition.y, in->startPosition.z};
@@ -1325,6 +1319,9 @@ int xmc_batch_create(const xmc_input* configs, int n, const xmc_options* opt, xm
         return fail(XMC_ERR_ARG, "a batch tallies no scattering grid or trajectories (use one scene per input)");
     auto* b = new xmc_batch();
     std::unique_ptr<xmc_batch, int (*)(xmc_batch*)> guard(b, xmc_batch_destroy);
+    // the batch kernel carries the rectangles' rule only: a configuration with
+    // heightfields walks them (keeping the lattice code out of its registers)
+    o.noLatticeRule = 1;
     b->device = o.device;
     b->n = n;
     b->prefix.assign(n + 1, 0);

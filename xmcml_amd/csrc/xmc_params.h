@@ -206,8 +206,9 @@ struct Params {
     // and adds to the tallies once at its end (0: every detection goes
     // straight to the tallies); the time bins always go straight
     int det_lds;
-    // 1: the step's nearest surface comes from the planar and lattice forms
-    // (planar.n rectangles, lattice.ns heightfields) instead of the BVH walk
+    // the step's nearest surface from the planar and lattice forms instead of the
+    // BVH walk: 0 no, 1 rectangles only (planar.n), 2 rectangles and heightfields
+    // (lattice.ns > 0)
     int surface_rule;
     Planar planar;
     Lattice lattice;

@@ -525,7 +525,7 @@ __device__ __forceinline__ Hit planar_hit(const Planar& S, V3 p, V3 d, float ste
 }
 
 // Float rule of the scenes whose every surface is a rectangle or a lattice
-// heightfield (surface_rule = 1): the rectangles' nearest hit as planar_hit
+// heightfield (surface_rule = 2): the rectangles' nearest hit as planar_hit
 // finds it; then every triangle of the heightfields whose cell lies in the
 // step's xy bounding box widened by L.mxy and whose cell z-range (per
 // heightfield) meets the step's z-range widened by L.mz, tested as a BVH leaf
@@ -1211,12 +1211,11 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                     phase = walk_begin(geo, ray, step, true, st, h, cur, best) ? kWalk : kResolve;
                 }
             } else {
-                // a scene under the surface rule may carry the grid's distance bound (applied
-                // in the begin phase) and separating slabs: a step they clear has no hit
-                if (C.dist.slabs != nullptr && slab_skips(C.dist, pos, dir, step)) {
-                    n_culled++;
+                if constexpr (BATCHED) {
+                    // batched configurations keep no lattice form (their heightfields walk)
+                    h = C.surface_rule != 0 ? planar_hit(C.planar, pos, dir, step) : walk(geo, ray, step, true, st);
                 } else {
-                    h = C.surface_rule ? rule_hit(C, geo, ray, pos, dir, step, st) : walk(geo, ray, step, true, st);
+                    h = C.surface_rule != 0 ? rule_hit(C, geo, ray, pos, dir, step, st) : walk(geo, ray, step, true, st);
                 }
                 phase = kResolve;
             }
@@ -1605,7 +1604,7 @@ extern "C" {
 // walks run to the end within the round otherwise.
 hipError_t xmc_launch_transport(const xmc::Params* p, int grid, int block, size_t lds, hipStream_t s, int trace) {
     const bool paths = p->paths.scattering != nullptr || p->paths.det_traj != nullptr;
-    const bool interleave = p->dist.cells != nullptr && p->surface_rule == 0;
+    const bool interleave = p->dist.cells != nullptr;
     if (trace && paths) launch_transport<true, true>(p, interleave, grid, block, lds, s);
     else if (trace) launch_transport<true, false>(p, interleave, grid, block, lds, s);
     else if (paths) launch_transport<false, true>(p, interleave, grid, block, lds, s);
