@@ -125,7 +125,7 @@ extern "C" int rule_query(const xmc_surface* surf, int ns, const double* o, cons
         const int r = rect_of[gb.tris[k].surface];
         if (r >= 0 && P.planar.tri[r] < 0) P.planar.tri[r] = (int)k;
     }
-    P.surface_rule = 1;
+    P.surface_rule = lf.ns > 0 ? 2 : 1;
     Lattice& L = P.lattice;
     L.ns = lf.ns;
     L.nx = lf.nx;
