@@ -314,6 +314,36 @@ def test_entry_trees_change_nothing_on_the_full_heightfield(engine):
     np.testing.assert_allclose(a.det_weight(), b.det_weight(), rtol=1e-11, atol=1e-12)
 
 
+def test_lattice_rule_changes_nothing_on_the_full_heightfield(engine):
+    """The 98,306-triangle heightfield under its lattice rule (plain kernel)
+    and on the BVH walk (interleaved kernel with the walk-skipping grids):
+    traces of 48 photons equal event for event, and the tallies and event
+    counts of 20,000 photons bin for bin."""
+    n = 20_000
+    spec = scenes.heightfield_scene(n)
+    res = []
+    for rule in (True, False):
+        with engine.Scene(spec, absorption=False, lattice_rule=rule) as sc:
+            traces, counts = sc.trace(43, 0, 48, max_events=20000)
+            assert counts.max() < 20000
+            sc.collect()
+            sc.stats()
+            sc.run(44, 0, n)
+            res.append((traces, sc.collect(), sc.stats()))
+    (ta, a, sa), (tb, b, sb) = res
+    assert sa["culledWalks"] == 0 and sb["culledWalks"] > 0  # the rule builds no grid
+    for p in range(48):
+        assert len(ta[p]) == len(tb[p]), p
+        for k in ta[p].dtype.names:
+            assert np.array_equal(ta[p][k], tb[p][k]), (p, k)
+    for k in ("scatterEvents", "boundaryEvents", "rouletteEvents"):
+        assert sa[k] == sb[k], k
+    assert np.array_equal(a.det_count(), b.det_count())
+    assert np.array_equal(a.bin_count(), b.bin_count())
+    np.testing.assert_allclose(a.det_weight(), b.det_weight(), rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(a.bin_weight(), b.bin_weight(), rtol=1e-11, atol=1e-12)
+
+
 @pytest.mark.parametrize("kind", ["heightfield", "head"])
 def test_separating_slabs_change_nothing(engine, kind):
     """The separating slabs only skip walks that cannot hit: with and without
