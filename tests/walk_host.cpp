@@ -149,7 +149,7 @@ extern "C" int rule_query(const xmc_surface* surf, int ns, const double* o, cons
         const float step = steps[i] > 0.0 ? (float)steps[i] : kInf;
         Ray r = make_ray(oo, dd);
         Stack st{lds.data(), 1, 0, spill.data(), 1, 0, depth, 0, 0};
-        Hit h = rule_hit(P, geo, r, oo, dd, step, st);
+        Hit h = rule_hit(P, geo, oo, dd, step, st);
         found[i] = h.found ? 1 : 0;
         dist[i] = h.found ? (double)h.dist : 0.0;
         surface[i] = h.found ? ref.tri_surface[h.tri] : -1;

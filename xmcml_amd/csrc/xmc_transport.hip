@@ -542,12 +542,11 @@ __device__ __forceinline__ void lattice_take(const GpuTri* tris, int k, const Ra
     }
 }
 
-__device__ __forceinline__ Hit rule_hit(const Params& C, const Geo& geo, const Ray& ray, V3 p, V3 d, float step,
-                                        Stack& st) {
+__device__ __forceinline__ Hit rule_hit(const Params& C, const Geo& geo, V3 p, V3 d, float step, Stack& st) {
     Hit h = planar_hit(C.planar, p, d, step);
     const Lattice& L = C.lattice;
     if (L.ns == 0) return h;
-    if (!(step < kInf)) return walk(geo, ray, step, true, st);
+    if (!(step < kInf)) return walk(geo, make_ray(p, d), step, true, st);
     const float ez = __builtin_fmaf(step, d.z, p.z);
     const float az = fminf(p.z, ez) - L.mz, bz = fmaxf(p.z, ez) + L.mz;
     if (bz < L.zlo || az > L.zhi) return h;
@@ -558,7 +557,10 @@ __device__ __forceinline__ Hit rule_hit(const Params& C, const Geo& geo, const R
     const int j0 = max(0, (int)floorf(fmaxf((fminf(p.y, ey) - L.mxy - L.y0) * L.inv_dy, -1.0f)));
     const int j1 = min(L.ny - 1, (int)floorf(fminf((fmaxf(p.y, ey) + L.mxy - L.y0) * L.inv_dy, fy)));
     if (i0 > i1 || j0 > j1) return h;
-    if ((i1 - i0 + 1) * (j1 - j0 + 1) > kLatticeMaxCells) return walk(geo, ray, step, true, st);
+    if ((i1 - i0 + 1) * (j1 - j0 + 1) > kLatticeMaxCells) return walk(geo, make_ray(p, d), step, true, st);
+    Ray ray;  // the triangle test reads the origin and direction only
+    ray.o = p;
+    ray.d = d;
     // the first two candidate cells' triangle pairs are tested after the scan, so that
     // lanes with a candidate in different cells share one test; further ones (rare) at once
     int a0 = -1, b0 = -1, a1 = -1, b1 = -1;
@@ -1200,8 +1202,8 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             if constexpr (INTERLEAVE) {
                 if (C.dist.slabs != nullptr) skip = slab_skips(C.dist, pos, dir, step);
             }
-            Ray ray = make_ray(pos, dir);
             if constexpr (INTERLEAVE) {
+                Ray ray = make_ray(pos, dir);
                 rinv = ray.inv;
                 raxes = (ray.ax ? 1u : 0u) | (ray.ay ? 2u : 0u) | (ray.az ? 4u : 0u);
                 if (skip) {
@@ -1211,11 +1213,14 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
                     phase = walk_begin(geo, ray, step, true, st, h, cur, best) ? kWalk : kResolve;
                 }
             } else {
+                // the ray's reciprocal direction serves the box tests of a walk only
                 if constexpr (BATCHED) {
                     // batched configurations keep no lattice form (their heightfields walk)
-                    h = C.surface_rule != 0 ? planar_hit(C.planar, pos, dir, step) : walk(geo, ray, step, true, st);
+                    h = C.surface_rule != 0 ? planar_hit(C.planar, pos, dir, step)
+                                            : walk(geo, make_ray(pos, dir), step, true, st);
                 } else {
-                    h = C.surface_rule != 0 ? rule_hit(C, geo, ray, pos, dir, step, st) : walk(geo, ray, step, true, st);
+                    h = C.surface_rule != 0 ? rule_hit(C, geo, pos, dir, step, st)
+                                            : walk(geo, make_ray(pos, dir), step, true, st);
                 }
                 phase = kResolve;
             }
