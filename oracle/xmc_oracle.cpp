@@ -918,7 +918,7 @@ template <class R> struct Scene {
                 hs.push_back(si);
             }
         }
-        if (!hs.empty() && (!g_lattice_rule || hs.size() > 8 || !init_lattice(input, hs))) {
+        if (!hs.empty() && (!g_lattice_rule || hs.size() > 4 || !init_lattice(input, hs))) {
             planar = PlanarForm();
             lattice = LatticeRule();
             return;
@@ -970,8 +970,8 @@ template <class R> struct Scene {
     // bounding box widened by mxy, per heightfield whose cell z-range meets the
     // step's z-range widened by mz, the cell's triangles with the BVH leaves' test:
     // a nearer one, or one as near with a lower bvh position, replaces the hit.
-    // Unlimited steps, and boxes of more than 16 cells, take the BVH walk over
-    // every surface.
+    // Unlimited steps, and boxes of more than 4 cells along x or y, take the BVH
+    // walk over every surface.
     Hit<float> rule_hit(const V3<float>& o, const V3<float>& d, float step, bool limited) const {
         const LatticeRule& L = lattice;
         if (L.ns == 0) return planar_hit(o, d, step, limited);
@@ -991,7 +991,7 @@ template <class R> struct Scene {
                 }
             }
         }
-        auto search = [&]() -> bool {  // false: the box is too large for the lattice
+        auto search = [&]() -> bool {  // false: the box is too large for the rule
             const float ez = std::fmaf(step, d.z, o.z);
             const float az = std::fmin(o.z, ez) - L.mz, bz = std::fmax(o.z, ez) + L.mz;
             if (bz < L.zlo || az > L.zhi) return true;
@@ -1003,7 +1003,7 @@ template <class R> struct Scene {
             const int j1 = std::min(L.ny - 1, (int)std::floor(std::fmin((std::fmax(o.y, ey) + L.mxy - L.y0) * L.inv_dy,
                                                                         (float)L.ny)));
             if (i0 > i1 || j0 > j1) return true;
-            if ((i1 - i0 + 1) * (j1 - j0 + 1) > 16) return false;
+            if (i1 - i0 >= 4 || j1 - j0 >= 4) return false;
             for (int j = j0; j <= j1; j++)
                 for (int i = i0; i <= i1; i++)
                     for (int l = 0; l < L.ns; l++) {

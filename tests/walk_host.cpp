@@ -20,6 +20,7 @@
 struct int2 { int x, y; };
 struct int4 { int x, y, z, w; };
 struct float4 { float x, y, z, w; };
+struct float2 { float x, y; };
 inline int2 make_int2(int a, int b) { return {a, b}; }
 inline int __float_as_int(float f) { int i; std::memcpy(&i, &f, 4); return i; }
 inline float __int_as_float(int i) { float f; std::memcpy(&f, &i, 4); return f; }

@@ -71,7 +71,7 @@ bool uniform_axis(std::vector<double>& v, double& origin, double& step) {
 bool lattice_form(const xmc_surface* surfaces, int n_surfaces, const std::vector<int>& which,
                   const std::vector<int>& leaf_of, LatticeForm& out) {
     out = LatticeForm();
-    if (which.empty() || which.size() > 8) return false;  // kLatticeMaxSurfaces
+    if (which.empty() || which.size() > 4) return false;  // kLatticeMaxSurfaces
     std::vector<int> base(n_surfaces + 1, 0);
     for (int s = 0; s < n_surfaces; s++) base[s + 1] = base[s] + surfaces[s].numberOfTriangles;
     std::vector<double> xs, ys;

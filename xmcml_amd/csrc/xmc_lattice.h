@@ -36,7 +36,7 @@ struct LatticeForm {
 // Builds the lattice form of surfaces[which[0..]] (in that order); leaf_of
 // maps base[s] + t (base = the running triangle count over surfaces) to the
 // leaf index of triangle t of surface s.  Returns false when they do not
-// share such a lattice (or there are more than 8 of them, or it would exceed
+// share such a lattice (or there are more than 4 of them, or it would exceed
 // 2^26 records).
 bool lattice_form(const xmc_surface* surfaces, int n_surfaces, const std::vector<int>& which,
                   const std::vector<int>& leaf_of, LatticeForm& out);

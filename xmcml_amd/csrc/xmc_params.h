@@ -134,8 +134,8 @@ struct Planar {
 // meets the step's z-range (widened by mz).  Triangles are tested exactly as
 // the BVH leaves test them, so a hit is the walk's hit (on equal distances the
 // lower leaf index).  ns = 0: none.
-constexpr int kLatticeMaxCells = 16;    // larger boxes (long steps) take the BVH walk
-constexpr int kLatticeMaxSurfaces = 8;  // heightfields per lattice
+constexpr int kLatticeMaxBox = 4;       // cells along x and y; larger boxes (long steps) take the BVH walk
+constexpr int kLatticeMaxSurfaces = 4;  // heightfields per lattice
 struct Lattice {
     int ns;              // lattice surfaces
     int nx, ny;          // cells along x and y

@@ -531,8 +531,8 @@ __device__ __forceinline__ Hit planar_hit(const Planar& S, V3 p, V3 d, float ste
 // heightfield) meets the step's z-range widened by L.mz, tested as a BVH leaf
 // tests it: a triangle nearer than the hit, or as near with a lower leaf
 // index, replaces it (so the order of the tests does not matter).  A step
-// whose box covers more than kLatticeMaxCells cells, or an unlimited one,
-// takes the BVH walk over every surface instead.
+// whose box spans more than kLatticeMaxBox cells along x or y, or an
+// unlimited one, takes the BVH walk over every surface instead.
 __device__ __forceinline__ void lattice_take(const GpuTri* tris, int k, const Ray& ray, Hit& h) {
     int last;
     const float t = tri_distance(tris, k, ray, last);
@@ -557,41 +557,36 @@ __device__ __forceinline__ Hit rule_hit(const Params& C, const Geo& geo, V3 p, V
     const int j0 = max(0, (int)floorf(fmaxf((fminf(p.y, ey) - L.mxy - L.y0) * L.inv_dy, -1.0f)));
     const int j1 = min(L.ny - 1, (int)floorf(fminf((fmaxf(p.y, ey) + L.mxy - L.y0) * L.inv_dy, fy)));
     if (i0 > i1 || j0 > j1) return h;
-    if ((i1 - i0 + 1) * (j1 - j0 + 1) > kLatticeMaxCells) return walk(geo, make_ray(p, d), step, true, st);
+    if (i1 - i0 >= kLatticeMaxBox || j1 - j0 >= kLatticeMaxBox) return walk(geo, make_ray(p, d), step, true, st);
     Ray ray;  // the triangle test reads the origin and direction only
     ray.o = p;
     ray.d = d;
-    // the first two candidate cells' triangle pairs are tested after the scan, so that
-    // lanes with a candidate in different cells share one test; further ones (rare) at once
-    int a0 = -1, b0 = -1, a1 = -1, b1 = -1;
+    // the scan marks the (cell, heightfield) slots whose z-ranges meet the step's in a
+    // 64-bit mask, bit ((j - j0) * 4 + (i - i0)) * 4 + s; the tests then run over the set
+    // bits, so that a wave runs as many test rounds as its lane with the most candidates
+    uint64_t mask = 0;
     for (int j = j0; j <= j1; ++j) {
         for (int i = i0; i <= i1; ++i) {
             const float4* R = L.cells + (size_t)(j * L.nx + i) * L.ns;
+            const int b0 = ((j - j0) * 4 + (i - i0)) * 4;
 #pragma unroll
             for (int s = 0; s < kLatticeMaxSurfaces; ++s) {
                 if (s < L.ns) {
-                    const float4 c = R[s];
-                    if (c.x <= bz && c.y >= az) {
-                        const int ta = __float_as_int(c.z), tb = __float_as_int(c.w);
-                        if (a0 < 0) {
-                            a0 = ta;
-                            b0 = tb;
-                        } else if (a1 < 0) {
-                            a1 = ta;
-                            b1 = tb;
-                        } else {
-                            lattice_take(geo.tris, ta, ray, h);
-                            if (tb >= 0) lattice_take(geo.tris, tb, ray, h);
-                        }
-                    }
+                    const float2 z = *reinterpret_cast<const float2*>(R + s);
+                    if (z.x <= bz && z.y >= az) mask |= 1ull << (b0 + s);
                 }
             }
         }
     }
-    if (a0 >= 0) lattice_take(geo.tris, a0, ray, h);
-    if (b0 >= 0) lattice_take(geo.tris, b0, ray, h);
-    if (a1 >= 0) lattice_take(geo.tris, a1, ray, h);
-    if (b1 >= 0) lattice_take(geo.tris, b1, ray, h);
+    while (mask != 0) {
+        const int b = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int s = b & 3, i = i0 + ((b >> 2) & 3), j = j0 + (b >> 4);
+        const float4 c = L.cells[(size_t)(j * L.nx + i) * L.ns + s];
+        const int ta = __float_as_int(c.z), tb = __float_as_int(c.w);
+        lattice_take(geo.tris, ta, ray, h);
+        if (tb >= 0) lattice_take(geo.tris, tb, ray, h);
+    }
     h.found = h.tri >= 0 && h.dist <= step;
     return h;
 }
