@@ -1473,12 +1473,15 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     }
 }
 
-// The production variant (no trace, no path tallies, interleaved) is held to 96
-// VGPRs, five waves per SIMD instead of four, at the price of a few spills: the
-// heightfield goes from 2.93e7 to 3.29e7 photons/s; six waves (80 VGPRs) spill
-// enough to fall to 2.54e7.  The other variants keep the registers they need.
+// The production variants (no trace, no path tallies) trade a few spills for more
+// waves per SIMD.  Interleaved: 96 VGPRs, five waves instead of four (heightfield on
+// the walk 2.93e7 -> 3.29e7 photons/s; six waves, 80 VGPRs, spill enough to fall to
+// 2.54e7).  Plain: 72 VGPRs, seven waves (the heightfield under its lattice rule:
+// 5.25e7 at five, 5.49e7 at six, 5.58e7 at seven, 4.64e7 at eight; the slab, whose
+// absorption tile's LDS caps its blocks per CU, is the same at five to seven).  The
+// trace and path-tally variants keep the registers they need.
 #ifndef XMC_PLAIN_WAVES
-#define XMC_PLAIN_WAVES 5
+#define XMC_PLAIN_WAVES 7
 #endif
 template <bool TRACE, bool PATHS, bool INTERLEAVE>
 __global__ void __launch_bounds__(256)
