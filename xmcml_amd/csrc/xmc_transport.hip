@@ -1483,9 +1483,12 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
 #ifndef XMC_PLAIN_WAVES
 #define XMC_PLAIN_WAVES 7
 #endif
+#ifndef XMC_INTERLEAVED_WAVES
+#define XMC_INTERLEAVED_WAVES 5
+#endif
 template <bool TRACE, bool PATHS, bool INTERLEAVE>
 __global__ void __launch_bounds__(256)
-    __attribute__((amdgpu_waves_per_eu((!TRACE && !PATHS) ? (INTERLEAVE ? 5 : XMC_PLAIN_WAVES) : 1)))
+    __attribute__((amdgpu_waves_per_eu((!TRACE && !PATHS) ? (INTERLEAVE ? XMC_INTERLEAVED_WAVES : XMC_PLAIN_WAVES) : 1)))
     transport_kernel(Params P) {
     transport_body<TRACE, false, PATHS, INTERLEAVE>(P, nullptr, nullptr, 0);
 }
