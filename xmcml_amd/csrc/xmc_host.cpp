@@ -29,6 +29,7 @@ hipError_t xmc_query_scratch(int device, int2** ptr, size_t* entries);
 hipError_t xmc_launch_zero(unsigned long long* p, size_t n, hipStream_t s);
 hipError_t xmc_transport_occupancy(int block, size_t lds, int* blocks_per_cu);
 hipError_t xmc_transport_occupancy_plain(int block, size_t lds, int* blocks_per_cu);
+hipError_t xmc_transport_occupancy_merge(int block, size_t lds, int* blocks_per_cu);
 hipError_t xmc_launch_batch(const xmc::Params* p, const xmc::Params* cfgs, const uint64_t* prefix, int ncfg, int grid,
                             int block, size_t lds, hipStream_t s);
 hipError_t xmc_batch_occupancy(int block, size_t lds, int* blocks_per_cu);
@@ -374,9 +375,11 @@ static int configure_launch(int device, int depth, int block, int blocks_per_cu,
 }
 
 // The occupancy query of the transport kernel xmc_launch_transport picks for
-// this scene (interleaved when it has a walk-skipping grid).
+// this scene (interleaved when it has a walk-skipping grid; the merging plain variant
+// when it tallies an absorption grid).
 static auto transport_occupancy_of(const xmc::Params& P) -> hipError_t (*)(int, size_t, int*) {
-    return P.dist.cells != nullptr ? xmc_transport_occupancy : xmc_transport_occupancy_plain;
+    if (P.dist.cells != nullptr) return xmc_transport_occupancy;
+    return P.tally.absorption != nullptr ? xmc_transport_occupancy_merge : xmc_transport_occupancy_plain;
 }
 
 // One trajectory list of traj_cap voxel indices per persistent lane.

@@ -966,7 +966,7 @@ constexpr uint32_t kParkWait = XMC_PARK_WAIT;  // rounds a boundary interaction 
 constexpr int kStartMin = XMC_START_MIN;
 constexpr uint32_t kStartWait = XMC_START_WAIT;
 
-template <bool TRACE, bool BATCHED, bool PATHS, bool INTERLEAVE>
+template <bool TRACE, bool BATCHED, bool PATHS, bool INTERLEAVE, bool MERGE = false>
 __device__ __forceinline__ void transport_body(const Params& P, const Params* __restrict__ cfgs,
                                                const uint64_t* __restrict__ prefix, int ncfg) {
     extern __shared__ int2 lds_stack[];
@@ -990,6 +990,18 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
         for (int i = tid; i < T.count; i += blockDim.x) tile[i] = 0.0;
     __syncthreads();
     double w_abs = 0.0;
+    // MERGE (a single scene with an absorption grid): a lane holds its last voxel's deposits
+    // and adds them to the tile or the grid when it deposits into another voxel (and at the
+    // end), which saves the atomics -- contended ones around the beam -- of consecutive
+    // deposits into one voxel
+    [[maybe_unused]] int pend_idx = -1, pend_tile = -1;
+    [[maybe_unused]] double pend_w = 0.0;
+    auto flush_pending = [&]() {
+        if (pend_idx >= 0) {
+            if (pend_tile >= 0) atomicAdd(tile + pend_tile, pend_w);
+            else unsafeAtomicAdd(P.tally.absorption + pend_idx, pend_w);
+        }
+    };
     // never take P's address: that would move the kernel-argument struct
     // into private memory
     const Params* cfg = cfgs;
@@ -1369,7 +1381,19 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             if (idx >= 0) {
                 float dw = w * Labs;  // ComputeDroppedWeightOfPhoton (:633-642)
                 w -= dw;
-                if (C.tally.absorption) {
+                if constexpr (MERGE) {
+                    if (idx != pend_idx) {
+                        flush_pending();
+                        pend_idx = idx;
+                        pend_tile = -1;
+                        unsigned tx = (unsigned)(bx - T.lo[0]), ty = (unsigned)(by - T.lo[1]),
+                                 tz = (unsigned)(bz - T.lo[2]);
+                        if (tx < (unsigned)T.size[0] && ty < (unsigned)T.size[1] && tz < (unsigned)T.size[2])
+                            pend_tile = ((int)tz * T.size[1] + (int)ty) * T.size[0] + (int)tx;
+                        pend_w = 0.0;
+                    }
+                    pend_w += (double)dw;
+                } else if (C.tally.absorption) {
                     bool in_tile = false;
                     if constexpr (kTile) {
                         unsigned tx = (unsigned)(bx - T.lo[0]), ty = (unsigned)(by - T.lo[1]),
@@ -1444,6 +1468,7 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             P.trace.counts[k] = rec.n;
         }
     }
+    if constexpr (MERGE) flush_pending();
     if (n_photons) atomicAdd(P.tally.stats + 0, (unsigned long long)n_photons);
     if (n_scatter) atomicAdd(P.tally.stats + 1, (unsigned long long)n_scatter);
     if (n_boundary) atomicAdd(P.tally.stats + 2, (unsigned long long)n_boundary);
@@ -1491,6 +1516,13 @@ __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu((!TRACE && !PATHS) ? (INTERLEAVE ? XMC_INTERLEAVED_WAVES : XMC_PLAIN_WAVES) : 1)))
     transport_kernel(Params P) {
     transport_body<TRACE, false, PATHS, INTERLEAVE>(P, nullptr, nullptr, 0);
+}
+
+// The plain variant of a single scene that tallies an absorption grid, with its
+// deposits merged per voxel (MERGE in transport_body).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XMC_PLAIN_WAVES)))
+    transport_merge_kernel(Params P) {
+    transport_body<false, false, false, false, true>(P, nullptr, nullptr, 0);
 }
 
 // Many configurations in one launch: P carries the run-wide fields and the
@@ -1614,6 +1646,8 @@ hipError_t xmc_launch_transport(const xmc::Params* p, int grid, int block, size_
     if (trace && paths) launch_transport<true, true>(p, interleave, grid, block, lds, s);
     else if (trace) launch_transport<true, false>(p, interleave, grid, block, lds, s);
     else if (paths) launch_transport<false, true>(p, interleave, grid, block, lds, s);
+    else if (!interleave && p->tally.absorption != nullptr)
+        hipLaunchKernelGGL(xmc::transport_merge_kernel, dim3(grid), dim3(block), lds, s, *p);
     else launch_transport<false, false>(p, interleave, grid, block, lds, s);
     return hipGetLastError();
 }
@@ -1672,6 +1706,10 @@ hipError_t xmc_transport_occupancy(int block, size_t lds, int* blocks_per_cu) {
 hipError_t xmc_transport_occupancy_plain(int block, size_t lds, int* blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, xmc::transport_kernel<false, false, false>,
                                                         block, lds);
+}
+
+hipError_t xmc_transport_occupancy_merge(int block, size_t lds, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, xmc::transport_merge_kernel, block, lds);
 }
 
 hipError_t xmc_launch_batch(const xmc::Params* p, const xmc::Params* cfgs, const uint64_t* prefix, int ncfg, int grid,
