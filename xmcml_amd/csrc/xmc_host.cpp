@@ -63,11 +63,12 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// dynamic LDS of the transport kernels: short stacks, the block's ledger, its detector sums
-// (three words per detector), then its absorption tile
-size_t transport_lds(int block, int tile_count = 0, int det_lds = 0) {
-    return (size_t)xmc::kStackLds * block * sizeof(int2) + 8 * sizeof(double) + (size_t)det_lds * 3 * sizeof(double) +
-           (size_t)tile_count * sizeof(double);
+// dynamic LDS of the transport kernels: short stacks (unless their rings are in global
+// memory), the block's ledger, its detector sums (three words per detector), then its
+// absorption tile
+size_t transport_lds(int block, int tile_count = 0, int det_lds = 0, bool stack_in_lds = true) {
+    return (stack_in_lds ? (size_t)xmc::kStackLds * block * sizeof(int2) : 0) + 8 * sizeof(double) +
+           (size_t)det_lds * 3 * sizeof(double) + (size_t)tile_count * sizeof(double);
 }
 
 // detectors up to which a block keeps its detector sums in LDS
@@ -76,7 +77,10 @@ constexpr int kDetLds = 64;
 // LDS absorption tile: at most this many voxels per block (24 kB; four 256-thread
 // blocks per CU with their stacks -- measured faster on the slab than 13.5 kB at
 // five blocks: 4.2e7 vs 3.8e7 photons/s, against 2.3e7 without a tile)
-constexpr int kAbsTileVoxels = 3072;
+#ifndef XMC_ABS_TILE_VOXELS
+#define XMC_ABS_TILE_VOXELS 3072
+#endif
+constexpr int kAbsTileVoxels = XMC_ABS_TILE_VOXELS;
 
 template <class T>
 struct DevBuf {
@@ -357,7 +361,11 @@ static int configure_launch(int device, int depth, int block, int blocks_per_cu,
                             DevBuf<int2>& spill, xmc::Params& P) {
     if (block <= 0) block = 256;
     if (block % 64 != 0 || block > 256) return fail(XMC_ERR_ARG, "blockSize must be 64, 128, 192 or 256");
-    size_t lds = transport_lds(block, P.abs_tile.count, P.det_lds);
+    // a single scene under the surface rule walks only the rare steps too long for its
+    // rule: its stacks' rings go to global memory, leaving the LDS to the tallies (more
+    // blocks per CU where an absorption tile fills it); batches keep them in LDS
+    const bool ring_global = kernel_occupancy != xmc_batch_occupancy && P.surface_rule != 0;
+    size_t lds = transport_lds(block, P.abs_tile.count, P.det_lds, !ring_global);
     int per_cu = 0;
     HIP_TRY(kernel_occupancy(block, lds, &per_cu));
     if (per_cu < 1) per_cu = 1;
@@ -369,8 +377,10 @@ static int configure_launch(int device, int depth, int block, int blocks_per_cu,
     int need = depth - xmc::kStackLds;
     P.spill_depth = need > 0 ? need : 0;
     size_t lanes = (size_t)(*blocks_out) * block;
-    HIP_TRY(spill.alloc(P.spill_depth > 0 ? (size_t)P.spill_depth * lanes : 1));
+    const size_t ring = ring_global ? (size_t)xmc::kStackLds * lanes : 0;
+    HIP_TRY(spill.alloc(std::max<size_t>((size_t)P.spill_depth * lanes + ring, 1)));
     P.spill = spill.p;
+    P.ring = ring_global ? spill.p + (size_t)P.spill_depth * lanes : nullptr;
     return XMC_OK;
 }
 
@@ -815,7 +825,7 @@ static int run_impl(xmc_scene* sc, uint64_t seed, uint64_t first, uint64_t n, hi
     P.seed_value = (2 * seed + 1) & xmc::kMcgMask;
     if (tr) P.trace = *tr;
     HIP_TRY(xmc_launch_zero(P.queue, 1, s));
-    size_t lds = transport_lds(sc->block, P.abs_tile.count, P.det_lds);
+    size_t lds = transport_lds(sc->block, P.abs_tile.count, P.det_lds, P.ring == nullptr);
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

@@ -198,6 +198,9 @@ struct Params {
     // short-stack spill (global) for entries beyond kStackLds
     int2* spill;
     int spill_depth;  // entries per lane in the spill
+    // the stacks' rings of kStackLds entries: null = in LDS, else in global memory
+    // ([entry][lane], every lane of the grid); plain single scenes only
+    int2* ring;
     Trace trace;
     Paths paths;
     AbsTile abs_tile;

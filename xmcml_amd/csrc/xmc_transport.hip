@@ -973,9 +973,14 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
     const int tid = threadIdx.x;
     const int lane = __lane_id();
     const int nlanes = gridDim.x * blockDim.x;
-    Stack st{lds_stack, (int)blockDim.x, tid, P.spill, nlanes, (int)(blockIdx.x * blockDim.x + tid),
-             P.spill_depth, 0, 0};
-    double* ledger = reinterpret_cast<double*>(lds_stack + kStackLds * blockDim.x);
+    const int glane = (int)(blockIdx.x * blockDim.x + tid);
+    // the stacks' rings: in LDS, or (a plain single scene under the surface rule, which
+    // walks only its rare long steps) in global memory, leaving the LDS to the tallies
+    bool ring_global = false;
+    if constexpr (!INTERLEAVE && !BATCHED) ring_global = P.ring != nullptr;
+    Stack st{ring_global ? P.ring : lds_stack, ring_global ? nlanes : (int)blockDim.x, ring_global ? glane : tid,
+             P.spill, nlanes, glane, P.spill_depth, 0, 0};
+    double* ledger = reinterpret_cast<double*>(lds_stack + (ring_global ? 0 : kStackLds * (int)blockDim.x));
     if (tid < kLedger) ledger[tid] = 0.0;
     // the block's detector sums (weight, target weight, count per detector) follow the
     // ledger in LDS, then its absorption tile

@@ -273,6 +273,17 @@ def small_scene(kind: str, photons: int = 2000) -> SceneSpec:
         s.area_partitions = (40, 40, 20)
         s.name = "heightfield-small"
         return s
+    if kind == "heightfield-glass":
+        # the small heightfield with a glass top layer: every step in it is unlimited, so
+        # the scene's surface rule hands it to the BVH walk
+        s = heightfield_scene(photons, cells=32, time_bins=64)
+        s.layers = _slab_layers([(1.5, 0.0, 0.0, 0.0), (1.4, 0.1, 10.0, 0.85), (1.37, 0.02, 5.0, 0.9)])
+        # tilted off the vertical: straight down from the lattice vertex the beam starts on, it
+        # would meet the conformal surface below at its vertex too, where six triangles tie
+        s.start_direction = (0.01, 0.02, math.sqrt(1.0 - 0.0005))
+        s.area_partitions = (40, 40, 20)
+        s.name = "heightfield-glass"
+        return s
     raise ValueError(kind)
 
 
