@@ -1533,8 +1533,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XMC_PL
 // Many configurations in one launch: P carries the run-wide fields and the
 // photon range [0, prefix[ncfg]); cfgs[c] the geometry, optics, detectors and
 // tallies of configuration c.
-__global__ void __launch_bounds__(256) batch_kernel(Params P, const Params* __restrict__ cfgs,
-                                                    const uint64_t* __restrict__ prefix, int ncfg) {
+// six waves per SIMD for the batch kernel (four kernel-time measurements on the 1024
+// layered configurations at a quarter of their budgets: 4345 ms at its natural five,
+// 4246 ms at six, 4803 ms at seven)
+#ifndef XMC_BATCH_WAVES
+#define XMC_BATCH_WAVES 6
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(XMC_BATCH_WAVES)))
+    batch_kernel(Params P, const Params* __restrict__ cfgs, const uint64_t* __restrict__ prefix, int ncfg) {
     transport_body<false, true, false, false>(P, cfgs, prefix, ncfg);
 }
 
