@@ -74,9 +74,10 @@ size_t transport_lds(int block, int tile_count = 0, int det_lds = 0, bool stack_
 // detectors up to which a block keeps its detector sums in LDS
 constexpr int kDetLds = 64;
 
-// LDS absorption tile: at most this many voxels per block (24 kB; four 256-thread
-// blocks per CU with their stacks -- measured faster on the slab than 13.5 kB at
-// five blocks: 4.2e7 vs 3.8e7 photons/s, against 2.3e7 without a tile)
+// LDS absorption tile: at most this many voxels per block (24 kB; measured on the slab
+// faster than 13.5 kB at five blocks, 4.2e7 vs 3.8e7 photons/s, against 2.3e7 without a
+// tile -- when the stacks' rings were in LDS; with them in global memory and merged
+// deposits, six blocks per CU, 2,752 and 2,048 voxels measured the same)
 #ifndef XMC_ABS_TILE_VOXELS
 #define XMC_ABS_TILE_VOXELS 3072
 #endif
