@@ -1159,7 +1159,10 @@ __device__ __forceinline__ void transport_body(const Params& P, const Params* __
             h.found = false;  // until a walk finds something within the step
             h.dist = kInf;
             h.tri = -1;
-            int e = (!Lglass && C.dist.cells != nullptr) ? walk_start(C.dist, pos, dir, step) : -2;
+            // (a single scene's plain variant runs only scenes without a grid)
+            int e = -2;
+            if constexpr (INTERLEAVE || BATCHED)
+                e = (!Lglass && C.dist.cells != nullptr) ? walk_start(C.dist, pos, dir, step) : -2;
             if (e == -1) {
                 n_culled++;
                 phase = kResolve;
