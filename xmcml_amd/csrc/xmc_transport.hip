@@ -543,12 +543,26 @@ __device__ __forceinline__ void lattice_take(const GpuTri* tris, int k, const Ra
 }
 
 __device__ __forceinline__ Hit rule_hit(const Params& C, const Geo& geo, V3 p, V3 d, float step, Stack& st) {
-    Hit h = planar_hit(C.planar, p, d, step);
     const Lattice& L = C.lattice;
-    if (L.ns == 0) return h;
+    if (L.ns == 0) return planar_hit(C.planar, p, d, step);
     if (!(step < kInf)) return walk(geo, make_ray(p, d), step, true, st);
     const float ez = __builtin_fmaf(step, d.z, p.z);
     const float az = fminf(p.z, ez) - L.mz, bz = fmaxf(p.z, ez) + L.mz;
+    // a rectangle whose plane lies outside the step's z-range (widened further by 1e-5 of
+    // its |z|) cannot be hit within the step: planar_hit runs only for steps that reach one
+    bool reach = false;
+    for (int s = 0; s < C.planar.n; ++s) {
+        const float z = C.planar.z[s], m = 1.0e-5f * fabsf(z);
+        reach = reach || (z >= az - m && z <= bz + m);
+    }
+    Hit h;
+    if (reach) {
+        h = planar_hit(C.planar, p, d, step);
+    } else {
+        h.found = false;
+        h.dist = kInf;
+        h.tri = -1;
+    }
     if (bz < L.zlo || az > L.zhi) return h;
     const float ex = __builtin_fmaf(step, d.x, p.x), ey = __builtin_fmaf(step, d.y, p.y);
     const float fx = (float)L.nx, fy = (float)L.ny;
