@@ -8,7 +8,11 @@ over 40 x 40 mm plus a flat bottom (98,306 triangles in the BVH), the
 3-layer tissue of the slab configuration, pencil beam, roulette at 1e-4 with
 survival 0.1, 20 ring detectors of 0.5 mm, a 256-bin time-of-flight
 histogram over 0-2 ns per ring.  Geometry, optics and tallies are resident in
-HBM before the timed region; photon streams continue from step to step.
+HBM before the timed region; photon streams continue from step to step.  The
+three heightfields share one uniform lattice and the bottom is a rectangle, so
+each step's nearest surface comes from the engine's surface rule (the
+triangles of the lattice cells the step covers, tested as the BVH leaves test
+them); `--no-lattice-rule` times the BVH-walk path of the same workload.
 
 Output: one JSON line.  `value` is photons/s over the timed steps;
 `roofline` prices the transport kernel's algorithmic FP32 work (80 FLOP per
