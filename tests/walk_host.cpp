@@ -111,10 +111,7 @@ extern "C" int rule_query(const xmc_surface* surf, int ns, const double* o, cons
             const int k = P.planar.n++;
             rect_of[s] = k;
             P.planar.z[k] = r.z;
-            P.planar.x0[k] = r.x0;
-            P.planar.x1[k] = r.x1;
-            P.planar.y0[k] = r.y0;
-            P.planar.y1[k] = r.y1;
+            P.planar.rect[k] = {r.x0, r.x1, r.y0, r.y1};
             P.planar.tri[k] = -1;
         } else {
             heights.push_back(s);

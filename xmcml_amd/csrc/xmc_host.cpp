@@ -315,10 +315,7 @@ static int surface_rule(const xmc_input* in, const xmc::GpuBvh& gb, const std::v
             if (pl.n == xmc::kPlanarMax) return XMC_OK;
             rect_of[s] = pl.n;
             pl.z[pl.n] = r.z;
-            pl.x0[pl.n] = r.x0;
-            pl.x1[pl.n] = r.x1;
-            pl.y0[pl.n] = r.y0;
-            pl.y1[pl.n] = r.y1;
+            pl.rect[pl.n] = make_float4(r.x0, r.x1, r.y0, r.y1);
             pl.tri[pl.n] = -1;
             pl.n++;
         } else {

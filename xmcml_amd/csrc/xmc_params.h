@@ -123,7 +123,8 @@ struct DistGrid {
 constexpr int kPlanarMax = 16;
 struct Planar {
     int n;
-    float z[kPlanarMax], x0[kPlanarMax], x1[kPlanarMax], y0[kPlanarMax], y1[kPlanarMax];
+    float4 rect[kPlanarMax];  // x0, x1, y0, y1 (one load per rectangle where a batch lane reads them)
+    float z[kPlanarMax];
     int tri[kPlanarMax];  // the surface's first triangle in BVH leaf order (its normal and surface id)
 };
 

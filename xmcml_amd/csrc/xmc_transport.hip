@@ -513,7 +513,8 @@ __device__ __forceinline__ Hit planar_hit(const Planar& S, V3 p, V3 d, float ste
             const float t = (S.z[s] - p.z) * iz;
             if (t >= 0.0f && t < h.dist) {
                 const float x = p.x + t * d.x, y = p.y + t * d.y;
-                if (x >= S.x0[s] && x <= S.x1[s] && y >= S.y0[s] && y <= S.y1[s]) {
+                const float4 r = S.rect[s];
+                if (x >= r.x && x <= r.y && y >= r.z && y <= r.w) {
                     h.dist = t;
                     h.tri = S.tri[s];
                 }
