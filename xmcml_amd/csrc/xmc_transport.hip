@@ -527,7 +527,9 @@ __device__ __forceinline__ Hit planar_hit(const Planar& S, V3 p, V3 d, float ste
 
 // Float rule of the scenes whose every surface is a rectangle or a lattice
 // heightfield (surface_rule = 2): the rectangles' nearest hit as planar_hit
-// finds it; then every triangle of the heightfields whose cell lies in the
+// finds it (for the steps whose z-range reaches one of their planes; for the
+// others there is none within the step, and the found hit is the same either
+// way); then every triangle of the heightfields whose cell lies in the
 // step's xy bounding box widened by L.mxy and whose cell z-range (per
 // heightfield) meets the step's z-range widened by L.mz, tested as a BVH leaf
 // tests it: a triangle nearer than the hit, or as near with a lower leaf
