@@ -21,7 +21,11 @@ FP32 peak, with the kernel time from HIP events around each launch;
 `cpu_baseline` is the double-precision oracle (a restatement of the
 reference's OpenMP LaunchOMP path) on a bounded sample of the same workload,
 with the reference's own LaunchOMP rate derived from it through the same-core
-ratio in oracle/reference_ratio.json.
+ratio in oracle/reference_ratio.json.  After the timed steps (and the
+surrogate's part), the default run also measures the other configurations --
+one 1e8-photon launch of the slab with its absorption grid, the 1024 batched
+configurations in their graph, one 1e8-photon launch of the 1.64M-triangle
+head -- into `other_workloads` (`--no-other-workloads` skips them).
 """
 from __future__ import annotations
 
@@ -63,6 +67,11 @@ WORKLOADS = {
 def bench_batch(args) -> int:
     """The batched configurations: every budget in one persistent launch (a
     captured HIP graph); value = photons/s of the launch, plus configs/s."""
+    print(json.dumps(measure_batch(args, args.cpu_seconds)), flush=True)
+    return 0
+
+
+def measure_batch(args, cpu_seconds: float) -> dict:
     from xmcml_amd import engine, scenes
     import oracle_lib as ol
     from xmcml_amd.abi import InputHandle, OutputHandle
@@ -97,11 +106,11 @@ def bench_batch(args) -> int:
                          "candidates_ms_at_5pct_budgets": b.tuning.get("timings")},
         "scatter_events_per_photon": st["scatterEvents"] / total,
     }
-    if args.cpu_seconds > 0:
+    if cpu_seconds > 0:
         # the double oracle on a sample: 32 configurations, 20,000 photons each, all host threads
         threads = host_threads()
         photons = seconds = 0.0
-        t_end = time.perf_counter() + args.cpu_seconds
+        t_end = time.perf_counter() + cpu_seconds
         k = 0
         while time.perf_counter() < t_end and k < 256:
             sp = specs[(k * 37) % len(specs)]
@@ -114,8 +123,35 @@ def bench_batch(args) -> int:
                                 "nproc": os.cpu_count(), "cpu_model": cpu_model(), "kind": "port",
                                 "sample": f"{k} configurations of the batch x 20,000 photons, double precision"}
         line["speedup_vs_cpu"] = value / line["cpu_baseline"]["value"]
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
+
+
+def measure_scene(name: str, photons: int, seed: int) -> dict:
+    """One untimed launch of a tenth of `photons`, then one launch of `photons`
+    of a single-scene workload: photons/s, kernel time, FP32 roofline."""
+    from xmcml_amd import engine, scenes
+    make = {"head": scenes.head_scene, "slab": scenes.slab_scene}[name]
+    spec = make(photons)
+    t_build = time.perf_counter()
+    sc = engine.Scene(spec, absorption=name == "slab")
+    t_build = time.perf_counter() - t_build
+    sc.run(seed, 0, photons // 10)
+    sc.sync()
+    sc.collect()
+    sc.stats()
+    t0 = time.perf_counter()
+    sc.run(seed, photons, photons)
+    sc.sync()
+    elapsed = time.perf_counter() - t0
+    st = sc.stats()
+    sc.collect()
+    sc.close()
+    flop = FLOP_SCATTER * st["scatterEvents"] + FLOP_BOUNDARY * st["boundaryEvents"]
+    achieved = flop / (st["kernelMs"] / 1e3) / 1e12
+    return {"workload": WORKLOADS[name], "value": photons / elapsed, "unit": "photons/s", "photons": photons,
+            "ms_per_step": elapsed * 1e3, "kernel_ms": st["kernelMs"], "scene_setup_s": t_build,
+            "roofline_achieved_tflops": achieved, "roofline_frac": achieved / PEAK_FP32_TFLOPS,
+            "scatter_events_per_photon": st["scatterEvents"] / photons}
 
 
 def host_threads() -> int:
@@ -333,6 +369,9 @@ def main(argv=None) -> int:
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--no-slab-grid", action="store_true",
                     help="build no separating slabs near the mesh (A/B of the walk skipping; same results)")
+    ap.add_argument("--no-other-workloads", action="store_true",
+                    help="skip the other configurations' lines (slab, batch, head), measured after the "
+                         "default workload's timed steps")
     ap.add_argument("--no-lattice-rule", action="store_true",
                     help="walk the BVH on the heightfield and slab workloads instead of testing the triangles of "
                          "the lattice cells each step covers (the interleaved walk kernel; same hits)")
@@ -461,6 +500,21 @@ def main(argv=None) -> int:
                                            args.mc_photons, args.answer_queries, args.fallback_photons)
     else:
         line["surrogate"] = None
+    if args.workload == "heightfield" and not args.no_other_workloads:
+        # the other configurations, each after the timed region above: one launch of the slab's
+        # 1e8 photons with its absorption grid, the 1024 batched configurations in their graph,
+        # and 1e8 photons of the 1.64M-triangle head
+        other = {}
+        print("bench: other workloads: slab", file=sys.stderr, flush=True)
+        other["slab"] = measure_scene("slab", 100_000_000, args.seed)
+        print("bench: other workloads: batch", file=sys.stderr, flush=True)
+        b = measure_batch(args, 0.0)
+        other["batch"] = {k: b[k] for k in ("value", "unit", "ms_per_step", "configs_per_s", "kernel_ms",
+                                            "launch_shape", "scatter_events_per_photon")}
+        other["batch"]["workload"] = WORKLOADS["batch"]
+        print("bench: other workloads: head", file=sys.stderr, flush=True)
+        other["head"] = measure_scene("head", 100_000_000, args.seed)
+        line["other_workloads"] = other
     print(json.dumps(line), flush=True)
     return 0
 
